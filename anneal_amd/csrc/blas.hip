@@ -1,0 +1,302 @@
+// Vector kernels with deterministic FP64 reductions (fixed grid, fixed tree).
+#include "common.hpp"
+#include "blas.hpp"
+
+namespace anneal {
+
+constexpr int kRedThreads = 256;
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(const TA* __restrict__ a, const TB* __restrict__ b,
+                                                             int64_t n, int64_t ld, double* __restrict__ part) {
+  const int r = blockIdx.y;
+  const TA* pa = a + (int64_t)3 * r * ld;
+  const TB* pb = b + (int64_t)3 * r * ld;
+  double s = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const TA* qa = pa + c * ld;
+    const TB* qb = pb + c * ld;
+    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedThreads)
+      s += (double)qa[i] * (double)qb[i];
+  }
+  __shared__ double sh[kRedThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
+    part[(int64_t)r * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Sum nblk partials per rhs in a fixed order; out[r] = sum (or out[r] = scale*sum).
+__global__ __launch_bounds__(kRedThreads) void k_finalize(const double* __restrict__ part, int nblk,
+                                                          double* __restrict__ out) {
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedThreads) s += part[(int64_t)r * nblk + i];
+  __shared__ double sh[kRedThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
+    out[r] = t;
+  }
+}
+
+template <typename TA, typename TB>
+void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld, part);
+  ANNEAL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out);
+  ANNEAL_HIP(hipGetLastError());
+}
+template void dot<float, float>(const float*, const float*, int64_t, int64_t, int, double*, double*, hipStream_t);
+template void dot<double, double>(const double*, const double*, int64_t, int64_t, int, double*, double*, hipStream_t);
+
+void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, nblk, out);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+// x = M * omega * dinv * b  (first Jacobi sweep from a zero guess)
+__global__ void k_jacobi0(const float* __restrict__ b, float* __restrict__ x, const float* __restrict__ dinv,
+                          const uint8_t* __restrict__ mask, float omega, int64_t n, int64_t ld) {
+  const int r = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = omega * dinv[i];
+  const uint8_t mk = mask[i];
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)(3 * r + c) * ld + i;
+    x[o] = ((mk >> c) & 1) ? d * b[o] : 0.f;
+  }
+}
+void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
+             int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi0, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, b, x, dinv, mask, omega, n,
+                     ld);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+// Dense coarse solve x = Ainv b with Ainv in FP64 (n x n, row major).
+__global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __restrict__ b, float* __restrict__ x,
+                              int nnode, int64_t ld) {
+  const int n = 3 * nnode;
+  const int r = blockIdx.y;
+  const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* a = Ainv + (int64_t)row * n;
+  const float* br = b + (int64_t)3 * r * ld;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const int node = j / 3, c = j - 3 * node;
+    s += a[j] * (double)br[c * ld + node];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) {
+    const int node = row / 3, c = row - 3 * node;
+    x[(int64_t)(3 * r + c) * ld + node] = (float)s;
+  }
+}
+void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s) {
+  const int n = 3 * nnode;
+  hipLaunchKernelGGL(k_dense_solve, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, Ainv, b, x, nnode, ld);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+// Dense assembly of the coarsest operator: entry (i, j) gathers the elements
+// shared by the two nodes, fixed order.  Non-free dofs get identity rows.
+__global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __restrict__ mask,
+                                 const double* __restrict__ ke, double* __restrict__ A, int nx, int ny, int nz) {
+  const int nnode = (nx + 1) * (ny + 1) * (nz + 1);
+  const int n = 3 * nnode;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = t / n, j = t % n;
+  const int ni = i / 3, ci = i % 3, nj = j / 3, cj = j % 3;
+  const bool fi = (mask[ni] >> ci) & 1, fj = (mask[nj] >> cj) & 1;
+  double v = 0.0;
+  if (!fi || !fj) {
+    v = (i == j) ? 1.0 : 0.0;
+  } else {
+    const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
+    const int ix = ni % sx, iy = (ni / sx) % (ny + 1), iz = ni / sxy;
+    const int jx = nj % sx, jy = (nj / sx) % (ny + 1), jz = nj / sxy;
+    if (abs(ix - jx) <= 1 && abs(iy - jy) <= 1 && abs(iz - jz) <= 1) {
+      for (int ez = max(iz, jz) - 1; ez <= min(iz, jz); ++ez)
+        for (int ey = max(iy, jy) - 1; ey <= min(iy, jy); ++ey)
+          for (int ex = max(ix, jx) - 1; ex <= min(ix, jx); ++ex) {
+            if (ex < 0 || ex >= nx || ey < 0 || ey >= ny || ez < 0 || ez >= nz) continue;
+            const double e = Ee[ex + nx * (ey + ny * ez)];
+            if (e == 0.0) continue;
+            const int qi = (ix - ex) + 2 * (iy - ey) + 4 * (iz - ez);
+            const int qj = (jx - ex) + 2 * (jy - ey) + 4 * (jz - ez);
+            v += e * ke[(3 * qi + ci) * 24 + 3 * qj + cj];
+          }
+    }
+    if (i == j && v <= 0.0) v = 1.0;  // dof with no stiffness: keep the matrix regular
+  }
+  A[t] = v;
+}
+
+// One Gauss-Jordan step of an in-place inversion of an SPD matrix (no pivoting),
+// out-of-place between two buffers so every entry reads the pre-step state.
+__global__ void k_gj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = t / n, j = t % n;
+  const double piv = A[(int64_t)k * n + k];
+  double v;
+  if (i == k && j == k) v = 1.0 / piv;
+  else if (i == k) v = A[t] / piv;
+  else if (j == k) v = -A[(int64_t)i * n + k] / piv;
+  else v = A[t] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] / piv;
+  B[t] = v;
+}
+
+void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
+                           int ny, int nz, hipStream_t s) {
+  const int n = 3 * (nx + 1) * (ny + 1) * (nz + 1);
+  const int64_t nn = (int64_t)n * n;
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  hipLaunchKernelGGL(k_dense_assemble, dim3(g), dim3(256), 0, s, Ee, mask, ke, A, nx, ny, nz);
+  ANNEAL_HIP(hipGetLastError());
+  double* src = A;
+  double* dst = B;
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, s, src, dst, n, k);
+    std::swap(src, dst);
+  }
+  ANNEAL_HIP(hipGetLastError());
+  if (src != A) ANNEAL_HIP(hipMemcpyAsync(A, src, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// PCG pieces (FP32 vectors, FP64 scalars on the device).
+//   st layout per rhs r: st[r*kSt + ...]
+__global__ void k_pcg_alpha(double* st, int nrhs) {
+  const int r = threadIdx.x;
+  if (r >= nrhs) return;
+  double* q = st + r * kSt;
+  const double pq = q[kPQ];
+  q[kAlpha] = (q[kDone] == 0.0 && pq > 0.0 && isfinite(pq)) ? q[kRZ] / pq : 0.0;
+}
+__global__ void k_pcg_beta(double* st, int nrhs) {
+  const int r = threadIdx.x;
+  if (r >= nrhs) return;
+  double* q = st + r * kSt;
+  const double rzn = q[kRZnew];
+  q[kBeta] = (q[kDone] == 0.0 && q[kRZ] > 0.0) ? rzn / q[kRZ] : 0.0;
+  if (q[kDone] == 0.0) q[kRZ] = rzn;
+}
+
+// x += alpha p ; r -= alpha q ; partial of r.r
+__global__ __launch_bounds__(kRedThreads) void k_pcg_update(float* __restrict__ x, float* __restrict__ rv,
+                                                             const float* __restrict__ p, const float* __restrict__ q,
+                                                             const double* __restrict__ st, int64_t n, int64_t ld,
+                                                             double* __restrict__ part) {
+  const int r = blockIdx.y;
+  const float al = (float)st[r * kSt + kAlpha];
+  double s = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)(3 * r + c) * ld;
+    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedThreads) {
+      x[o + i] += al * p[o + i];
+      const float rn = rv[o + i] - al * q[o + i];
+      rv[o + i] = rn;
+      s += (double)rn * (double)rn;
+    }
+  }
+  __shared__ double sh[kRedThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
+    part[(int64_t)r * gridDim.x + blockIdx.x] = t;
+  }
+}
+void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
+                double* part, double* out_rr, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, x, r, p, q, st, n, ld, part);
+  ANNEAL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out_rr);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+// p = z + beta p
+__global__ void k_pcg_dir(float* __restrict__ p, const float* __restrict__ z, const double* __restrict__ st, int64_t n,
+                          int64_t ld, int first) {
+  const int r = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float be = first ? 0.f : (float)st[r * kSt + kBeta];
+  const bool done = st[r * kSt + kDone] != 0.0;
+  if (done && !first) return;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)(3 * r + c) * ld + i;
+    p[o] = z[o] + be * p[o];
+  }
+}
+void pcg_alpha(double* st, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, s, st, nrhs);
+  ANNEAL_HIP(hipGetLastError());
+}
+void pcg_beta(double* st, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, st, nrhs);
+  ANNEAL_HIP(hipGetLastError());
+}
+void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_dir, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, p, z, st, n, ld, first);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+// Refinement glue: r32 = scale_r * r64 ; x64 += inv_scale_r * x32
+__global__ void k_to_f32_scaled(const double* __restrict__ a, float* __restrict__ b, const double* __restrict__ sc,
+                                int64_t n, int64_t ld) {
+  const int r = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = sc[r];
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)(3 * r + c) * ld + i;
+    b[o] = (float)(a[o] * s);
+  }
+}
+__global__ void k_axpy_f32_to_f64(const float* __restrict__ x, double* __restrict__ y, const double* __restrict__ sc,
+                                  int64_t n, int64_t ld) {
+  const int r = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = sc[r];
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)(3 * r + c) * ld + i;
+    y[o] += s * (double)x[o];
+  }
+}
+void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_f32_scaled, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, a, b, sc, n, ld);
+  ANNEAL_HIP(hipGetLastError());
+}
+void axpy_f32_to_f64(const float* x, double* y, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_f32_to_f64, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, x, y, sc, n, ld);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+__global__ void k_cast_f64_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+void cast_f64_f32(const double* a, float* b, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_cast_f64_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+}  // namespace anneal

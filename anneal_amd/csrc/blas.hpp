@@ -1,0 +1,49 @@
+// Declarations shared by the solver, stencil and vector kernels.
+#pragma once
+#include "common.hpp"
+
+namespace anneal {
+
+constexpr int kDotBlocks = 512;  // fixed reduction grid: results do not depend on timing
+
+// Per right-hand-side PCG scalars on the device (FP64).
+constexpr int kSt = 8;
+constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kRR = 3, kAlpha = 4, kBeta = 5, kDone = 6;
+
+template <typename T>
+struct KeCoef;
+template <typename T>
+struct ApplyArgs;
+
+template <typename T>
+void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s);
+int apply_blocks(int nx, int ny, int nz);
+
+void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
+                     hipStream_t s);
+void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, hipStream_t s);
+template <typename T>
+void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
+                     hipStream_t s);
+template <typename T>
+void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
+                        hipStream_t s);
+
+template <typename TA, typename TB>
+void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s);
+void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s);
+void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
+             int nrhs, hipStream_t s);
+void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s);
+void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
+                           int ny, int nz, hipStream_t s);
+void pcg_alpha(double* st, int nrhs, hipStream_t s);
+void pcg_beta(double* st, int nrhs, hipStream_t s);
+void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
+                double* part, double* out_rr, hipStream_t s);
+void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
+void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
+void axpy_f32_to_f64(const float* x, double* y, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
+void cast_f64_f32(const double* a, float* b, int64_t n, hipStream_t s);
+
+}  // namespace anneal

@@ -1,0 +1,269 @@
+// extern "C" boundary of libanneal_hip.so (declared in include/anneal_hip.h).
+#include "../../include/anneal_hip.h"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "fem_ctx.hpp"
+#include "topopt.hpp"
+
+namespace anneal {
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+const char* get_error() { return g_err.c_str(); }
+}  // namespace anneal
+
+using namespace anneal;
+
+namespace {
+template <typename F>
+int guarded(F&& f) {
+  try {
+    anneal::set_error("");
+    return f();
+  } catch (const std::invalid_argument& e) {
+    set_error(e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return 2;
+  } catch (...) {
+    set_error("unknown error");
+    return 2;
+  }
+}
+struct FemHandle {
+  FemContext* fem;
+  uint8_t* flags;  // device copy of the active set (bit0)
+};
+inline FemHandle* fem_of(void* p) {
+  if (!p) throw std::invalid_argument("null FEM context");
+  return static_cast<FemHandle*>(p);
+}
+inline FilterContext* filt_of(void* p) {
+  if (!p) throw std::invalid_argument("null filter context");
+  return static_cast<FilterContext*>(p);
+}
+}  // namespace
+
+extern "C" {
+
+const char* anneal_last_error(void) { return get_error(); }
+int anneal_abi_version(void) { return 1; }
+
+int anneal_ke_host(double nu, double h, double* ke_host, double* coef_host) {
+  return guarded([&] {
+    ANNEAL_CHECK(ke_host && coef_host, "null argument");
+    ANNEAL_CHECK(nu > -1.0 && nu < 0.5 && h > 0.0, "need -1 < nu < 0.5 and h > 0");
+    hex8_stiffness(nu, h, ke_host);
+    const KeCoef<double> k = walsh_coefficients(ke_host);
+    const double c[8] = {k.nAmB, k.nB, k.g, k.pc, k.pd, k.tAmB, k.tB, k.w};
+    for (int i = 0; i < 8; ++i) coef_host[i] = c[i];
+    return 0;
+  });
+}
+
+int anneal_fem_create(void** ctx, int nx, int ny, int nz, double h, double nu, const uint8_t* active_host,
+                      const uint8_t* fixed_host, int nrhs_max, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(ctx && active_host && fixed_host, "null argument");
+    ANNEAL_CHECK(h > 0.0, "voxel size must be positive");
+    const int64_t ne = (int64_t)nx * ny * nz;
+    bool any = false;
+    for (int64_t e = 0; e < ne && !any; ++e) any = active_host[e] != 0;
+    if (!any) throw std::invalid_argument("No active voxels: the design space is empty.");
+    auto* fem = new FemContext(nx, ny, nz, h, nu, active_host, fixed_host, nrhs_max, (hipStream_t)stream, 200);
+    uint8_t* flags = nullptr;
+    try {
+      ANNEAL_HIP(hipMalloc(&flags, ne));
+      std::string tmp(ne, '\0');
+      for (int64_t e = 0; e < ne; ++e) tmp[e] = active_host[e] ? kActive : 0;
+      ANNEAL_HIP(hipMemcpy(flags, tmp.data(), ne, hipMemcpyHostToDevice));
+    } catch (...) {
+      delete fem;
+      throw;
+    }
+    *ctx = new FemHandle{fem, flags};
+    return 0;
+  });
+}
+
+int anneal_fem_destroy(void* ctx) {
+  return guarded([&] {
+    if (!ctx) return 0;
+    FemHandle* h = static_cast<FemHandle*>(ctx);
+    delete h->fem;
+    (void)hipFree(h->flags);
+    delete h;
+    return 0;
+  });
+}
+
+int anneal_fem_info(void* ctx, int64_t* ldn, int* levels, uint8_t* mask_host) {
+  return guarded([&] {
+    FemHandle* h = fem_of(ctx);
+    if (ldn) *ldn = h->fem->ldn();
+    if (levels) *levels = h->fem->levels();
+    if (mask_host) ANNEAL_HIP(hipMemcpy(mask_host, h->fem->mask(), h->fem->grid().nnode(), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int anneal_fem_set_Ee(void* ctx, const double* Ee_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(Ee_dev, "null Ee");
+    fem_of(ctx)->fem->set_Ee(Ee_dev);
+    return 0;
+  });
+}
+
+int anneal_fem_apply_f32(void* ctx, const float* u_dev, float* y_dev, int nrhs) {
+  return guarded([&] {
+    ANNEAL_CHECK(u_dev && y_dev && u_dev != (const float*)y_dev, "apply needs distinct u and y");
+    ANNEAL_CHECK(nrhs >= 1 && nrhs <= 64, "nrhs out of range");
+    fem_of(ctx)->fem->apply_f32(u_dev, y_dev, nrhs);
+    return 0;
+  });
+}
+
+int anneal_fem_apply_f64(void* ctx, const double* u_dev, double* y_dev, int nrhs) {
+  return guarded([&] {
+    ANNEAL_CHECK(u_dev && y_dev && u_dev != (const double*)y_dev, "apply needs distinct u and y");
+    ANNEAL_CHECK(nrhs >= 1 && nrhs <= 64, "nrhs out of range");
+    fem_of(ctx)->fem->apply_f64(u_dev, y_dev, nrhs);
+    return 0;
+  });
+}
+
+int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, double rtol, int maxiter, int use_x0,
+                     int* iters_host) {
+  return guarded([&] {
+    ANNEAL_CHECK(F_dev && U_dev && iters_host, "null argument");
+    ANNEAL_CHECK(rtol > 0.0 && maxiter >= 1, "rtol must be positive and maxiter >= 1");
+    const int st = fem_of(ctx)->fem->solve(F_dev, U_dev, nrhs, rtol, maxiter, use_x0 != 0, iters_host);
+    if (st == 1) {
+      set_error("not converged within maxiter");
+      return 3;
+    }
+    if (st == 2) {
+      set_error("non-finite residual");
+      return 4;
+    }
+    return 0;
+  });
+}
+
+int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs) {
+  return guarded([&] {
+    ANNEAL_CHECK(r_dev && z_dev && r_dev != (const float*)z_dev, "vcycle needs distinct r and z");
+    fem_of(ctx)->fem->vcycle(0, r_dev, z_dev, nrhs);
+    return 0;
+  });
+}
+
+int anneal_fem_element_energy(void* ctx, const double* U_dev, int nrhs, double* ce_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(U_dev && ce_dev, "null argument");
+    FemHandle* h = fem_of(ctx);
+    element_energy(U_dev, nrhs, h->fem->ldn(), h->fem->grid(), h->fem->coef64(), h->flags, ce_dev,
+                   h->fem->stream());
+    return 0;
+  });
+}
+
+int anneal_fem_sensitivity(void* ctx, const double* U_dev, int nrhs, const uint8_t* flags_dev, const double* xphys_dev,
+                           const double* dxp_dev, const double* phi_dev, const double* coef_dev, double p, double emin,
+                           double E0, double* dcdx_dev, double* dvdx_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(U_dev && flags_dev && xphys_dev && dxp_dev && phi_dev && coef_dev && dcdx_dev && dvdx_dev,
+                 "null argument");
+    FemHandle* h = fem_of(ctx);
+    sensitivity(U_dev, nrhs, h->fem->ldn(), h->fem->grid(), h->fem->coef64(), flags_dev, xphys_dev, dxp_dev, phi_dev,
+                coef_dev, p, emin, E0, dcdx_dev, dvdx_dev, h->fem->stream());
+    return 0;
+  });
+}
+
+int anneal_filter_create(void** ctx, int nx, int ny, int nz, double r_vox, const uint8_t* flags_host, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(ctx && flags_host, "null argument");
+    *ctx = new FilterContext(nx, ny, nz, r_vox, flags_host, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_filter_destroy(void* ctx) {
+  return guarded([&] {
+    delete static_cast<FilterContext*>(ctx);
+    return 0;
+  });
+}
+
+int anneal_filter_forward(void* ctx, const double* x_dev, double* y_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && y_dev && x_dev != (const double*)y_dev, "filter needs distinct input and output");
+    filt_of(ctx)->forward(x_dev, y_dev);
+    return 0;
+  });
+}
+
+int anneal_filter_backward(void* ctx, const double* d_dev, double* y_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(d_dev && y_dev && d_dev != (const double*)y_dev, "filter needs distinct input and output");
+    filt_of(ctx)->backward(d_dev, y_dev);
+    return 0;
+  });
+}
+
+int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const double* phi_dev, double* xphys_dev,
+                           double* dxp_dev, double* vol_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && phi_dev && xphys_dev && dxp_dev, "null argument");
+    filt_of(ctx)->physical(x_dev, beta, phi_dev, xphys_dev, dxp_dev, vol_dev);
+    return 0;
+  });
+}
+
+int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                     const double* phi_dev, double beta, double move, double lo, double hi, double vtarget,
+                     double* xnew_dev, int* steps_host) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && dcx_dev && dvx_dev && phi_dev && xnew_dev, "null argument");
+    ANNEAL_CHECK(lo > 0.0 && hi >= lo, "bisection bracket must satisfy 0 < lo <= hi");
+    const int s = filt_of(ctx)->oc_bisect(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, lo, hi, vtarget, xnew_dev);
+    if (steps_host) *steps_host = s;
+    return 0;
+  });
+}
+
+int anneal_stiffness_law(const double* xphys_dev, const double* phi_dev, const uint8_t* flags_dev, double E0,
+                         double emin, double p, int64_t n, double* Ee_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(xphys_dev && phi_dev && flags_dev && Ee_dev, "null argument");
+    if (n > 0) stiffness_law(xphys_dev, phi_dev, flags_dev, E0, emin, p, n, Ee_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_design_stats(const double* xold_dev, const double* xnew_dev, const double* xphys_dev,
+                        const double* phi_dev, const double* dcx_dev, const double* dvx_dev,
+                        const uint8_t* flags_dev, int64_t n, double* out_dev, double* work_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(phi_dev && flags_dev && out_dev && work_dev, "null argument");
+    design_stats(xold_dev, xnew_dev, xphys_dev, phi_dev, dcx_dev, dvx_dev, flags_dev, n, out_dev, work_dev,
+                 (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs, double* out_dev,
+                   double* work_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(a_dev && b_dev && out_dev && work_dev && nrhs >= 1, "null argument");
+    dot(a_dev, b_dev, nnode, ldn, nrhs, work_dev, out_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,452 @@
+// Multigrid hierarchy and mixed-precision PCG driver (host side of the solve).
+#include "fem_ctx.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace anneal {
+
+// ---------------------------------------------------------------------------
+// Element stiffness (restates the quadrature of anneal/fem.py:36-84 with the
+// corner order q = a + 2b + 4c).
+void hex8_stiffness(double nu, double h, double* ke) {
+  const double c = 1.0 / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double D[6][6] = {};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) D[a][b] = c * (a == b ? 1.0 - nu : nu);
+  for (int a = 3; a < 6; ++a) D[a][a] = c * 0.5 * (1.0 - 2.0 * nu);
+  const double gp = 1.0 / std::sqrt(3.0), half = 0.5 * h, w = half * half * half;
+  std::fill(ke, ke + 576, 0.0);
+  for (int iz = 0; iz < 2; ++iz)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int ix = 0; ix < 2; ++ix) {
+        const double xi = ix ? gp : -gp, eta = iy ? gp : -gp, zeta = iz ? gp : -gp;
+        double B[6][24] = {};
+        for (int q = 0; q < 8; ++q) {
+          const double sa = (q & 1) ? 1.0 : -1.0, sb = (q & 2) ? 1.0 : -1.0, sc = (q & 4) ? 1.0 : -1.0;
+          const double gx = sa * (1 + sb * eta) * (1 + sc * zeta) / 8.0 / half;
+          const double gy = sb * (1 + sa * xi) * (1 + sc * zeta) / 8.0 / half;
+          const double gz = sc * (1 + sa * xi) * (1 + sb * eta) / 8.0 / half;
+          B[0][3 * q] = gx;
+          B[1][3 * q + 1] = gy;
+          B[2][3 * q + 2] = gz;
+          B[3][3 * q] = gy;
+          B[3][3 * q + 1] = gx;
+          B[4][3 * q + 1] = gz;
+          B[4][3 * q + 2] = gy;
+          B[5][3 * q] = gz;
+          B[5][3 * q + 2] = gx;
+        }
+        double DB[6][24];
+        for (int a = 0; a < 6; ++a)
+          for (int j = 0; j < 24; ++j) {
+            double s = 0;
+            for (int b = 0; b < 6; ++b) s += D[a][b] * B[b][j];
+            DB[a][j] = s;
+          }
+        for (int i = 0; i < 24; ++i)
+          for (int j = 0; j < 24; ++j) {
+            double s = 0;
+            for (int a = 0; a < 6; ++a) s += B[a][i] * DB[a][j];
+            ke[i * 24 + j] += s * w;
+          }
+      }
+}
+
+static inline int widx(int comp, int fm, int mz) { return 3 * (fm + 4 * mz) + comp; }
+
+KeCoef<double> walsh_coefficients(const double* ke) {
+  // Khat = T^T ke T / 64 with T[3q+c][3m+c] = W[q][m], W = Walsh signs.
+  double W[8][8];
+  for (int q = 0; q < 8; ++q)
+    for (int m = 0; m < 8; ++m) {
+      const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+      const int mx = m & 1, my = (m >> 1) & 1, mz = (m >> 2) & 1;
+      double s = 1.0;
+      if (mx && !a) s = -s;
+      if (my && !b) s = -s;
+      if (mz && !c) s = -s;
+      W[q][m] = s;
+    }
+  double K[24][24];
+  double maxabs = 0.0;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) {
+      const int m = i / 3, ci = i % 3, m2 = j / 3, cj = j % 3;
+      double s = 0.0;
+      for (int q = 0; q < 8; ++q)
+        for (int q2 = 0; q2 < 8; ++q2) s += W[q][m] * ke[(3 * q + ci) * 24 + 3 * q2 + cj] * W[q2][m2];
+      K[i][j] = s / 64.0;
+      maxabs = std::max(maxabs, std::fabs(K[i][j]));
+    }
+  KeCoef<double> k;
+  const double nA = K[widx(0, 1, 0)][widx(0, 1, 0)];
+  k.nB = K[widx(0, 1, 0)][widx(1, 2, 0)];
+  k.nAmB = nA - k.nB;
+  k.g = K[widx(0, 2, 0)][widx(0, 2, 0)];
+  k.pc = K[widx(0, 3, 0)][widx(0, 3, 0)];
+  k.pd = K[widx(0, 3, 0)][widx(2, 2, 1)];
+  const double tA = K[widx(2, 3, 0)][widx(2, 3, 0)];
+  k.tB = K[widx(2, 3, 0)][widx(0, 2, 1)];
+  k.tAmB = tA - k.tB;
+  k.w = K[widx(0, 3, 1)][widx(0, 3, 1)];
+  // Verify: the structured matvec must reproduce every column of Khat.
+  for (int j = 0; j < 24; ++j) {
+    double S[3][4] = {}, Dm[3][4] = {};
+    const int m = j / 3, c = j % 3, fm = m & 3, mz = m >> 2;
+    (mz ? Dm : S)[c][fm] = 1.0;
+    double gb[3][4], gt[3][4];
+    element_walsh(S, Dm, k, gb, gt);
+    for (int i = 0; i < 24; ++i) {
+      const int mi = i / 3, cc = i % 3, fmi = mi & 3, mzi = mi >> 2;
+      const double ys = 0.5 * (gb[cc][fmi] + gt[cc][fmi]), yd = 0.5 * (gt[cc][fmi] - gb[cc][fmi]);
+      const double got = mzi ? yd : ys;
+      const double want = K[i][j];
+      if (std::fabs(got - want) > 1e-11 * maxabs) {
+        char b[160];
+        snprintf(b, sizeof b, "Walsh pattern of ke does not hold at (%d,%d): %g vs %g", i, j, got, want);
+        throw std::runtime_error(b);
+      }
+    }
+  }
+  return k;
+}
+
+template <typename T>
+static KeCoef<T> cast_coef(const KeCoef<double>& k, double s) {
+  return KeCoef<T>{T(k.nAmB * s), T(k.nB * s), T(k.g * s), T(k.pc * s), T(k.pd * s), T(k.tAmB * s), T(k.tB * s),
+                   T(k.w * s)};
+}
+
+// ---------------------------------------------------------------------------
+void* FemContext::alloc(size_t bytes) {
+  void* p = nullptr;
+  ANNEAL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  owned_.push_back(p);
+  return p;
+}
+
+FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_t* active_host,
+                       const uint8_t* fixed_host, int nrhs_max, hipStream_t stream, int max_coarse_nodes)
+    : s_(stream), nrhs_max_(nrhs_max) {
+  ANNEAL_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be positive");
+  ANNEAL_CHECK(nrhs_max >= 1 && nrhs_max <= 64, "nrhs_max must be in [1, 64]");
+  ANNEAL_CHECK(nu > -1.0 && nu < 0.5, "Poisson ratio must be in (-1, 0.5)");
+  try {
+    hex8_stiffness(nu, h, ke_);
+    k64_ = walsh_coefficients(ke_);
+    // Level geometry
+    std::vector<GridDims> dims{GridDims{nx, ny, nz}};
+    while (true) {
+      const GridDims& g = dims.back();
+      if (g.nnode() <= max_coarse_nodes || g.nx < 2 || g.ny < 2 || g.nz < 2) break;
+      dims.push_back(GridDims{(g.nx + 1) / 2, (g.ny + 1) / 2, (g.nz + 1) / 2});
+    }
+    ANNEAL_CHECK(dims.back().nnode() <= 4096, "coarsest multigrid level too large for the dense solve");
+    // Host masks per level
+    std::vector<uint8_t> act(active_host, active_host + dims[0].nelem());
+    std::vector<uint8_t> fix(fixed_host, fixed_host + dims[0].nnode());
+    lv_.resize(dims.size());
+    for (size_t l = 0; l < dims.size(); ++l) {
+      const GridDims g = dims[l];
+      if (l > 0) {
+        const GridDims f = dims[l - 1];
+        std::vector<uint8_t> a2(g.nelem(), 0), f2(g.nnode(), 0);
+        for (int k = 0; k < f.nz; ++k)
+          for (int j = 0; j < f.ny; ++j)
+            for (int i = 0; i < f.nx; ++i)
+              if (act[i + (int64_t)f.nx * (j + (int64_t)f.ny * k)])
+                a2[(i / 2) + (int64_t)g.nx * ((j / 2) + (int64_t)g.ny * (k / 2))] = 1;
+        for (int K = 0; K <= g.nz; ++K)
+          for (int J = 0; J <= g.ny; ++J)
+            for (int I = 0; I <= g.nx; ++I) {
+              uint8_t bits = 0;
+              for (int k = std::max(0, 2 * K - 1); k <= std::min(f.nz, 2 * K + 1); ++k)
+                for (int j = std::max(0, 2 * J - 1); j <= std::min(f.ny, 2 * J + 1); ++j)
+                  for (int i = std::max(0, 2 * I - 1); i <= std::min(f.nx, 2 * I + 1); ++i)
+                    bits |= fix[i + (int64_t)(f.nx + 1) * (j + (int64_t)(f.ny + 1) * k)];
+              f2[I + (int64_t)(g.nx + 1) * (J + (int64_t)(g.ny + 1) * K)] = bits;
+            }
+        act.swap(a2);
+        fix.swap(f2);
+      }
+      std::vector<uint8_t> mk(g.nnode(), 0);
+      for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+          for (int i = 0; i < g.nx; ++i)
+            if (act[i + (int64_t)g.nx * (j + (int64_t)g.ny * k)])
+              for (int c = 0; c < 8; ++c)
+                mk[(i + (c & 1)) + (int64_t)(g.nx + 1) * ((j + ((c >> 1) & 1)) + (int64_t)(g.ny + 1) * (k + (c >> 2)))] =
+                    7;
+      for (int64_t n = 0; n < g.nnode(); ++n) mk[n] &= (uint8_t)(~fix[n] & 7);
+      Level& L = lv_[l];
+      L.g = g;
+      L.h = h * double(1 << l);
+      L.k = cast_coef<float>(k64_, double(1 << l));
+      L.kdiag = ke_[0] * double(1 << l);
+      L.mask = (uint8_t*)alloc(g.nnode());
+      ANNEAL_HIP(hipMemcpy(L.mask, mk.data(), g.nnode(), hipMemcpyHostToDevice));
+      L.Ee = (float*)alloc(g.nelem() * sizeof(float));
+      L.dinv = (float*)alloc(g.nnode() * sizeof(float));
+      const size_t vb = (size_t)nrhs_max * 3 * g.ldn() * sizeof(float);
+      L.x = (float*)alloc(vb);
+      L.x2 = (float*)alloc(vb);
+      L.b = (float*)alloc(vb);
+      L.r = (float*)alloc(vb);
+      ANNEAL_HIP(hipMemset(L.x, 0, vb));
+      ANNEAL_HIP(hipMemset(L.x2, 0, vb));
+      ANNEAL_HIP(hipMemset(L.b, 0, vb));
+      ANNEAL_HIP(hipMemset(L.r, 0, vb));
+    }
+    for (int i = 0; i < 24; ++i)
+      if (std::fabs(ke_[i * 24 + i] - ke_[0]) > 1e-12 * std::fabs(ke_[0]))
+        throw std::runtime_error("hex8 stiffness diagonal is not constant");
+    const GridDims& gc = lv_.back().g;
+    const int64_t nd = 3 * gc.nnode();
+    Ainv_ = (double*)alloc(nd * nd * sizeof(double));
+    Awork_ = (double*)alloc(nd * nd * sizeof(double));
+    ke_coarse_dev_ = (double*)alloc(576 * sizeof(double));
+    {
+      double kc[576];
+      const double sc = double(1 << (lv_.size() - 1));
+      for (int i = 0; i < 576; ++i) kc[i] = ke_[i] * sc;
+      ANNEAL_HIP(hipMemcpy(ke_coarse_dev_, kc, sizeof kc, hipMemcpyHostToDevice));
+    }
+    const GridDims& g0 = lv_[0].g;
+    Ee64_ = (double*)alloc(g0.nelem() * sizeof(double));
+    const size_t v32 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(float);
+    const size_t v64 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(double);
+    px_ = (float*)alloc(v32);
+    pr_ = (float*)alloc(v32);
+    pz_ = (float*)alloc(v32);
+    pp_ = (float*)alloc(v32);
+    pq_ = (float*)alloc(v32);
+    x64_ = (double*)alloc(v64);
+    r64_ = (double*)alloc(v64);
+    ANNEAL_HIP(hipMemset(px_, 0, v32));
+    ANNEAL_HIP(hipMemset(pr_, 0, v32));
+    ANNEAL_HIP(hipMemset(pz_, 0, v32));
+    ANNEAL_HIP(hipMemset(pp_, 0, v32));
+    ANNEAL_HIP(hipMemset(pq_, 0, v32));
+    ANNEAL_HIP(hipMemset(r64_, 0, v64));
+    const int nblk = std::max(kDotBlocks, apply_blocks(g0.nx, g0.ny, g0.nz));
+    part_ = (double*)alloc((size_t)nblk * nrhs_max * sizeof(double));
+    st_ = (double*)alloc(64 * kSt * sizeof(double));
+    red_ = (double*)alloc(4 * 64 * sizeof(double));
+    scale_ = (double*)alloc(64 * sizeof(double));
+    ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
+  } catch (...) {
+    for (void* p : owned_) (void)hipFree(p);
+    owned_.clear();
+    throw;
+  }
+}
+
+FemContext::~FemContext() {
+  (void)hipStreamSynchronize(s_);
+  for (void* p : owned_) (void)hipFree(p);
+  if (host_red_) (void)hipHostFree(host_red_);
+}
+
+template <>
+ApplyArgs<float> FemContext::args<float>(int l) const {
+  const Level& L = lv_[l];
+  ApplyArgs<float> a{};
+  a.Ee = L.Ee;
+  a.mask = L.mask;
+  a.dinv = L.dinv;
+  a.k = L.k;
+  a.omega = omega;
+  a.nx = L.g.nx;
+  a.ny = L.g.ny;
+  a.nz = L.g.nz;
+  a.ldn = L.g.ldn();
+  return a;
+}
+template <>
+ApplyArgs<double> FemContext::args<double>(int l) const {
+  const Level& L = lv_[0];
+  ApplyArgs<double> a{};
+  a.Ee = Ee64_;
+  a.mask = L.mask;
+  a.dinv = L.dinv;
+  a.k = k64_;
+  a.omega = omega;
+  a.nx = L.g.nx;
+  a.ny = L.g.ny;
+  a.nz = L.g.nz;
+  a.ldn = L.g.ldn();
+  (void)l;
+  return a;
+}
+
+void FemContext::set_Ee(const double* Ee_dev) {
+  const GridDims& g0 = lv_[0].g;
+  ANNEAL_HIP(hipMemcpyAsync(Ee64_, Ee_dev, g0.nelem() * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
+  for (size_t l = 1; l < lv_.size(); ++l) launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, s_);
+  for (size_t l = 0; l < lv_.size(); ++l) launch_inv_diag(lv_[l].Ee, lv_[l].mask, lv_[l].dinv, lv_[l].g, lv_[l].kdiag, s_);
+  const Level& C = lv_.back();
+  dense_assemble_invert(C.Ee, C.mask, ke_coarse_dev_, Ainv_, Awork_, C.g.nx, C.g.ny, C.g.nz, s_);
+}
+
+void FemContext::apply_f32(const float* u, float* y, int nrhs) {
+  ApplyArgs<float> a = args<float>(0);
+  a.u = u;
+  a.y = y;
+  launch_apply(a, kApply, nrhs, s_);
+}
+void FemContext::apply_f64(const double* u, double* y, int nrhs) {
+  ApplyArgs<double> a = args<double>(0);
+  a.u = u;
+  a.y = y;
+  launch_apply(a, kApply, nrhs, s_);
+}
+
+void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
+  Level& L = lv_[l];
+  if (l == (int)lv_.size() - 1) {
+    dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_);
+    return;
+  }
+  const int64_t n = L.g.nnode(), ld = L.g.ldn();
+  // 2*nu_pre - 1 Jacobi sweeps after the first: start in x2 so the result lands in x.
+  float* cur = L.x2;
+  float* nxt = x;
+  jacobi0(b, cur, L.dinv, L.mask, omega, n, ld, nrhs, s_);
+  ApplyArgs<float> a = args<float>(l);
+  a.b = b;
+  for (int s = 1; s < nu_pre; ++s) {
+    a.u = cur;
+    a.y = nxt;
+    launch_apply(a, kJacobi, nrhs, s_);
+    std::swap(cur, nxt);
+  }
+  a.u = cur;
+  a.y = L.r;
+  launch_apply(a, kResid, nrhs, s_);
+  Level& C = lv_[l + 1];
+  launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_);
+  vcycle(l + 1, C.b, C.x, nrhs);
+  launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_);
+  for (int s = 0; s < nu_pre; ++s) {
+    a.u = cur;
+    a.y = nxt;
+    launch_apply(a, kJacobi, nrhs, s_);
+    std::swap(cur, nxt);
+  }
+  if (cur != x) ANNEAL_HIP(hipMemcpyAsync(x, cur, (size_t)nrhs * 3 * ld * sizeof(float), hipMemcpyDeviceToDevice, s_));
+}
+
+int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out) {
+  ANNEAL_CHECK(nrhs >= 1 && nrhs <= nrhs_max_, "nrhs out of range for this context");
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  const size_t v64 = (size_t)nrhs * 3 * ld * sizeof(double);
+  if (!use_x0) ANNEAL_HIP(hipMemsetAsync(U, 0, v64, s_));
+  dot(F, F, n, ld, nrhs, part_, red_, s_);
+  ANNEAL_HIP(hipMemcpyAsync(host_red_, red_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+  ANNEAL_HIP(hipStreamSynchronize(s_));
+  std::vector<double> fnorm(nrhs), rnorm(nrhs), tol2(nrhs), sc(nrhs);
+  for (int r = 0; r < nrhs; ++r) fnorm[r] = std::sqrt(host_red_[r]);
+  std::vector<int> its(nrhs, 0);
+  std::vector<char> done(nrhs, 0), idone(nrhs, 0);
+  std::vector<double> hst((size_t)nrhs * kSt, 0.0);
+  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz);
+  int status = 1;
+  int total = 0;
+  bool bad = false;
+  for (int outer = 0; outer < 50 && !bad; ++outer) {
+    // true residual in FP64
+    ApplyArgs<double> a = args<double>(0);
+    a.u = U;
+    a.b = F;
+    a.y = r64_;
+    launch_apply(a, kResid, nrhs, s_);
+    dot(r64_, r64_, n, ld, nrhs, part_, red_, s_);
+    ANNEAL_HIP(hipMemcpyAsync(host_red_, red_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+    ANNEAL_HIP(hipStreamSynchronize(s_));
+    bool all = true;
+    for (int r = 0; r < nrhs; ++r) {
+      rnorm[r] = std::sqrt(host_red_[r]);
+      if (!std::isfinite(rnorm[r])) bad = true;
+      done[r] = (rnorm[r] <= rtol * fnorm[r]) ? 1 : 0;
+      all = all && done[r];
+    }
+    if (bad) break;
+    if (all) {
+      status = 0;
+      break;
+    }
+    if (total >= maxiter) break;
+    // inner FP32 PCG on the residual scaled to unit norm
+    for (int r = 0; r < nrhs; ++r) {
+      sc[r] = done[r] ? 0.0 : 1.0 / rnorm[r];
+      const double want = done[r] ? 1.0 : 0.5 * rtol * fnorm[r] / rnorm[r];
+      const double t = std::max(want, inner_tol_floor);
+      tol2[r] = t * t;
+      for (int k = 0; k < kSt; ++k) hst[r * kSt + k] = 0.0;
+      hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
+      idone[r] = done[r];
+    }
+    ANNEAL_HIP(hipMemcpyAsync(scale_, sc.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s_));
+    ANNEAL_HIP(hipMemcpyAsync(st_, hst.data(), hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
+    ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
+    vcycle(0, pr_, pz_, nrhs);
+    dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
+    ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZ, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
+                                hipMemcpyDeviceToDevice, s_));
+    pcg_dir(pp_, pz_, st_, n, ld, nrhs, 1, s_);
+    for (int it = 0; it < inner_max && total < maxiter; ++it) {
+      ApplyArgs<float> af = args<float>(0);
+      af.u = pp_;
+      af.y = pq_;
+      af.dot_part = part_;
+      launch_apply(af, kApply, nrhs, s_);
+      finalize(part_, nblk_apply, nrhs, red_, s_);
+      ANNEAL_HIP(hipMemcpy2DAsync(st_ + kPQ, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
+                                  hipMemcpyDeviceToDevice, s_));
+      pcg_alpha(st_, nrhs, s_);
+      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_);
+      ANNEAL_HIP(hipMemcpyAsync(host_red_ + 64, red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+      ANNEAL_HIP(hipStreamSynchronize(s_));
+      ++total;
+      bool iall = true, changed = false;
+      for (int r = 0; r < nrhs; ++r) {
+        if (idone[r]) continue;
+        ++its[r];
+        const double rr = host_red_[64 + r];
+        if (!std::isfinite(rr)) bad = true;
+        if (rr <= tol2[r]) {
+          idone[r] = 1;
+          changed = true;
+        }
+        iall = iall && idone[r];
+      }
+      if (bad || iall) break;
+      if (changed) {
+        for (int r = 0; r < nrhs; ++r) hst[r * kSt + kDone] = idone[r] ? 1.0 : 0.0;
+        ANNEAL_HIP(hipMemcpy2DAsync(st_ + kDone, kSt * sizeof(double), hst.data() + kDone, kSt * sizeof(double),
+                                    sizeof(double), nrhs, hipMemcpyHostToDevice, s_));
+      }
+      vcycle(0, pr_, pz_, nrhs);
+      dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
+      ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZnew, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
+                                  hipMemcpyDeviceToDevice, s_));
+      pcg_beta(st_, nrhs, s_);
+      pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+    }
+    if (bad) break;
+    for (int r = 0; r < nrhs; ++r) sc[r] = done[r] ? 0.0 : rnorm[r];
+    ANNEAL_HIP(hipMemcpyAsync(scale_, sc.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s_));
+    axpy_f32_to_f64(px_, U, scale_, n, ld, nrhs, s_);
+  }
+  if (bad) status = 2;
+  ANNEAL_HIP(hipStreamSynchronize(s_));
+  for (int r = 0; r < nrhs; ++r) iters_out[r] = its[r];
+  return status;
+}
+
+}  // namespace anneal

@@ -1,0 +1,88 @@
+// Device-resident hex8 voxel operator, geometric multigrid and mixed-precision PCG.
+//
+// Replaces FEM.__init__/assemble/solve of anneal/fem.py:133-232: instead of an
+// assembled CSR and pyamg, the operator is (ke, Ee, dof mask) and the solve is
+// PCG preconditioned by a rediscretised geometric V-cycle, with FP32 vectors,
+// FP64 dot products and FP64 true-residual refinement to the requested rtol.
+#pragma once
+#include <vector>
+#include "stencil.hpp"
+
+namespace anneal {
+
+// 24x24 hex8 stiffness for E = 1, cube of side h, 2x2x2 Gauss quadrature.
+// Corner q = a + 2b + 4c for the corner at (a, b, c) in {0,1}^3, dof 3q + comp.
+void hex8_stiffness(double nu, double h, double* ke /*576*/);
+// Walsh-domain coefficients of ke; throws if the block pattern does not hold.
+KeCoef<double> walsh_coefficients(const double* ke);
+
+struct Level {
+  GridDims g{};
+  double h = 1.0;
+  uint8_t* mask = nullptr;  // device, per node
+  float* Ee = nullptr;      // device, per element
+  float* dinv = nullptr;    // device, per node
+  float *x = nullptr, *x2 = nullptr, *b = nullptr, *r = nullptr;  // nrhs_max x 3 x ldn
+  KeCoef<float> k{};
+  double kdiag = 0.0;
+};
+
+struct SolveStats {
+  int iters[64];
+  int outer;
+};
+
+class FemContext {
+ public:
+  FemContext(int nx, int ny, int nz, double h, double nu, const uint8_t* active_host, const uint8_t* fixed_host,
+             int nrhs_max, hipStream_t stream, int max_coarse_nodes);
+  ~FemContext();
+  FemContext(const FemContext&) = delete;
+  FemContext& operator=(const FemContext&) = delete;
+
+  void set_Ee(const double* Ee_dev);
+  void apply_f32(const float* u, float* y, int nrhs);
+  void apply_f64(const double* u, double* y, int nrhs);
+  // F, U: device, [nrhs][3][ldn] FP64; U holds x0 when use_x0.  Returns 0 on
+  // convergence, 1 when maxiter was exhausted, 2 on a non-finite result.
+  int solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out);
+  void vcycle(int l, const float* b, float* x, int nrhs);
+
+  const GridDims& grid() const { return lv_[0].g; }
+  int levels() const { return (int)lv_.size(); }
+  int64_t ldn() const { return lv_[0].g.ldn(); }
+  const uint8_t* mask() const { return lv_[0].mask; }
+  const double* ke() const { return ke_; }  // host, 576
+  hipStream_t stream() const { return s_; }
+  const double* Ee64() const { return Ee64_; }
+  const KeCoef<double>& coef64() const { return k64_; }
+  const Level& level(int l) const { return lv_[l]; }
+
+  int nu_pre = 2;        // Jacobi sweeps per side
+  float omega = 0.6f;    // Jacobi damping
+  double inner_tol_floor = 1e-5;
+  int inner_max = 400;
+
+ private:
+  template <typename T>
+  ApplyArgs<T> args(int l) const;
+  void* alloc(size_t bytes);
+
+  hipStream_t s_;
+  int nrhs_max_;
+  std::vector<Level> lv_;
+  std::vector<void*> owned_;
+  double ke_[576];
+  KeCoef<double> k64_{};
+  double* Ee64_ = nullptr;
+  double* ke_coarse_dev_ = nullptr;  // 576 doubles, coarsest h
+  double* Ainv_ = nullptr;           // coarsest dense inverse
+  double* Awork_ = nullptr;
+  // PCG buffers
+  float *px_ = nullptr, *pr_ = nullptr, *pz_ = nullptr, *pp_ = nullptr, *pq_ = nullptr;
+  double *x64_ = nullptr, *r64_ = nullptr;
+  double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
+  double* host_red_ = nullptr;  // pinned
+};
+
+}  // namespace anneal

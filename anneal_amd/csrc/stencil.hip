@@ -1,0 +1,347 @@
+// Matrix-free hex8 operator on the voxel grid and the geometric-multigrid
+// transfer kernels (gfx950).
+//
+// The operator replaces the assembled CSR of anneal/fem.py:156-201: K u =
+// sum_e Ee_e * ke * u_e, with ke the 2x2x2-Gauss hex8 stiffness (fem.py:71-84).
+//
+// Apply kernel design (z-marching, owner-computes, no atomics):
+//   * a workgroup owns a TX x TY tile of element columns (one thread each) and
+//     walks a chunk of ZC node planes in +z;
+//   * each node plane of u is staged in LDS once; a thread forms the 2-D
+//     Walsh (sum/difference) transform of its column's 4 corner nodes, so an
+//     element's 8-corner transform is one add/sub per mode of two consecutive
+//     planes (register carried);
+//   * in the Walsh basis ke has 45 non-zeros in tiny blocks (about 33 FMAs per
+//     element instead of 576);
+//   * the face-domain outputs of the element below and above a node plane are
+//     summed in registers, transformed back, written to LDS, and each node
+//     gathers its 4 element columns in a fixed order: bit-reproducible.
+// The tile has one halo element column on the low x and y sides, so it
+// produces (TX-1) x (TY-1) node columns.
+#include "stencil.hpp"
+
+namespace anneal {
+
+template <typename T, int TX, int TY, int ZC, int MODE>
+__global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
+  constexpr int LX = TX + 1, LY = TY + 1, NT = TX * TY;
+  __shared__ T su[3][3][LY * LX];  // three planes: load p while p-1 is still read
+  __shared__ T sp[12][NT];
+  __shared__ double sred[NT / 64];
+
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  // blockIdx.x enumerates (tile x, tile y, z chunk); blockIdx.y = right-hand side.
+  int bid = blockIdx.x;
+  const int bx = bid % a.ntx;
+  bid /= a.ntx;
+  const int by = bid % a.nty;
+  const int bz = bid / a.nty;
+  const int r = blockIdx.y;
+  const int X0 = bx * (TX - 1), Y0 = by * (TY - 1), Z0 = bz * ZC;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
+  const int64_t ldn = a.ldn;
+  const T* __restrict__ u = a.u + (int64_t)3 * r * ldn;
+  T* __restrict__ y = a.y + (int64_t)3 * r * ldn;
+  const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
+
+  const int ei = X0 - 1 + tx, ej = Y0 - 1 + ty;
+  const bool ecol = (ei >= 0) && (ei < nx) && (ej >= 0) && (ej < ny);
+  const int ni = X0 + tx, nj = Y0 + ty;
+  const bool ncol = (tx < TX - 1) && (ty < TY - 1) && (ni <= nx) && (nj <= ny);
+
+  T Fprev[3][4], gup[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      Fprev[c][m] = T(0);
+      gup[c][m] = T(0);
+    }
+  double dot = 0.0;
+
+  const int pend = Z0 + ZC;  // last plane to load (inclusive)
+  for (int p = Z0 - 1; p <= pend; ++p) {
+    T* buf = &su[(p - Z0 + 1) % 3][0][0];
+    const bool pin = (p >= 0) && (p <= nz);
+    for (int idx = threadIdx.x; idx < LX * LY; idx += NT) {
+      const int lx = idx % LX, ly = idx / LX;
+      const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
+      T v0 = T(0), v1 = T(0), v2 = T(0);
+      if (pin && gx >= 0 && gx <= nx && gy >= 0 && gy <= ny) {
+        const int64_t n = gx + (int64_t)sx * gy + (int64_t)sxy * p;
+        v0 = u[n];
+        v1 = u[ldn + n];
+        v2 = u[2 * ldn + n];
+      }
+      buf[idx] = v0;
+      buf[LX * LY + idx] = v1;
+      buf[2 * LX * LY + idx] = v2;
+    }
+    __syncthreads();
+    T Fcur[3][4];
+    {
+      const int o = tx + LX * ty;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T* s = buf + c * LX * LY;
+        face_wht(s[o], s[o + 1], s[o + LX], s[o + LX + 1], Fcur[c]);
+      }
+    }
+    if (p >= Z0) {
+      const int L = p - 1;  // element layer between planes p-1 and p
+      T e = T(0);
+      if (ecol && L >= 0 && L < nz) e = a.Ee[ei + (int64_t)nx * (ej + (int64_t)ny * L)];
+      T gb[3][4], gt[3][4];
+      if (e != T(0)) {
+        T S[3][4], D[3][4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            S[c][m] = e * (Fcur[c][m] + Fprev[c][m]);
+            D[c][m] = e * (Fcur[c][m] - Fprev[c][m]);
+          }
+        element_walsh(S, D, a.k, gb, gt);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            gb[c][m] = T(0);
+            gt[c][m] = T(0);
+          }
+      }
+      if (L >= Z0 && L <= nz) {
+        // plane L now has both its element layers: finish it
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T G[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) G[m] = gup[c][m] + gb[c][m];
+          T c00, c10, c01, c11;
+          face_iwht(G, c00, c10, c01, c11);
+          sp[4 * c + 0][threadIdx.x] = c00;
+          sp[4 * c + 1][threadIdx.x] = c10;
+          sp[4 * c + 2][threadIdx.x] = c01;
+          sp[4 * c + 3][threadIdx.x] = c11;
+        }
+        __syncthreads();
+        if (ncol) {
+          const int t = threadIdx.x;
+          const int64_t n = ni + (int64_t)sx * nj + (int64_t)sxy * L;
+          const uint8_t mk = a.mask[n];
+          const T* prevbuf = &su[(p - Z0) % 3][0][0];
+          const int o = (tx + 1) + LX * (ty + 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            // corners: element (tx,ty) -> its (1,1) corner, (tx+1,ty) -> (0,1),
+            // (tx,ty+1) -> (1,0), (tx+1,ty+1) -> (0,0)
+            T v = sp[4 * c + 3][t] + sp[4 * c + 2][t + 1] + sp[4 * c + 1][t + TX] + sp[4 * c + 0][t + TX + 1];
+            T out;
+            if (MODE == kApply) {
+              out = v;
+            } else if (MODE == kResid) {
+              out = bb[c * ldn + n] - v;
+            } else {
+              const T uc = prevbuf[c * LX * LY + o];
+              out = uc + a.omega * T(a.dinv[n]) * (bb[c * ldn + n] - v);
+            }
+            if (!((mk >> c) & 1)) out = T(0);
+            y[c * ldn + n] = out;
+            if (a.dot_part) dot += (double)prevbuf[c * LX * LY + o] * (double)out;
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) gup[c][m] = gt[c][m];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) Fprev[c][m] = Fcur[c][m];
+  }
+  if (a.dot_part) {
+    // fixed-order block reduction
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < NT / 64; ++w) s += sred[w];
+      a.dot_part[(int64_t)r * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi diagonal: diag(n) = kdiag * sum of Ee over the <= 8 elements at n.
+__global__ void k_inv_diag(const float* __restrict__ Ee, const uint8_t* __restrict__ mask, float* __restrict__ dinv,
+                           int nx, int ny, int nz, double kdiag) {
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
+  double s = 0.0;
+  for (int dk = -1; dk <= 0; ++dk)
+    for (int dj = -1; dj <= 0; ++dj)
+      for (int di = -1; di <= 0; ++di) {
+        const int ei = i + di, ej = j + dj, ek = k + dk;
+        if (ei >= 0 && ei < nx && ej >= 0 && ej < ny && ek >= 0 && ek < nz)
+          s += Ee[ei + (int64_t)nx * (ej + (int64_t)ny * ek)];
+      }
+  const double d = s * kdiag;
+  dinv[n] = (mask[n] != 0 && d > 0.0) ? float(1.0 / d) : 0.f;
+}
+
+// Rediscretised coarse stiffness scale: mean of the (up to 8) children.
+__global__ void k_coarsen_Ee(const float* __restrict__ Ef, float* __restrict__ Ec, int nx, int ny, int nz, int cx,
+                             int cy, int cz) {
+  const int64_t nc = (int64_t)cx * cy * cz;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  const int I = e % cx, J = (e / cx) % cy, K = e / ((int64_t)cx * cy);
+  double s = 0.0;
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+        if (i < nx && j < ny && k < nz) s += Ef[i + (int64_t)nx * (j + (int64_t)ny * k)];
+      }
+  Ec[e] = float(s * 0.125);
+}
+
+// Full-weighting restriction b_c = P^T r (P = trilinear prolongation).
+template <typename T>
+__global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ maskc, int nx,
+                           int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
+  const int64_t nc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
+  const int64_t N = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (N >= nc) return;
+  const int I = N % (cx + 1), J = (N / (cx + 1)) % (cy + 1), K = N / ((int64_t)(cx + 1) * (cy + 1));
+  const uint8_t mk = maskc[N];
+  const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
+  for (int r = 0; r < nrhs; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      if ((mk >> c) & 1) {
+        const T* src = rf + (int64_t)(3 * r + c) * ldf;
+        for (int dk = -1; dk <= 1; ++dk) {
+          const int k = 2 * K + dk;
+          if (k < 0 || k > nz) continue;
+          const double wk = dk == 0 ? 1.0 : 0.5;
+          for (int dj = -1; dj <= 1; ++dj) {
+            const int j = 2 * J + dj;
+            if (j < 0 || j > ny) continue;
+            const double wj = wk * (dj == 0 ? 1.0 : 0.5);
+            for (int di = -1; di <= 1; ++di) {
+              const int i = 2 * I + di;
+              if (i < 0 || i > nx) continue;
+              s += wj * (di == 0 ? 1.0 : 0.5) * (double)src[i + (int64_t)sx * j + (int64_t)sxy * k];
+            }
+          }
+        }
+      }
+      bc[(int64_t)(3 * r + c) * ldc + N] = T(s);
+    }
+  }
+}
+
+// Prolongation with correction: x_f += M_f P x_c.
+template <typename T>
+__global__ void k_prolong_add(const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ maskf, int nx,
+                              int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
+  const int64_t nf = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nf) return;
+  const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
+  const uint8_t mk = maskf[n];
+  if (!mk) return;
+  const int csx = cx + 1, csxy = (cx + 1) * (cy + 1);
+  int I[2], J[2], K[2];
+  double wi[2], wj[2], wk[2];
+  int ni = 1, nj = 1, nk = 1;
+  I[0] = i >> 1; wi[0] = 1.0;
+  if (i & 1) { I[1] = I[0] + 1; wi[0] = wi[1] = 0.5; ni = 2; }
+  J[0] = j >> 1; wj[0] = 1.0;
+  if (j & 1) { J[1] = J[0] + 1; wj[0] = wj[1] = 0.5; nj = 2; }
+  K[0] = k >> 1; wk[0] = 1.0;
+  if (k & 1) { K[1] = K[0] + 1; wk[0] = wk[1] = 0.5; nk = 2; }
+  for (int r = 0; r < nrhs; ++r)
+    for (int c = 0; c < 3; ++c) {
+      if (!((mk >> c) & 1)) continue;
+      const T* src = xc + (int64_t)(3 * r + c) * ldc;
+      double s = 0.0;
+      for (int a = 0; a < nk; ++a)
+        for (int b = 0; b < nj; ++b)
+          for (int q = 0; q < ni; ++q)
+            s += wk[a] * wj[b] * wi[q] * (double)src[I[q] + (int64_t)csx * J[b] + (int64_t)csxy * K[a]];
+      xf[(int64_t)(3 * r + c) * ldf + n] += T(s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+template <typename T, int MODE>
+static void launch_apply_mode(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
+  constexpr int TX = 16, TY = 16, ZC = 8;
+  ApplyArgs<T> a = a0;
+  a.ntx = (a.nx + 1 + TX - 2) / (TX - 1);
+  a.nty = (a.ny + 1 + TY - 2) / (TY - 1);
+  a.nzc = (a.nz + 1 + ZC - 1) / ZC;
+  dim3 grid(a.ntx * a.nty * a.nzc, nrhs);
+  hipLaunchKernelGGL((k_apply<T, TX, TY, ZC, MODE>), grid, dim3(TX * TY), 0, s, a);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+int apply_blocks(int nx, int ny, int nz) {
+  constexpr int TX = 16, TY = 16, ZC = 8;
+  return ((nx + 1 + TX - 2) / (TX - 1)) * ((ny + 1 + TY - 2) / (TY - 1)) * ((nz + 1 + ZC - 1) / ZC);
+}
+
+template <typename T>
+void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s) {
+  switch (mode) {
+    case kApply: launch_apply_mode<T, kApply>(a, nrhs, s); break;
+    case kResid: launch_apply_mode<T, kResid>(a, nrhs, s); break;
+    case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, s); break;
+    default: throw std::invalid_argument("bad apply mode");
+  }
+}
+template void launch_apply<float>(const ApplyArgs<float>&, int, int, hipStream_t);
+template void launch_apply<double>(const ApplyArgs<double>&, int, int, hipStream_t);
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_inv_diag, dim3(nblk(g.nnode(), 256)), dim3(256), 0, s, Ee, mask, dinv, g.nx, g.ny, g.nz, kdiag);
+  ANNEAL_HIP(hipGetLastError());
+}
+void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_coarsen_Ee, dim3(nblk(c.nelem(), 256)), dim3(256), 0, s, Ef, Ec, f.nx, f.ny, f.nz, c.nx, c.ny,
+                     c.nz);
+  ANNEAL_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(c.nnode(), 256)), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
+  ANNEAL_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(f.nnode(), 256)), dim3(256), 0, s, xc, xf, maskf, f.nx, f.ny, f.nz,
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
+  ANNEAL_HIP(hipGetLastError());
+}
+template void launch_restrict<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
+                                     hipStream_t);
+template void launch_prolong_add<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
+                                        hipStream_t);
+
+}  // namespace anneal

@@ -1,0 +1,537 @@
+// Density filter, projection, sensitivities and OC bisection (FP64, gfx950).
+//
+// The filter is a direct LDS stencil: a workgroup stages a (TX+2R)(TY+2R)(TZ+2R)
+// tile of its input (computed on the fly: masked design, scaled sensitivity or
+// the OC trial design), and each thread produces TZ outputs along z, reading
+// each (dy, dx) column of the tile once and applying the 2R+1 weights of that
+// column to all its outputs.  The hat kernel max(0, r-|d|) is not separable
+// (its unfoldings have rank 2-3), so the direct form is the exact one.
+#include "topopt.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace anneal {
+
+enum FilterMode : int { kHs = 0, kFwd = 1, kBwd = 2, kPhys = 3, kOc = 4 };
+
+struct FilterArgs {
+  const double* in;
+  const double* dcx;
+  const double* dvx;
+  const uint8_t* flags;
+  const double* invHs;
+  const double* phi;
+  double* out;
+  double* out2;
+  double* part;
+  const double* oc;
+  double w[125];
+  double beta, move;
+  int nx, ny, nz;
+};
+
+constexpr int FTX = 32, FTY = 8, FTZ = 8;
+
+__device__ __forceinline__ void project(double xt, double beta, double& xp, double& dxp) {
+  if (beta <= 0.0) {
+    xp = xt;
+    dxp = 1.0;
+    return;
+  }
+  const double eta = 0.5;
+  const double t0 = tanh(beta * eta), t1 = tanh(beta * (1.0 - eta));
+  const double th = tanh(beta * (xt - eta));
+  xp = (t0 + th) / (t0 + t1);
+  dxp = beta * (1.0 - th * th) / (t0 + t1);
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
+  constexpr int LX = FTX + 2 * R, LY = FTY + 2 * R, LZ = FTZ + 2 * R, K = 2 * R + 1;
+  __shared__ double tile[LZ * LY * LX];
+  __shared__ double sred[FTX * FTY / 64];
+  if (MODE == kOc) {
+    if (a.oc[kOcDone] != 0.0) return;
+  }
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
+  int bid = blockIdx.x;
+  const int bx = bid % ntx;
+  bid /= ntx;
+  const int by = bid % nty;
+  const int bz = bid / nty;
+  const int X0 = bx * FTX, Y0 = by * FTY, Z0 = bz * FTZ;
+  double lam = 0.0, mv = 0.0;
+  if (MODE == kOc) {
+    lam = a.oc[kOcLam];
+    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
+  }
+  for (int idx = threadIdx.x; idx < LX * LY * LZ; idx += FTX * FTY) {
+    const int lx = idx % LX, ly = (idx / LX) % LY, lz = idx / (LX * LY);
+    const int gx = X0 - R + lx, gy = Y0 - R + ly, gz = Z0 - R + lz;
+    double v = 0.0;
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+      const int64_t e = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+      const uint8_t f = a.flags[e];
+      if (f & kActive) {
+        if (MODE == kHs) {
+          v = 1.0;
+        } else if (MODE == kFwd || MODE == kPhys) {
+          v = a.in[e];
+        } else if (MODE == kBwd) {
+          v = a.in[e] * a.invHs[e];
+        } else {  // kOc: trial design of optimize.py:150-153
+          if (f & kFrozen) {
+            v = 1.0;
+          } else {
+            const double x = a.in[e];
+            const double ratio = sqrt(fmax(-a.dcx[e], 0.0) / fmax(a.dvx[e] * lam, 1e-300));
+            v = fmin(fmax(x * ratio, fmax(0.0, x - mv)), fmin(1.0, x + mv));
+          }
+        }
+      }
+    }
+    tile[idx] = v;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % FTX, ty = threadIdx.x / FTX;
+  double acc[FTZ];
+#pragma unroll
+  for (int z = 0; z < FTZ; ++z) acc[z] = 0.0;
+#pragma unroll
+  for (int dy = 0; dy < K; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < K; ++dx) {
+      double col[LZ];
+#pragma unroll
+      for (int z = 0; z < LZ; ++z) col[z] = tile[(z * LY + ty + dy) * LX + tx + dx];
+#pragma unroll
+      for (int dz = 0; dz < K; ++dz) {
+        const double w = a.w[(dz * K + dy) * K + dx];
+        if (w != 0.0) {
+#pragma unroll
+          for (int z = 0; z < FTZ; ++z) acc[z] += w * col[z + dz];
+        }
+      }
+    }
+  const int gx = X0 + tx, gy = Y0 + ty;
+  double vol = 0.0;
+#pragma unroll
+  for (int z = 0; z < FTZ; ++z) {
+    const int gz = Z0 + z;
+    if (gx >= nx || gy >= ny || gz >= nz) continue;
+    const int64_t e = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+    const uint8_t f = a.flags[e];
+    const bool act = f & kActive;
+    if (MODE == kHs) {
+      a.out[e] = acc[z];
+    } else if (MODE == kFwd) {
+      a.out[e] = act ? acc[z] * a.invHs[e] : 0.0;
+    } else if (MODE == kBwd) {
+      a.out[e] = act ? acc[z] : 0.0;
+    } else {
+      double xp = 0.0, dxp = 0.0;
+      if (act) {
+        project(acc[z] * a.invHs[e], a.beta, xp, dxp);
+        if (f & kFrozen) xp = 1.0;
+        vol += a.phi[e] * xp;
+      }
+      if (MODE == kPhys) {
+        a.out[e] = xp;
+        a.out2[e] = dxp;
+      }
+    }
+  }
+  if (MODE == kPhys || MODE == kOc) {
+    for (int off = 32; off > 0; off >>= 1) vol += __shfl_down(vol, off, 64);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = vol;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < FTX * FTY / 64; ++w) s += sred[w];
+      a.part[blockIdx.x] = s;
+    }
+  }
+}
+
+// Sum the volume partials in a fixed order and advance the bisection.
+__global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc) {
+  if (oc[kOcDone] != 0.0) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) s += part[i];
+  __shared__ double sh[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double vol = sh[0] + sh[1] + sh[2] + sh[3];
+    double lo = oc[kOcLo], hi = oc[kOcHi];
+    const double lam = oc[kOcLam];
+    if (vol > oc[kOcVt]) lo = lam;
+    else hi = lam;
+    oc[kOcLo] = lo;
+    oc[kOcHi] = hi;
+    oc[kOcLamLast] = lam;
+    oc[kOcVol] = vol;
+    oc[kOcSteps] += 1.0;
+    if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) oc[kOcDone] = 1.0;
+    else oc[kOcLam] = sqrt(lo * hi);
+  }
+}
+
+__global__ void k_oc_final(const double* __restrict__ x, const double* __restrict__ dcx, const double* __restrict__ dvx,
+                           const uint8_t* __restrict__ flags, const double* __restrict__ oc, double beta, double move,
+                           int64_t n, double* __restrict__ xnew) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint8_t f = flags[e];
+  double v = 0.0;
+  if (f & kActive) {
+    if (f & kFrozen) {
+      v = 1.0;
+    } else {
+      const double lam = oc[kOcLamLast];
+      const double mv = beta <= 1.0 ? move : 0.5 * move;
+      const double xv = x[e];
+      const double ratio = sqrt(fmax(-dcx[e], 0.0) / fmax(dvx[e] * lam, 1e-300));
+      v = fmin(fmax(xv * ratio, fmax(0.0, xv - mv)), fmin(1.0, xv + mv));
+    }
+  }
+  xnew[e] = v;
+}
+
+__global__ void k_inv_hs(double* __restrict__ Hs, double* __restrict__ invHs, int64_t n) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double h = Hs[e];
+  if (h == 0.0) h = 1.0;
+  Hs[e] = h;
+  invHs[e] = 1.0 / h;
+}
+
+// ---------------------------------------------------------------------------
+static void cone_weights(double r, int R, double* w) {
+  const int K = 2 * R + 1;
+  for (int dz = 0; dz < K; ++dz)
+    for (int dy = 0; dy < K; ++dy)
+      for (int dx = 0; dx < K; ++dx) {
+        const double d = std::sqrt(double((dx - R) * (dx - R) + (dy - R) * (dy - R) + (dz - R) * (dz - R)));
+        w[(dz * K + dy) * K + dx] = std::max(0.0, r - d);
+      }
+}
+
+template <int MODE>
+static void launch_filter_mode(int R, const FilterArgs& a, int nblk, hipStream_t s) {
+  switch (R) {
+    case 1: hipLaunchKernelGGL((k_filter<1, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_filter<2, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
+    default: throw std::invalid_argument("filter radius above 3 voxels is not supported");
+  }
+  ANNEAL_HIP(hipGetLastError());
+}
+static void launch_filter(int mode, int R, const FilterArgs& a, int nblk, hipStream_t s) {
+  switch (mode) {
+    case kHs: launch_filter_mode<kHs>(R, a, nblk, s); break;
+    case kFwd: launch_filter_mode<kFwd>(R, a, nblk, s); break;
+    case kBwd: launch_filter_mode<kBwd>(R, a, nblk, s); break;
+    case kPhys: launch_filter_mode<kPhys>(R, a, nblk, s); break;
+    case kOc: launch_filter_mode<kOc>(R, a, nblk, s); break;
+  }
+}
+
+FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t* flags_host, hipStream_t stream)
+    : g_{nx, ny, nz}, s_(stream) {
+  ANNEAL_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be positive");
+  ANNEAL_CHECK(r_vox > 0.0 && std::isfinite(r_vox), "filter radius must be positive");
+  R_ = std::max((int)std::ceil(r_vox) - 1, 1);
+  ANNEAL_CHECK(R_ <= 2, "filter radius above 3 voxels is not supported");
+  std::fill(w_, w_ + 125, 0.0);
+  cone_weights(r_vox, R_, w_);
+  const int64_t n = g_.nelem();
+  nblk_ = ((nx + FTX - 1) / FTX) * ((ny + FTY - 1) / FTY) * ((nz + FTZ - 1) / FTZ);
+  ANNEAL_HIP(hipMalloc(&flags_, n));
+  ANNEAL_HIP(hipMalloc(&Hs_, n * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&part_, std::max(nblk_, 6 * 512) * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&oc_, kOcLen * sizeof(double)));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_, 16 * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
+  FilterArgs a{};
+  a.flags = flags_;
+  a.out = Hs_;
+  std::copy(w_, w_ + 125, a.w);
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  launch_filter(kHs, R_, a, nblk_, s_);
+  hipLaunchKernelGGL(k_inv_hs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, Hs_, invHs_, n);
+  ANNEAL_HIP(hipGetLastError());
+  ANNEAL_HIP(hipStreamSynchronize(s_));
+}
+
+FilterContext::~FilterContext() {
+  (void)hipStreamSynchronize(s_);
+  (void)hipFree(flags_);
+  (void)hipFree(Hs_);
+  (void)hipFree(invHs_);
+  (void)hipFree(part_);
+  (void)hipFree(oc_);
+  (void)hipHostFree(host_);
+}
+
+void FilterContext::forward(const double* x, double* y) {
+  FilterArgs a{};
+  a.in = x;
+  a.flags = flags_;
+  a.invHs = invHs_;
+  a.out = y;
+  std::copy(w_, w_ + 125, a.w);
+  a.nx = g_.nx;
+  a.ny = g_.ny;
+  a.nz = g_.nz;
+  launch_filter(kFwd, R_, a, nblk_, s_);
+}
+
+void FilterContext::backward(const double* d, double* y) {
+  FilterArgs a{};
+  a.in = d;
+  a.flags = flags_;
+  a.invHs = invHs_;
+  a.out = y;
+  std::copy(w_, w_ + 125, a.w);
+  a.nx = g_.nx;
+  a.ny = g_.ny;
+  a.nz = g_.nz;
+  launch_filter(kBwd, R_, a, nblk_, s_);
+}
+
+void FilterContext::physical(const double* x, double beta, const double* phi, double* xphys, double* dxp,
+                             double* vol_dev) {
+  FilterArgs a{};
+  a.in = x;
+  a.flags = flags_;
+  a.invHs = invHs_;
+  a.phi = phi;
+  a.out = xphys;
+  a.out2 = dxp;
+  a.part = part_;
+  a.beta = beta;
+  std::copy(w_, w_ + 125, a.w);
+  a.nx = g_.nx;
+  a.ny = g_.ny;
+  a.nz = g_.nz;
+  launch_filter(kPhys, R_, a, nblk_, s_);
+  if (vol_dev) finalize(part_, nblk_, 1, vol_dev, s_);
+}
+
+int FilterContext::oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
+                             double move, double lo, double hi, double vtarget, double* xnew) {
+  double st[kOcLen] = {};
+  st[kOcLo] = lo;
+  st[kOcHi] = hi;
+  st[kOcLam] = std::sqrt(lo * hi);
+  st[kOcVt] = vtarget;
+  ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
+  FilterArgs a{};
+  a.in = x;
+  a.dcx = dcx;
+  a.dvx = dvx;
+  a.flags = flags_;
+  a.invHs = invHs_;
+  a.phi = phi;
+  a.part = part_;
+  a.oc = oc_;
+  a.beta = beta;
+  a.move = move;
+  std::copy(w_, w_ + 125, a.w);
+  a.nx = g_.nx;
+  a.ny = g_.ny;
+  a.nz = g_.nz;
+  int launched = 0;
+  while (true) {
+    // the schedule from 1e16 down to 1+1e-9 takes 36 halvings; kernels after
+    // the stop are no-ops that read the device flag
+    const int batch = launched == 0 ? 37 : 8;
+    for (int i = 0; i < batch && launched < 100; ++i, ++launched) {
+      launch_filter(kOc, R_, a, nblk_, s_);
+      hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_, nblk_, oc_);
+      ANNEAL_HIP(hipGetLastError());
+    }
+    ANNEAL_HIP(hipMemcpyAsync(host_, oc_, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
+    ANNEAL_HIP(hipStreamSynchronize(s_));
+    if (host_[kOcDone] != 0.0 || launched >= 100) break;
+  }
+  const int64_t n = g_.nelem();
+  hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, x, dcx, dvx, flags_, oc_, beta,
+                     move, n, xnew);
+  ANNEAL_HIP(hipGetLastError());
+  return (int)host_[kOcSteps];
+}
+
+// ---------------------------------------------------------------------------
+// Element energies in the Walsh basis: ce = sum over modes of u_hat * (Khat u_hat).
+__device__ __forceinline__ void element_modes(const double* __restrict__ u, int64_t ldn, int i, int j, int k, int nx,
+                                              int ny, double (&S)[3][4], double (&D)[3][4]) {
+  const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+  const int64_t n0 = i + sx * j + sxy * k;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* p = u + c * ldn;
+    double Fb[4], Ft[4];
+    face_wht(p[n0], p[n0 + 1], p[n0 + sx], p[n0 + sx + 1], Fb);
+    face_wht(p[n0 + sxy], p[n0 + sxy + 1], p[n0 + sxy + sx], p[n0 + sxy + sx + 1], Ft);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      S[c][m] = Ft[m] + Fb[m];
+      D[c][m] = Ft[m] - Fb[m];
+    }
+  }
+}
+
+__device__ __forceinline__ double element_ce(const double (&S)[3][4], const double (&D)[3][4],
+                                             const KeCoef<double>& k) {
+  double YS[3][4], YD[3][4];
+  element_walsh_modes(S, D, k, YS, YD);
+  double ce = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) ce += S[c][m] * YS[c][m] + D[c][m] * YD[c][m];
+  return ce;
+}
+
+__global__ void k_element_energy(const double* __restrict__ U, int nrhs, int64_t ldn, int nx, int ny, int nz,
+                                 KeCoef<double> k, const uint8_t* __restrict__ flags, double* __restrict__ out) {
+  const int64_t ne = (int64_t)nx * ny * nz;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int i = e % nx, j = (e / nx) % ny, kk = e / ((int64_t)nx * ny);
+  const bool act = flags[e] & kActive;
+  for (int r = 0; r < nrhs; ++r) {
+    double ce = 0.0;
+    if (act) {
+      double S[3][4], D[3][4];
+      element_modes(U + (int64_t)3 * r * ldn, ldn, i, j, kk, nx, ny, S, D);
+      ce = element_ce(S, D, k);
+    }
+    out[(int64_t)r * ne + e] = ce;
+  }
+}
+
+__global__ void k_sensitivity(const double* __restrict__ U, int nrhs, int64_t ldn, int nx, int ny, int nz,
+                              KeCoef<double> k, const uint8_t* __restrict__ flags, const double* __restrict__ xphys,
+                              const double* __restrict__ dxp, const double* __restrict__ phi,
+                              const double* __restrict__ coef, double p, double emin, double E0,
+                              double* __restrict__ dcdx, double* __restrict__ dvdx) {
+  const int64_t ne = (int64_t)nx * ny * nz;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  if (!(flags[e] & kActive)) {
+    dcdx[e] = 0.0;
+    dvdx[e] = 0.0;
+    return;
+  }
+  const int i = e % nx, j = (e / nx) % ny, kk = e / ((int64_t)nx * ny);
+  const double fac = -p * pow(xphys[e], p - 1.0) * (1.0 - emin) * E0 * phi[e];
+  double dc = 0.0;
+  for (int r = 0; r < nrhs; ++r) {
+    double S[3][4], D[3][4];
+    element_modes(U + (int64_t)3 * r * ldn, ldn, i, j, kk, nx, ny, S, D);
+    dc += coef[r] * (fac * element_ce(S, D, k));
+  }
+  dc = fmin(dc, 0.0);
+  dcdx[e] = dc * dxp[e];
+  dvdx[e] = phi[e] * dxp[e];
+}
+
+__global__ void k_stiffness_law(const double* __restrict__ xphys, const double* __restrict__ phi,
+                                const uint8_t* __restrict__ flags, double E0, double emin, double p, int64_t n,
+                                double* __restrict__ Ee) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Ee[e] = (flags[e] & kActive) ? E0 * phi[e] * (emin + pow(xphys[e], p) * (1.0 - emin)) : 0.0;
+}
+
+constexpr int kStatBlocks = 512;
+__global__ __launch_bounds__(256) void k_design_stats(const double* __restrict__ xold, const double* __restrict__ xnew,
+                                                      const double* __restrict__ xphys, const double* __restrict__ phi,
+                                                      const double* __restrict__ dcx, const double* __restrict__ dvx,
+                                                      const uint8_t* __restrict__ flags, int64_t n,
+                                                      double* __restrict__ part) {
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const uint8_t f = flags[e];
+    if (!(f & kActive)) continue;
+    if (xphys) s[0] += phi[e] * xphys[e];
+    if (f & kFrozen) continue;
+    s[3] += 1.0;
+    if (xphys) {
+      const double v = xphys[e];
+      if (v > 0.1 && v < 0.9) s[1] += 1.0;
+    }
+    if (xold && xnew) s[2] = fmax(s[2], fabs(xnew[e] - xold[e]));
+    if (dcx) s[4] += -dcx[e];
+    if (dvx) s[5] += dvx[e];
+  }
+  __shared__ double sh[6][4];
+  for (int q = 0; q < 6; ++q) {
+    double v = s[q];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(v, off, 64);
+      v = (q == 2) ? fmax(v, o) : v + o;
+    }
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    double v = sh[q][0];
+    for (int w = 1; w < 4; ++w) v = (q == 2) ? fmax(v, sh[q][w]) : v + sh[q][w];
+    part[q * gridDim.x + blockIdx.x] = v;
+  }
+}
+__global__ __launch_bounds__(64) void k_design_stats_final(const double* __restrict__ part, int nblk,
+                                                           double* __restrict__ out) {
+  if (threadIdx.x >= 6) return;
+  const int q = threadIdx.x;
+  double v = 0.0;
+  for (int b = 0; b < nblk; ++b) v = (q == 2) ? fmax(v, part[q * nblk + b]) : v + part[q * nblk + b];
+  out[q] = v;
+}
+
+void element_energy(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
+                    const uint8_t* flags, double* out, hipStream_t s) {
+  const int64_t n = g.nelem();
+  hipLaunchKernelGGL(k_element_energy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, U, nrhs, ldn, g.nx, g.ny,
+                     g.nz, k, flags, out);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+void sensitivity(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
+                 const uint8_t* flags, const double* xphys, const double* dxp, const double* phi, const double* coef,
+                 double p, double emin, double E0, double* dcdx, double* dvdx, hipStream_t s) {
+  const int64_t n = g.nelem();
+  hipLaunchKernelGGL(k_sensitivity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, U, nrhs, ldn, g.nx, g.ny,
+                     g.nz, k, flags, xphys, dxp, phi, coef, p, emin, E0, dcdx, dvdx);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+void stiffness_law(const double* xphys, const double* phi, const uint8_t* flags, double E0, double emin, double p,
+                   int64_t n, double* Ee, hipStream_t s) {
+  hipLaunchKernelGGL(k_stiffness_law, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, xphys, phi, flags, E0,
+                     emin, p, n, Ee);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+void design_stats(const double* xold, const double* xnew, const double* xphys, const double* phi, const double* dcx,
+                  const double* dvx, const uint8_t* flags, int64_t n, double* out_dev, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_design_stats, dim3(kStatBlocks), dim3(256), 0, s, xold, xnew, xphys, phi, dcx, dvx, flags, n,
+                     part);
+  ANNEAL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_design_stats_final, dim3(1), dim3(64), 0, s, part, kStatBlocks, out_dev);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+}  // namespace anneal

@@ -1,0 +1,67 @@
+// Density-side kernels of the SIMP loop: hat filter, Heaviside projection,
+// sensitivities and the OC bisection, all FP64 on the full voxel grid.
+#pragma once
+#include "stencil.hpp"
+
+namespace anneal {
+
+// Element flags
+constexpr uint8_t kActive = 1, kFrozen = 2;
+
+// OC bisection state on the device (FP64).
+constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7;
+constexpr int kOcLen = 8;
+
+class FilterContext {
+ public:
+  // Linear hat kernel max(0, r - |d|) on [-R, R]^3, R = max(ceil(r) - 1, 1)
+  // (anneal/optimize.py:24-34), normalised by Hs = conv(active) with 0 -> 1.
+  FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t* flags_host, hipStream_t stream);
+  ~FilterContext();
+  FilterContext(const FilterContext&) = delete;
+  FilterContext& operator=(const FilterContext&) = delete;
+
+  void forward(const double* x, double* y);   // optimize.py:41-43
+  void backward(const double* d, double* y);  // optimize.py:45-47
+  // xPhys, dxp = project(forward(x)), frozen -> 1 (optimize.py:81-93).
+  // Also returns sum(phi * xPhys) over active voxels in *vol_dev (device).
+  void physical(const double* x, double beta, const double* phi, double* xphys, double* dxp, double* vol_dev);
+  // The whole geometric bisection (optimize.py:144-161) on the device.  Returns
+  // the number of trial steps; xnew receives the last trial design.
+  int oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
+                double lo, double hi, double vtarget, double* xnew);
+
+  const GridDims& grid() const { return g_; }
+  const uint8_t* flags() const { return flags_; }
+  const double* Hs() const { return Hs_; }
+  int R() const { return R_; }
+  hipStream_t stream() const { return s_; }
+
+ private:
+  GridDims g_;
+  int R_;
+  double w_[125];
+  hipStream_t s_;
+  uint8_t* flags_ = nullptr;
+  double *Hs_ = nullptr, *invHs_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
+  int nblk_ = 0;
+};
+
+// ce_r(e) = u_e^T ke u_e for E = 1 (anneal/fem.py:239-242), out [nrhs][nelem].
+void element_energy(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
+                    const uint8_t* flags, double* out, hipStream_t s);
+// dc*dxp and phi*dxp (anneal/optimize.py:118, 125-142):
+//   dce_r = -p x^(p-1) (1-emin) E0 phi ce_r; dc = min(sum_r coef_r dce_r, 0)
+void sensitivity(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
+                 const uint8_t* flags, const double* xphys, const double* dxp, const double* phi, const double* coef,
+                 double p, double emin, double E0, double* dcdx, double* dvdx, hipStream_t s);
+// Ee = E0 phi (emin + x^p (1 - emin)) on active voxels, 0 elsewhere (optimize.py:118).
+void stiffness_law(const double* xphys, const double* phi, const uint8_t* flags, double E0, double emin, double p,
+                   int64_t n, double* Ee, hipStream_t s);
+// Per-iteration statistics over design voxels (active, not frozen):
+//   out[0] = sum phi*xphys (active), out[1] = #grey (0.1 < xphys < 0.9), out[2] = max |xnew - xold|,
+//   out[3] = #design, out[4] = sum(-dcx) over design, out[5] = sum(dvx) over design
+void design_stats(const double* xold, const double* xnew, const double* xphys, const double* phi, const double* dcx,
+                  const double* dvx, const uint8_t* flags, int64_t n, double* out_dev, double* part, hipStream_t s);
+
+}  // namespace anneal

@@ -1,0 +1,127 @@
+/* C ABI of libanneal_hip.so: the MI355X (gfx950) SIMP hot path of Anneal.
+ *
+ * The reference (ImpurestTadpole/anneal) is pure Python with no FFI; these
+ * entry points are what its hot-path calls bind to.  Each declaration names
+ * the reference code it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero status on failure;
+ *    anneal_last_error() then returns a message for the calling thread.
+ *  - Pointers named *_dev are device pointers (HIP); *_host are host pointers.
+ *  - Voxel grid nx*ny*nz, element e = i + nx*(j + ny*k); node
+ *    n = i + (nx+1)*(j + (ny+1)*k) (anneal/fem.py:87-127).
+ *  - Nodal vectors are SoA: component c of right-hand side r starts at
+ *    (3*r + c) * ldn, where ldn (>= nnode) is reported by anneal_fem_info.
+ *    The reference's interleaved dof 3n+c (fem.py:143) maps to [c*ldn + n].
+ *  - Element fields cover the full grid; flags bit0 = active, bit1 = frozen.
+ *  - stream is a hipStream_t (may be NULL for the default stream); all work of
+ *    a context is ordered on it.  Contexts are not thread-safe; use one per
+ *    thread (the reference runs one job per worker thread, gui/server.py:52-56).
+ */
+#ifndef ANNEAL_HIP_H
+#define ANNEAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* anneal_last_error(void);
+/* ABI version (increments on incompatible changes). */
+int anneal_abi_version(void);
+
+/* ---- hex8 voxel solve: replaces anneal/fem.py FEM ------------------------ */
+
+/* hex8_matrices(nu, h)[0] (fem.py:71-84) as the library computes it (host
+ * only, no device needed): ke_host receives the 24x24 stiffness for E = 1 in
+ * the library's corner order q = a + 2b + 4c (corner at (a,b,c)), dof 3q+comp;
+ * coef_host receives the 8 Walsh-basis coefficients the kernels use
+ * (normal A-B, B, shear, pair C, pair D, triple a-b, b, trilinear), already
+ * divided by 64.  Fails if ke does not have the block pattern they assume. */
+int anneal_ke_host(double nu, double h, double* ke_host, double* coef_host);
+
+/* FEM.__init__ (fem.py:133-196): active voxels and fixed dofs define the free
+ * set.  active_host: nelem bytes (non-zero = active).  fixed_host: nnode bytes,
+ * bit c set when dof 3n+c is fixed.  nrhs_max: largest right-hand-side block
+ * (load cases) solved at once.  Builds the multigrid hierarchy. */
+int anneal_fem_create(void** ctx, int nx, int ny, int nz, double h, double nu, const uint8_t* active_host,
+                      const uint8_t* fixed_host, int nrhs_max, void* stream);
+int anneal_fem_destroy(void* ctx);
+/* Leading dimension of nodal vectors, number of multigrid levels, and the
+ * free-dof mask (nnode bytes, bit c = dof 3n+c free) copied to mask_host if
+ * not NULL. */
+int anneal_fem_info(void* ctx, int64_t* ldn, int* levels, uint8_t* mask_host);
+/* FEM.assemble(Ee) (fem.py:198-201): Ee_dev holds nelem FP64 stiffness
+ * scales (0 on inactive voxels).  No matrix is formed: this sets the operator
+ * and rebuilds the multigrid levels. */
+int anneal_fem_set_Ee(void* ctx, const double* Ee_dev);
+/* y = K u on free dofs (fixed/unused dofs give 0): the product the reference
+ * forms with its CSR matrix (fem.py:201), FP32 and FP64 variants. */
+int anneal_fem_apply_f32(void* ctx, const float* u_dev, float* y_dev, int nrhs);
+int anneal_fem_apply_f64(void* ctx, const double* u_dev, double* y_dev, int nrhs);
+/* FEM.solve (fem.py:206-232): K U = F for nrhs right-hand sides at once, stop
+ * when ||F - K U||_2 <= rtol ||F||_2 (FP64 true residual) for every one.
+ * U_dev holds the initial guess when use_x0.  iters_host[r] receives the PCG
+ * iterations of rhs r.  Returns 0, or 3 = not converged within maxiter,
+ * 4 = non-finite result (the reference raises SolverError for both). */
+int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, double rtol, int maxiter, int use_x0,
+                     int* iters_host);
+/* One V-cycle of the multigrid preconditioner on FP32 vectors (z = M r). */
+int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs);
+/* FEM.element_energy (fem.py:239-242): ce[r*nelem + e] = u_e^T ke u_e, E = 1. */
+int anneal_fem_element_energy(void* ctx, const double* U_dev, int nrhs, double* ce_dev);
+/* Sensitivities of optimize.py:118,125-142 fused: with
+ * dce_r = -p xPhys^(p-1) (1-emin) E0 phi ce_r, dc = min(sum_r coef_r dce_r, 0),
+ * writes dcdx = dc*dxp and dvdx = phi*dxp (0 on inactive voxels).
+ * coef_dev: nrhs FP64 weights (w for "sum", the p-norm chain factor for "worst"). */
+int anneal_fem_sensitivity(void* ctx, const double* U_dev, int nrhs, const uint8_t* flags_dev, const double* xphys_dev,
+                           const double* dxp_dev, const double* phi_dev, const double* coef_dev, double p, double emin,
+                           double E0, double* dcdx_dev, double* dvdx_dev);
+
+/* ---- density filter and OC update: replaces anneal/optimize.py ---------- */
+
+/* Filter(grid, active, r_vox) (optimize.py:24-34); flags_host: nelem bytes. */
+int anneal_filter_create(void** ctx, int nx, int ny, int nz, double r_vox, const uint8_t* flags_host, void* stream);
+int anneal_filter_destroy(void* ctx);
+/* Filter.forward / Filter.backward (optimize.py:41-47) on full-grid FP64
+ * arrays; inactive voxels read as and are written as 0. */
+int anneal_filter_forward(void* ctx, const double* x_dev, double* y_dev);
+int anneal_filter_backward(void* ctx, const double* d_dev, double* y_dev);
+/* physical(x, with_grad=True) (optimize.py:81-93): filter, Heaviside
+ * projection with sharpness beta (identity for beta <= 0), frozen -> 1.
+ * vol_dev (may be NULL) receives sum(phi * xPhys). */
+int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const double* phi_dev, double* xphys_dev,
+                           double* dxp_dev, double* vol_dev);
+/* The OC bisection loop (optimize.py:144-161) on the device, starting from the
+ * bracket [lo, hi]; xnew_dev receives the last trial design and *steps_host
+ * the number of trials. */
+int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                     const double* phi_dev, double beta, double move, double lo, double hi, double vtarget,
+                     double* xnew_dev, int* steps_host);
+
+/* ---- element-wise pieces of simp() ---------------------------------------- */
+
+/* Ee = E0 phi (emin + xPhys^p (1-emin)) on active voxels, else 0 (optimize.py:118). */
+int anneal_stiffness_law(const double* xphys_dev, const double* phi_dev, const uint8_t* flags_dev, double E0,
+                         double emin, double p, int64_t n, double* Ee_dev, void* stream);
+/* Reductions used by simp() for its history (optimize.py:145, 162-167):
+ * out_dev[0] = sum phi*xPhys over active, [1] = #grey design voxels,
+ * [2] = max |xnew - xold| over design, [3] = #design voxels,
+ * [4] = sum(-dcx) over design, [5] = sum(dvx) over design.  Any input may be
+ * NULL (its outputs are then 0).  work_dev: 3072 doubles. */
+int anneal_design_stats(const double* xold_dev, const double* xnew_dev, const double* xphys_dev,
+                        const double* phi_dev, const double* dcx_dev, const double* dvx_dev,
+                        const uint8_t* flags_dev, int64_t n, double* out_dev, double* work_dev, void* stream);
+/* FP64 dot products of nodal vectors: out_dev[r] = sum over the 3 components
+ * of a_r . b_r (compliance f.u, optimize.py:125).  work_dev: 512*nrhs doubles. */
+int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs, double* out_dev,
+                   double* work_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANNEAL_HIP_H */

@@ -1,0 +1,148 @@
+"""GPU parity of the matrix-free operator and the multigrid PCG solve against the FP64 oracle.
+
+Tolerances: the FP64 apply must equal the assembled CSR product to 1e-12
+relative; the FP32 apply to 2e-6 relative (single rounding of inputs and
+stiffness scales); solves meet the requested rtol on the FP64 residual and
+agree with the oracle's converged solution to the accuracy rtol implies.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import aos_to_soa, oracle_fem, random_problem, soa_to_aos
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _product_fem(g, active, fixed):
+    from anneal_amd.fem import FEM, Grid
+    return FEM(Grid(g.shape, g.h, g.origin), active, fixed, 0.3)
+
+
+@pytest.mark.parametrize("shape,inactive,fix,seed", [
+    ((3, 2, 2), 0.0, "clamp", 0),
+    ((7, 5, 4), 0.3, "random", 1),
+    ((17, 16, 9), 0.2, "random", 2),
+    ((33, 18, 17), 0.0, "clamp", 3),
+    ((48, 24, 24), 0.1, "clamp", 4),
+])
+def test_apply_matches_csr(torch, shape, inactive, fix, seed):
+    print(f"apply parity shape={shape} inactive={inactive} fix={fix} seed={seed}", flush=True)
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem(shape, seed, inactive, fix)
+    ref = oracle_fem(g, active, fixed)
+    K = ref.assemble(Ee[ref.act])
+    fem = _product_fem(g, active, fixed)
+    assert np.array_equal(fem.free, ref.free), "free dof set differs from fem.py:145-155"
+    fem.assemble(Ee[fem.act])
+    rng = np.random.default_rng(seed + 100)
+    nrhs = 2
+    us = [rng.standard_normal(ref.nfree) for _ in range(nrhs)]
+    want = [K @ u for u in us]
+    U = np.stack([aos_to_soa(ref.expand(u), g.nnode, fem.ldn) for u in us])
+    U64 = torch.from_numpy(U).cuda()
+    Y64 = torch.empty_like(U64)
+    _lib.call("anneal_fem_apply_f64", fem._ctx, _lib.ptr(U64), _lib.ptr(Y64), nrhs)
+    U32 = U64.float().contiguous()
+    Y32 = torch.empty_like(U32)
+    _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U32), _lib.ptr(Y32), nrhs)
+    torch.cuda.synchronize()
+    for r in range(nrhs):
+        y64 = soa_to_aos(Y64[r].cpu().numpy(), g.nnode)
+        y32 = soa_to_aos(Y32[r].double().cpu().numpy(), g.nnode)
+        scale = np.abs(want[r]).max()
+        assert np.abs(y64[ref.free] - want[r]).max() <= 1e-12 * scale
+        assert np.abs(y32[ref.free] - want[r]).max() <= 2e-6 * scale
+        notfree = np.setdiff1d(np.arange(3 * g.nnode), ref.free)
+        assert np.all(y64[notfree] == 0) and np.all(y32[notfree] == 0)
+
+
+def test_apply_is_deterministic(torch):
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem((40, 20, 20), 7, 0.1)
+    fem = _product_fem(g, active, fixed)
+    fem.assemble(Ee[fem.act])
+    U = torch.randn(1, 3, fem.ldn, dtype=torch.float32, device="cuda")
+    a, b = torch.empty_like(U), torch.empty_like(U)
+    _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(a), 1)
+    _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(b), 1)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape,inactive,seed,rtol", [
+    ((12, 6, 6), 0.0, 0, 1e-10),
+    ((20, 10, 9), 0.15, 1, 1e-8),
+    ((48, 24, 24), 0.0, 2, 1e-8),
+])
+def test_solve_matches_oracle(torch, shape, inactive, seed, rtol):
+    print(f"solve parity shape={shape} inactive={inactive} rtol={rtol}", flush=True)
+    g, active, fixed, Ee = random_problem(shape, seed, inactive)
+    ref = oracle_fem(g, active, fixed)
+    K = ref.assemble(Ee[ref.act])
+    rng = np.random.default_rng(seed)
+    Fs = [rng.standard_normal(ref.nfree), rng.standard_normal(ref.nfree)]
+    fem = _product_fem(g, active, fixed)
+    Kd = fem.assemble(Ee[fem.act])
+    us = fem.solve(Kd, Fs, rtol=rtol)
+    assert len(fem.cg_iters) == 2
+    for f, u in zip(Fs, us):
+        res = np.linalg.norm(f - K @ u) / np.linalg.norm(f)
+        assert res <= rtol * 1.0001, res
+    uref = ref.solve(K, Fs, rtol=1e-12)
+    for u, v in zip(us, uref):
+        # energy-norm error is bounded by the residual: compare compliances
+        c, cref = Fs[0] @ u, Fs[0] @ v
+        assert abs(c - cref) <= 1e2 * rtol * abs(cref) + 1e-300
+
+
+def test_solve_dense_bruteforce(torch):
+    """Tiny grid: against numpy.linalg.solve of the assembled matrix."""
+    g, active, fixed, Ee = random_problem((4, 2, 2), 3, 0.0, "random")
+    ref = oracle_fem(g, active, fixed)
+    A = ref.assemble(Ee[ref.act]).toarray()
+    f = np.random.default_rng(1).standard_normal(ref.nfree)
+    exact = np.linalg.solve(A, f)
+    fem = _product_fem(g, active, fixed)
+    u = fem.solve(fem.assemble(Ee[fem.act]), [f], rtol=1e-12)[0]
+    assert np.abs(u - exact).max() <= 1e-8 * np.abs(exact).max()
+
+
+def test_warm_start_and_errors(torch):
+    from anneal_amd.fem import SolverError
+    g, active, fixed, Ee = random_problem((16, 8, 8), 5)
+    fem = _product_fem(g, active, fixed)
+    K = fem.assemble(Ee[fem.act])
+    f = np.random.default_rng(2).standard_normal(fem.nfree)
+    u = fem.solve(K, [f], rtol=1e-8)[0]
+    n_cold = fem.cg_iters[-1]
+    fem.solve(K, [f], x0_list=[u], rtol=1e-8)
+    assert fem.cg_iters[-1] <= 1 < n_cold + 1
+    with pytest.raises(SolverError, match="did not converge in 1 iterations"):
+        fem.solve(K, [f], rtol=1e-12, maxiter=1)
+
+
+def test_floating_structure_raises(torch):
+    """No fixed dofs: the system is singular; the reference raises SolverError (fem.py:225-229)."""
+    from anneal_amd.fem import SolverError
+    g, active, _, Ee = random_problem((6, 3, 3), 6)
+    fem = _product_fem(g, active, np.zeros(0, int))
+    f = np.random.default_rng(3).standard_normal(fem.nfree)
+    with pytest.raises(SolverError):
+        fem.solve(fem.assemble(Ee[fem.act]), [f], rtol=1e-8, maxiter=200)
+
+
+def test_element_energy(torch):
+    g, active, fixed, Ee = random_problem((9, 7, 5), 8, 0.2)
+    ref = oracle_fem(g, active, fixed)
+    fem = _product_fem(g, active, fixed)
+    u = np.random.default_rng(4).standard_normal(3 * g.nnode)
+    want = ref.element_energy(u)
+    got = fem.element_energy(u)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
