@@ -120,174 +120,209 @@ def _schedule_p(it, iters, p_final, warm):
     return p_final if warm else min(p_final, 1.0 + 0.25 * ((it - 1) // hold))
 
 
+class SimpRun:
+    """One SIMP optimisation (optimize.py:50-180) with device-resident state, stepped one iteration at a time.
+
+    ``simp()`` is ``SimpRun(...).run()``; benchmarks call :meth:`step` to time
+    individual iterations.  Arguments are those of ``simp``.
+    """
+
+    def __init__(self, grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
+                 iters=80, r_mm=None, p_final=3.0, move=0.2, combine="sum", log=print, tol=0.01, x0=None,
+                 beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None):
+        torch = _torch()
+        self.t_start = time.time()
+        self.torch = torch
+        self.grid = grid
+        self.emin = EMIN if emin is None else float(emin)
+        self.E0, self.vf, self.iters, self.p_final, self.move = float(E0), vf, iters, p_final, move
+        self.combine, self.log, self.tol, self.beta_max = combine, log, tol, beta_max
+        self.rtol, self.maxiter = rtol, maxiter
+        fill = np.asarray(fill, float).ravel()
+        active = np.asarray(active).astype(bool).ravel()
+        frozen = np.asarray(frozen).astype(bool).ravel()
+        self.active = active
+        self.cases = list(loads)
+        if fem is None:
+            fem = FEM(grid, active, fixed, nu, log=log, nrhs_max=max(1, len(self.cases)))
+        self.fem = fem
+        act = fem.act
+        dev = fem.device
+        phi_h = fill[act]
+        frz_h = frozen[act]
+        des_h = ~frz_h
+        h = grid.h
+        self.r_vox = max((r_mm or 1.5 * h) / h, 1.5)
+        self.filt = DeviceFilter(grid, active, self.r_vox, frozen)
+        Fr = [fem.reduce_load(loads[c]) for c in self.cases]
+        self.w = np.array([weights[c] for c in self.cases], dtype=float)
+        if all(np.allclose(f, 0) for f in Fr):
+            raise ValueError("All loads are zero after mapping onto the grid. Check the load anchors' regions.")
+        self.Vtot = phi_h.sum()
+        self.Vtarget = vf * self.Vtot
+        Vfrozen = phi_h[frz_h].sum()
+        x_h = np.ones(act.size)
+        x_h[des_h] = np.clip((self.Vtarget - Vfrozen) / max(phi_h[des_h].sum(), 1e-12), 0.01, 1.0)
+        self.warm = x0 is not None
+        if self.warm:       # optimize.py:72-77
+            xs = np.clip(np.asarray(x0, float).ravel()[act], 0.0, 1.0)
+            cur = (phi_h[des_h] * xs[des_h]).sum()
+            if cur > 0:
+                x_h[des_h] = np.clip(xs[des_h] * (self.Vtarget - Vfrozen) / cur, 0.001, 1.0)
+        nel = self.nel = grid.nelem
+
+        def full_dev(v_act):
+            f = np.zeros(nel)
+            f[act] = v_act
+            return torch.from_numpy(f).to(dev)
+
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.x = full_dev(x_h)
+        self.phi = full_dev(phi_h)
+        self.xphys, self.dxp = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
+        self.Ee = torch.zeros(nel, **f64)
+        self.dcdx, self.dvdx = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
+        self.dcx, self.dvx = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
+        self.xnew = torch.zeros(nel, **f64)
+        self.stats = torch.zeros(8, **f64)
+        self.work = torch.zeros(4096, **f64)
+        self.ncase = len(self.cases)
+        self.F = fem._to_device(Fr)
+        self.U = torch.zeros_like(self.F)
+        self.cs_dev = torch.zeros(self.ncase, **f64)
+        self.coef_dev = torch.zeros(self.ncase, **f64)
+        self.stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.T = timers if timers is not None else {}
+        for k in ("physical", "solve", "sensitivity", "oc", "stats"):
+            self.T.setdefault(k, 0.0)
+        self.beta = beta_max / 2 if self.warm else 0.0      # optimize.py:79, 100-101
+        self.hist = []
+        self.c0 = None
+        self.base_iters = None
+        self.have_u = False
+        self.filt.physical(self.x, self.beta, self.phi, self.xphys, self.dxp)
+        self.beta_step = max(5, int(0.12 * iters))
+        self.beta_since = 0
+        self.it = 0
+        self.converged = False
+
+    def _tick(self):
+        self.torch.cuda.synchronize()
+        return time.perf_counter()
+
+    def step(self):
+        """One SIMP iteration (optimize.py:105-175); returns its history entry."""
+        torch, fem, filt, T = self.torch, self.fem, self.filt, self.T
+        self.it += 1
+        it = self.it
+        t0 = time.time()
+        p = _schedule_p(it, self.iters, self.p_final, self.warm)
+        if p >= self.p_final and not self.warm and self.beta_max > 0:        # optimize.py:111-116
+            if self.beta == 0:
+                self.beta, self.beta_since = 1.0, 0
+            elif self.beta_since >= self.beta_step and self.beta < self.beta_max:
+                self.beta, self.beta_since = min(self.beta_max, 2 * self.beta), 0
+            self.beta_since += 1
+        beta = self.beta
+        nel, flags, stream = self.nel, filt.flags, self.stream
+        ta = self._tick()
+        filt.physical(self.x, beta, self.phi, self.xphys, self.dxp)
+        _lib.call("anneal_stiffness_law", _lib.ptr(self.xphys), _lib.ptr(self.phi), _lib.ptr(flags), self.E0,
+                  self.emin, float(p), nel, _lib.ptr(self.Ee), stream)
+        fem.assemble_device(self.Ee)
+        tb = self._tick()
+        T["physical"] += tb - ta
+        last = fem.cg_iters[-1] if fem.cg_iters else None
+        rebuild = self.base_iters is None or (last is not None and last > 1.6 * self.base_iters)
+        if rebuild:
+            fem.amg_builds += 1
+        its = fem.solve_device(self.F, self.U, self.ncase, self.rtol, self.maxiter, use_x0=self.have_u)
+        fem.cg_iters.extend(its)
+        self.have_u = True
+        if rebuild:
+            self.base_iters = fem.cg_iters[-1]
+        _lib.call("anneal_dot_f64", _lib.ptr(self.F), _lib.ptr(self.U), self.grid.nnode, fem.ldn, self.ncase,
+                  _lib.ptr(self.cs_dev), _lib.ptr(self.work), stream)
+        cs = self.cs_dev.cpu().numpy()
+        tc = self._tick()
+        T["solve"] += tc - tb
+        if self.c0 is None:
+            self.c0 = cs.copy()
+        w = self.w
+        if self.combine == "sum" or self.ncase == 1:                          # optimize.py:132-139
+            obj = float(w @ cs)
+            coef = w
+        else:
+            r = w * cs / self.c0
+            obj = float((r ** PNORM).sum() ** (1 / PNORM))
+            coef = (r / obj) ** (PNORM - 1) * w / self.c0
+        self.coef_dev.copy_(torch.from_numpy(np.asarray(coef, float)))
+        _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(self.U), self.ncase, _lib.ptr(flags),
+                  _lib.ptr(self.xphys), _lib.ptr(self.dxp), _lib.ptr(self.phi), _lib.ptr(self.coef_dev), float(p),
+                  self.emin, self.E0, _lib.ptr(self.dcdx), _lib.ptr(self.dvdx))
+        filt.backward(self.dcdx, self.dcx)
+        filt.backward(self.dvdx, self.dvx)
+        td = self._tick()
+        T["sensitivity"] += td - tc
+        # OC (optimize.py:144-163)
+        _lib.call("anneal_design_stats", None, None, None, _lib.ptr(self.phi), _lib.ptr(self.dcx), _lib.ptr(self.dvx),
+                  _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work), stream)
+        st = self.stats.cpu().numpy()
+        ndes = st[3]
+        scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
+        steps = filt.oc_bisect(self.x, self.dcx, self.dvx, self.phi, beta, self.move, scale * 1e-8, scale * 1e8,
+                               self.Vtarget, self.xnew)
+        te = self._tick()
+        T["oc"] += te - td
+        filt.physical(self.xnew, beta, self.phi, self.xphys, self.dxp)
+        _lib.call("anneal_design_stats", _lib.ptr(self.x), _lib.ptr(self.xnew), _lib.ptr(self.xphys),
+                  _lib.ptr(self.phi), None, None, _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work),
+                  stream)
+        st = self.stats.cpu().numpy()
+        self.x, self.xnew = self.xnew, self.x
+        change = float(st[2]) if ndes > 0 else 0.0
+        grey = float(st[1] / ndes) if ndes > 0 else 0.0
+        volfrac = float(st[0] / self.Vtot)
+        T["stats"] += self._tick() - te
+        ncase = self.ncase
+        h = {"iter": it, "objective": obj, "compliance": cs.tolist(), "p": p, "beta": beta,
+             "volumeFraction": volfrac, "change": change, "grey": grey,
+             "cgIters": fem.cg_iters[-ncase:], "amgRebuilt": bool(rebuild),
+             "seconds": time.time() - t0, "ocSteps": steps}
+        self.hist.append(h)
+        self.log(f"  it {it:3d}  p {p:.2f}  b {beta:.0f}  obj {obj:.4g}  vol {volfrac:.3f}  "
+                 f"change {change:.3f}  grey {grey:.2f}  cg {fem.cg_iters[-1]}{' amg' if rebuild else ''}  "
+                 f"{h['seconds']:.1f}s")
+        if p >= self.p_final and beta >= self.beta_max and it > (5 if self.warm else 10) and change < self.tol:
+            self.log(f"  converged at iteration {it}")
+            self.converged = True
+        return h
+
+    def result(self):
+        full = self.xphys.cpu().numpy().copy()
+        full[~self.active] = 0.0
+        return {"x": full, "history": self.hist, "fem_builds": self.fem.amg_builds, "rMM": self.r_vox * self.grid.h,
+                "seconds": time.time() - self.t_start, "cases": self.cases}
+
+    def run(self):
+        while self.it < self.iters and not self.converged:
+            self.step()
+        return self.result()
+
+
 def simp(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
          iters=80, r_mm=None, p_final=3.0, move=0.2, combine="sum", log=print, tol=0.01, x0=None,
          beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None):
-    """SIMP loop of optimize.py:50-180 with device-resident state.
+    """SIMP loop of optimize.py:50-180 on the GPU; returns the reference's result dict.
 
     Extra keyword-only arguments (defaults reproduce the reference): ``emin``
     (module ``EMIN`` when None, optimize.py:18), ``rtol`` of the inner solves
     (optimize.py:122 uses 1e-6), ``maxiter`` (fem.py:206), an existing ``fem``
     context to reuse, and ``timers`` (a dict that receives per-phase seconds).
     """
-    torch = _torch()
-    t_start = time.time()
-    emin = EMIN if emin is None else float(emin)
-    fill = np.asarray(fill, float).ravel()
-    active = np.asarray(active).astype(bool).ravel()
-    frozen = np.asarray(frozen).astype(bool).ravel()
-    cases = list(loads)
-    if fem is None:
-        fem = FEM(grid, active, fixed, nu, log=log, nrhs_max=max(1, len(cases)))
-    act = fem.act
-    dev = fem.device
-    phi_h = fill[act]
-    frz_h = frozen[act]
-    des_h = ~frz_h
-    h = grid.h
-    r_vox = max((r_mm or 1.5 * h) / h, 1.5)
-    filt = DeviceFilter(grid, active, r_vox, frozen)
-    Fr = [fem.reduce_load(loads[c]) for c in cases]
-    w = np.array([weights[c] for c in cases], dtype=float)
-    if all(np.allclose(f, 0) for f in Fr):
-        raise ValueError("All loads are zero after mapping onto the grid. Check the load anchors' regions.")
-
-    Vtot = phi_h.sum()
-    Vtarget = vf * Vtot
-    Vfrozen = phi_h[frz_h].sum()
-    x_h = np.ones(act.size)
-    x_h[des_h] = np.clip((Vtarget - Vfrozen) / max(phi_h[des_h].sum(), 1e-12), 0.01, 1.0)
-    warm = x0 is not None
-    if warm:
-        xs = np.clip(np.asarray(x0, float).ravel()[act], 0.0, 1.0)
-        cur = (phi_h[des_h] * xs[des_h]).sum()
-        if cur > 0:
-            x_h[des_h] = np.clip(xs[des_h] * (Vtarget - Vfrozen) / cur, 0.001, 1.0)
-
-    nel = grid.nelem
-
-    def full_dev(v_act):
-        f = np.zeros(nel)
-        f[act] = v_act
-        return torch.from_numpy(f).to(dev)
-
-    f64 = dict(dtype=torch.float64, device=dev)
-    x = full_dev(x_h)
-    phi = full_dev(phi_h)
-    xphys, dxp = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
-    Ee = torch.zeros(nel, **f64)
-    dcdx, dvdx = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
-    dcx, dvx = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
-    xnew = torch.zeros(nel, **f64)
-    stats = torch.zeros(8, **f64)
-    work = torch.zeros(4096, **f64)
-    ncase = len(cases)
-    F = fem._to_device(Fr)
-    U = torch.zeros_like(F)
-    cs_dev = torch.zeros(ncase, **f64)
-    coef_dev = torch.zeros(ncase, **f64)
-    stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
-    flags = filt.flags
-    T = timers if timers is not None else {}
-    for k in ("physical", "solve", "sensitivity", "oc", "stats"):
-        T.setdefault(k, 0.0)
-
-    def tick():
-        torch.cuda.synchronize()
-        return time.perf_counter()
-
-    beta = beta_max / 2 if warm else 0.0
-    hist = []
-    c0 = None
-    base_iters = None
-    have_u = False
-    filt.physical(x, beta, phi, xphys, dxp)
-    beta_step = max(5, int(0.12 * iters))
-    beta_since = 0
-    p = 1.0
-    for it in range(1, iters + 1):
-        t0 = time.time()
-        p = _schedule_p(it, iters, p_final, warm)
-        if p >= p_final and not warm and beta_max > 0:         # optimize.py:111-116
-            if beta == 0:
-                beta, beta_since = 1.0, 0
-            elif beta_since >= beta_step and beta < beta_max:
-                beta, beta_since = min(beta_max, 2 * beta), 0
-            beta_since += 1
-        ta = tick()
-        filt.physical(x, beta, phi, xphys, dxp)
-        _lib.call("anneal_stiffness_law", _lib.ptr(xphys), _lib.ptr(phi), _lib.ptr(flags), float(E0), emin,
-                  float(p), nel, _lib.ptr(Ee), stream)
-        K = fem.assemble_device(Ee)
-        tb = tick()
-        T["physical"] += tb - ta
-        last = fem.cg_iters[-1] if fem.cg_iters else None
-        rebuild = base_iters is None or (last is not None and last > 1.6 * base_iters)
-        if rebuild:
-            fem.amg_builds += 1
-        its = fem.solve_device(F, U, ncase, rtol, maxiter, use_x0=have_u)
-        fem.cg_iters.extend(its)
-        have_u = True
-        if rebuild:
-            base_iters = fem.cg_iters[-1]
-        _lib.call("anneal_dot_f64", _lib.ptr(F), _lib.ptr(U), grid.nnode, fem.ldn, ncase, _lib.ptr(cs_dev),
-                  _lib.ptr(work), stream)
-        cs = cs_dev.cpu().numpy()
-        tc = tick()
-        T["solve"] += tc - tb
-        if c0 is None:
-            c0 = cs.copy()
-        if combine == "sum" or ncase == 1:
-            obj = float(w @ cs)
-            coef = w
-        else:
-            r = w * cs / c0
-            obj = float((r ** PNORM).sum() ** (1 / PNORM))
-            coef = (r / obj) ** (PNORM - 1) * w / c0
-        coef_dev.copy_(torch.from_numpy(np.asarray(coef, float)))
-        _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(U), ncase, _lib.ptr(flags), _lib.ptr(xphys),
-                  _lib.ptr(dxp), _lib.ptr(phi), _lib.ptr(coef_dev), float(p), emin, float(E0), _lib.ptr(dcdx),
-                  _lib.ptr(dvdx))
-        filt.backward(dcdx, dcx)
-        filt.backward(dvdx, dvx)
-        td = tick()
-        T["sensitivity"] += td - tc
-        # OC (optimize.py:144-163)
-        _lib.call("anneal_design_stats", None, None, None, _lib.ptr(phi), _lib.ptr(dcx), _lib.ptr(dvx),
-                  _lib.ptr(flags), nel, _lib.ptr(stats), _lib.ptr(work), stream)
-        st = stats.cpu().numpy()
-        ndes = st[3]
-        if ndes > 0:
-            scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300)
-        else:
-            scale = 1.0
-        steps = filt.oc_bisect(x, dcx, dvx, phi, beta, move, scale * 1e-8, scale * 1e8, Vtarget, xnew)
-        te = tick()
-        T["oc"] += te - td
-        filt.physical(xnew, beta, phi, xphys, dxp)
-        _lib.call("anneal_design_stats", _lib.ptr(x), _lib.ptr(xnew), _lib.ptr(xphys), _lib.ptr(phi), None, None,
-                  _lib.ptr(flags), nel, _lib.ptr(stats), _lib.ptr(work), stream)
-        st = stats.cpu().numpy()
-        x, xnew = xnew, x
-        change = float(st[2]) if ndes > 0 else 0.0
-        grey = float(st[1] / ndes) if ndes > 0 else 0.0
-        volfrac = float(st[0] / Vtot)
-        T["stats"] += tick() - te
-        hist.append({"iter": it, "objective": obj, "compliance": cs.tolist(), "p": p, "beta": beta,
-                     "volumeFraction": volfrac, "change": change, "grey": grey,
-                     "cgIters": fem.cg_iters[-ncase:], "amgRebuilt": bool(rebuild),
-                     "seconds": time.time() - t0, "ocSteps": steps})
-        log(f"  it {it:3d}  p {p:.2f}  b {beta:.0f}  obj {obj:.4g}  vol {volfrac:.3f}  "
-            f"change {change:.3f}  grey {grey:.2f}  cg {fem.cg_iters[-1]}{' amg' if rebuild else ''}  "
-            f"{hist[-1]['seconds']:.1f}s")
-        if p >= p_final and beta >= beta_max and it > (5 if warm else 10) and change < tol:
-            log(f"  converged at iteration {it}")
-            break
-
-    full = xphys.cpu().numpy().copy()
-    full[~active] = 0.0
-    return {"x": full, "history": hist, "fem_builds": fem.amg_builds, "rMM": r_vox * h,
-            "seconds": time.time() - t_start, "cases": cases}
+    return SimpRun(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf, iters=iters, r_mm=r_mm,
+                   p_final=p_final, move=move, combine=combine, log=log, tol=tol, x0=x0, beta_max=beta_max,
+                   emin=emin, rtol=rtol, maxiter=maxiter, fem=fem, timers=timers).run()
 
 
 def benchmark_cantilever(log=lambda *a: None):
