@@ -113,6 +113,29 @@ KeCoef<double> walsh_coefficients(const double* ke) {
   return k;
 }
 
+// lambda_max(ke) / ke_00 by power iteration (ke is symmetric positive semi-definite
+// with a constant diagonal).
+double element_jacobi_bound(const double* ke) {
+  double v[24], w[24];
+  for (int i = 0; i < 24; ++i) v[i] = 1.0 + 0.01 * i * (i % 3 == 0 ? 1.0 : -1.0);
+  double lam = 0.0;
+  for (int it = 0; it < 5000; ++it) {
+    double nrm = 0.0;
+    for (int i = 0; i < 24; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < 24; ++j) s += ke[i * 24 + j] * v[j];
+      w[i] = s;
+      nrm += s * s;
+    }
+    nrm = std::sqrt(nrm);
+    double vn = 0.0;
+    for (int i = 0; i < 24; ++i) vn += v[i] * v[i];
+    lam = nrm / std::sqrt(vn);
+    for (int i = 0; i < 24; ++i) v[i] = w[i] / nrm;
+  }
+  return lam / ke[0];
+}
+
 template <typename T>
 static KeCoef<T> cast_coef(const KeCoef<double>& k, double s) {
   return KeCoef<T>{T(k.nAmB * s), T(k.nB * s), T(k.g * s), T(k.pc * s), T(k.pd * s), T(k.tAmB * s), T(k.tB * s),
@@ -136,6 +159,7 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
   try {
     hex8_stiffness(nu, h, ke_);
     k64_ = walsh_coefficients(ke_);
+    cheb_lmax = 1.001 * element_jacobi_bound(ke_);
     // Level geometry
     std::vector<GridDims> dims{GridDims{nx, ny, nz}};
     while (true) {
@@ -311,16 +335,30 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
     return;
   }
   const int64_t n = L.g.nnode(), ld = L.g.ldn();
-  // 2*nu_pre - 1 Jacobi sweeps after the first: start in x2 so the result lands in x.
+  // Chebyshev recurrence (three-term form): x_{k+1} = x_k + c1_k (x_k - x_{k-1}) + c2_k D^-1 (b - K x_k)
+  const double lmax = cheb_lmax, lmin = lmax / cheb_ratio;
+  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sg = th / de;
+  double c1[16], c2[16];
+  double rho = 1.0 / sg;
+  for (int k = 1; k < nu_pre && k < 16; ++k) {
+    const double rn = 1.0 / (2.0 * sg - rho);
+    c1[k] = rn * rho;
+    c2[k] = 2.0 * rn / de;
+    rho = rn;
+  }
+  // 2*nu_pre - 1 buffer swaps in total: start in x2 so the result lands in x.
   float* cur = L.x2;
   float* nxt = x;
-  jacobi0(b, cur, L.dinv, L.mask, omega, n, ld, nrhs, s_);
+  jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
-  for (int s = 1; s < nu_pre; ++s) {
+  for (int k = 1; k < nu_pre; ++k) {
     a.u = cur;
+    a.prev = (k == 1) ? nullptr : nxt;
     a.y = nxt;
-    launch_apply(a, kJacobi, nrhs, s_);
+    a.c1 = float(c1[k]);
+    a.c2 = float(c2[k]);
+    launch_apply(a, kCheb, nrhs, s_);
     std::swap(cur, nxt);
   }
   a.u = cur;
@@ -330,10 +368,18 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_);
   vcycle(l + 1, C.b, C.x, nrhs);
   launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_);
-  for (int s = 0; s < nu_pre; ++s) {
+  a.u = cur;
+  a.y = nxt;
+  a.omega = float(1.0 / th);
+  launch_apply(a, kJacobi, nrhs, s_);
+  std::swap(cur, nxt);
+  for (int k = 1; k < nu_pre; ++k) {
     a.u = cur;
+    a.prev = nxt;
     a.y = nxt;
-    launch_apply(a, kJacobi, nrhs, s_);
+    a.c1 = float(c1[k]);
+    a.c2 = float(c2[k]);
+    launch_apply(a, kCheb, nrhs, s_);
     std::swap(cur, nxt);
   }
   if (cur != x) ANNEAL_HIP(hipMemcpyAsync(x, cur, (size_t)nrhs * 3 * ld * sizeof(float), hipMemcpyDeviceToDevice, s_));

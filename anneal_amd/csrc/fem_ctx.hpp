@@ -15,6 +15,8 @@ namespace anneal {
 void hex8_stiffness(double nu, double h, double* ke /*576*/);
 // Walsh-domain coefficients of ke; throws if the block pattern does not hold.
 KeCoef<double> walsh_coefficients(const double* ke);
+// lambda_max(ke) / ke_00: upper bound of lambda_max(D^-1 K) for any Ee >= 0.
+double element_jacobi_bound(const double* ke);
 
 struct Level {
   GridDims g{};
@@ -58,8 +60,15 @@ class FemContext {
   const KeCoef<double>& coef64() const { return k64_; }
   const Level& level(int l) const { return lv_[l]; }
 
-  int nu_pre = 2;        // Jacobi sweeps per side
-  float omega = 0.6f;    // Jacobi damping
+  // Chebyshev smoother of degree nu_pre on [lmax / cheb_ratio, lmax] of D^-1 K.
+  // lmax is the element bound lambda_max(ke) / ke_ii, an upper bound of
+  // lambda_max(D^-1 K) for any stiffness scales (x^T K x = sum_e Ee x_e^T ke x_e
+  // <= bound * sum_e Ee x_e^T diag(ke) x_e = bound * x^T D x), so the V-cycle
+  // stays symmetric positive definite however heterogeneous Ee is.
+  int nu_pre = 2;
+  double cheb_ratio = 15.0;
+  double cheb_lmax = 0.0;  // set from ke at construction
+  float omega = 1.0f;      // unused by the V-cycle; kept for kJacobi callers
   double inner_tol_floor = 1e-5;
   int inner_max = 400;
 

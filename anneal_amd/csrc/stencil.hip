@@ -42,7 +42,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
   const int64_t ldn = a.ldn;
   const T* __restrict__ u = a.u + (int64_t)3 * r * ldn;
-  T* __restrict__ y = a.y + (int64_t)3 * r * ldn;
+  T* y = a.y + (int64_t)3 * r * ldn;  // not restrict: kCheb may write over prev
   const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
 
   const int ei = X0 - 1 + tx, ej = Y0 - 1 + ty;
@@ -145,7 +145,12 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
               out = bb[c * ldn + n] - v;
             } else {
               const T uc = prevbuf[c * LX * LY + o];
-              out = uc + a.omega * T(a.dinv[n]) * (bb[c * ldn + n] - v);
+              if (MODE == kJacobi) {
+                out = uc + a.omega * T(a.dinv[n]) * (bb[c * ldn + n] - v);
+              } else {
+                const T pv = a.prev ? a.prev[(int64_t)(3 * r + c) * ldn + n] : T(0);
+                out = uc + a.c1 * (uc - pv) + a.c2 * T(a.dinv[n]) * (bb[c * ldn + n] - v);
+              }
             }
             if (!((mk >> c) & 1)) out = T(0);
             y[c * ldn + n] = out;
@@ -307,6 +312,7 @@ void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s) {
     case kApply: launch_apply_mode<T, kApply>(a, nrhs, s); break;
     case kResid: launch_apply_mode<T, kResid>(a, nrhs, s); break;
     case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, s); break;
+    case kCheb: launch_apply_mode<T, kCheb>(a, nrhs, s); break;
     default: throw std::invalid_argument("bad apply mode");
   }
 }

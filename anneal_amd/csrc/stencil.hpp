@@ -14,6 +14,7 @@ enum ApplyMode : int {
   kApply = 0,   // y = M K u
   kResid = 1,   // y = M (b - K u)
   kJacobi = 2,  // y = M (u + omega * dinv * (b - K u))
+  kCheb = 3,    // y = M (u + c1 (u - prev) + c2 * dinv * (b - K u)); prev may be null (0) or alias y
 };
 
 template <typename T>
@@ -27,6 +28,8 @@ struct ApplyArgs {
   double* dot_part;    // optional: per-block partial of sum u.y  [nrhs][nblk_xyz]
   KeCoef<T> k;
   T omega;
+  const T* prev;       // kCheb: previous iterate (read at the node only, so it may alias y)
+  T c1, c2;            // kCheb recurrence coefficients
   int nx, ny, nz;
   int64_t ldn;
   int ntx, nty, nzc;   // tiles in x, y and z chunks

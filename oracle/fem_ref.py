@@ -142,13 +142,17 @@ def _prolongation_1d(n_fine):
 
 
 class _GalerkinVCycle:
-    """Symmetric V-cycle (damped Jacobi, Galerkin coarse operators, sparse LU at the bottom).
+    """Symmetric V-cycle (Chebyshev smoothing, Galerkin coarse operators, sparse LU at the bottom).
 
     Oracle-only preconditioner standing in for pyamg (fem.py:212-215), which is
-    absent: it changes the CG iteration count, not the converged answer.
+    absent: it changes the CG iteration count, not the converged answer.  The
+    smoother is a degree-2 Chebyshev polynomial in D^-1 A on [lmax/15, lmax]
+    with lmax 1.1x a Lanczos estimate, which keeps the cycle positive definite
+    for strongly heterogeneous stiffness (damped Jacobi with a fixed weight
+    does not: lambda_max(D^-1 A) ranges from about 3.1 to 5.3 for hex8).
     """
 
-    def __init__(self, A, grid_shape, free, sweeps=2, omega=0.6, coarse_limit=4000):
+    def __init__(self, A, grid_shape, free, sweeps=2, ratio=15.0, coarse_limit=4000):
         self.levels = []
         shape = tuple(grid_shape)
         keep = free          # fine-level DOF ids (into 3*nnode_full of `shape`)
@@ -165,22 +169,34 @@ class _GalerkinVCycle:
             P = P[:, used]
             Ac = (P.T @ A @ P).tocsr()
             dinv = 1.0 / A.diagonal()
-            self.levels.append((A, P, dinv))
+            dh = sp.diags(np.sqrt(dinv))
+            lmax = 1.1 * spla.eigsh(dh @ A @ dh, k=1, which="LA", return_eigenvectors=False, tol=1e-3)[0]
+            self.levels.append((A, P, dinv, lmax))
             A, shape, keep = Ac, tuple(cshape), used
         self.coarse = spla.splu(A.tocsc())
-        self.sweeps, self.omega = sweeps, omega
+        self.sweeps, self.ratio = sweeps, ratio
+
+    def _smooth(self, A, dinv, lmax, b, x):
+        lmin = lmax / self.ratio
+        th, de = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+        sg = th / de
+        rho = 1.0 / sg
+        d = dinv * (b if x is None else b - A @ x) / th
+        x = d if x is None else x + d
+        for _ in range(self.sweeps - 1):
+            rn = 1.0 / (2.0 * sg - rho)
+            d = rn * rho * d + (2.0 * rn / de) * dinv * (b - A @ x)
+            rho = rn
+            x = x + d
+        return x
 
     def _cycle(self, lvl, b):
         if lvl == len(self.levels):
             return self.coarse.solve(b)
-        A, P, dinv = self.levels[lvl]
-        x = self.omega * dinv * b
-        for _ in range(self.sweeps - 1):
-            x += self.omega * dinv * (b - A @ x)
-        x += P @ self._cycle(lvl + 1, P.T @ (b - A @ x))
-        for _ in range(self.sweeps):
-            x += self.omega * dinv * (b - A @ x)
-        return x
+        A, P, dinv, lmax = self.levels[lvl]
+        x = self._smooth(A, dinv, lmax, b, None)
+        x = x + P @ self._cycle(lvl + 1, P.T @ (b - A @ x))
+        return self._smooth(A, dinv, lmax, b, x)
 
     def operator(self, n):
         return spla.LinearOperator((n, n), matvec=lambda b: self._cycle(0, np.asarray(b).ravel()))

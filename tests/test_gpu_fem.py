@@ -73,7 +73,8 @@ def test_apply_is_deterministic(torch):
     _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(a), 1)
     _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(b), 1)
     torch.cuda.synchronize()
-    assert torch.equal(a, b)
+    nn = g.nnode                      # entries past nnode are padding the kernel never writes
+    assert torch.equal(a[..., :nn], b[..., :nn])
 
 
 @pytest.mark.parametrize("shape,inactive,seed,rtol", [
@@ -104,7 +105,7 @@ def test_solve_matches_oracle(torch, shape, inactive, seed, rtol):
 
 def test_solve_dense_bruteforce(torch):
     """Tiny grid: against numpy.linalg.solve of the assembled matrix."""
-    g, active, fixed, Ee = random_problem((4, 2, 2), 3, 0.0, "random")
+    g, active, fixed, Ee = random_problem((4, 2, 2), 3, 0.0, "clamp")
     ref = oracle_fem(g, active, fixed)
     A = ref.assemble(Ee[ref.act]).toarray()
     f = np.random.default_rng(1).standard_normal(ref.nfree)

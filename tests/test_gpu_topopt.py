@@ -141,9 +141,11 @@ def test_sensitivity_kernel(torch):
     flags = torch.from_numpy(np.asarray(active, np.uint8)).cuda()
     DC, DV = torch.zeros(g.nelem, dtype=torch.float64, device="cuda"), torch.zeros(g.nelem, dtype=torch.float64,
                                                                                      device="cuda")
-    _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(U), 2, _lib.ptr(flags), _lib.ptr(full(xphys)),
-              _lib.ptr(full(dxp)), _lib.ptr(full(phi)), _lib.ptr(torch.from_numpy(coef).cuda()), p, emin, E0,
-              _lib.ptr(DC), _lib.ptr(DV))
+    # keep every device buffer referenced until the kernel has run
+    X, DXP, PHI, COEF = full(xphys), full(dxp), full(phi), torch.from_numpy(coef).cuda()
+    _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(U), 2, _lib.ptr(flags), _lib.ptr(X), _lib.ptr(DXP),
+              _lib.ptr(PHI), _lib.ptr(COEF), p, emin, E0, _lib.ptr(DC), _lib.ptr(DV))
+    torch.cuda.synchronize()
     assert np.abs(DC.cpu().numpy()[act] - dc * dxp).max() <= 1e-12 * np.abs(dc * dxp).max()
     assert np.abs(DV.cpu().numpy()[act] - phi * dxp).max() == 0.0
 
