@@ -27,6 +27,8 @@ SIGNATURES = {
     "anneal_fem_apply_f32": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_apply_f64": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_solve": [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_intp],
+    "anneal_fem_autotune": [c_void_p, c_int, c_intp, c_void_p, c_intp],
+    "anneal_fem_set_option": [c_void_p, ctypes.c_char_p, c_double],
     "anneal_fem_vcycle": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_element_energy": [c_void_p, c_void_p, c_int, c_void_p],
     "anneal_fem_sensitivity": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -37,7 +39,7 @@ SIGNATURES = {
     "anneal_filter_backward": [c_void_p, c_void_p, c_void_p],
     "anneal_filter_physical": [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "anneal_oc_bisect": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double,
-                         c_double, c_void_p, c_intp],
+                         c_double, c_double, c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],
     "anneal_stiffness_law": [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_int64, c_void_p,
                              c_void_p],
     "anneal_design_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,

@@ -2,6 +2,8 @@
 #include "common.hpp"
 #include "blas.hpp"
 
+#include <algorithm>
+
 namespace anneal {
 
 constexpr int kRedThreads = 256;
@@ -144,35 +146,97 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
   A[t] = v;
 }
 
-// One Gauss-Jordan step of an in-place inversion of an SPD matrix (no pivoting),
-// out-of-place between two buffers so every entry reads the pre-step state.
-__global__ void k_gj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * n) return;
-  const int i = t / n, j = t % n;
-  const double piv = A[(int64_t)k * n + k];
-  double v;
-  if (i == k && j == k) v = 1.0 / piv;
-  else if (i == k) v = A[t] / piv;
-  else if (j == k) v = -A[(int64_t)i * n + k] / piv;
-  else v = A[t] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] / piv;
-  B[t] = v;
+// Block Gauss-Jordan inversion of an SPD matrix (no pivoting), one launch per
+// pivot block K of up to kGJB rows, out of place between two buffers:
+//   new_KK = P = M_KK^-1, new_KJ = P M_KJ, new_IK = -M_IK P,
+//   new_IJ = M_IJ - (M_IK P) M_KJ.
+// Each workgroup inverts the small pivot block in LDS (redundantly) and
+// updates kGJB rows; the pivot rows M_K,: are staged in LDS.
+constexpr int kGJB = 16;
+__global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k0,
+                                                  int bk) {
+  extern __shared__ double sm[];
+  double* P = sm;                  // kGJB x kGJB
+  double* T = P + kGJB * kGJB;     // kGJB x kGJB: rows of this workgroup times P
+  double* MK = T + kGJB * kGJB;    // bk x n pivot rows
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * kGJB;
+  const int nr = min(kGJB, n - r0);
+  for (int idx = tid; idx < bk * n; idx += 256) {
+    const int l = idx / n, j = idx - l * n;
+    MK[idx] = A[(int64_t)(k0 + l) * n + j];
+  }
+  __syncthreads();
+  const int pi = tid / kGJB, pj = tid % kGJB;
+  const bool inP = pi < bk && pj < bk;
+  if (inP) P[pi * kGJB + pj] = MK[pi * n + k0 + pj];
+  __syncthreads();
+  for (int k = 0; k < bk; ++k) {
+    const double piv = P[k * kGJB + k];
+    const double aik = inP ? P[pi * kGJB + k] : 0.0, akj = inP ? P[k * kGJB + pj] : 0.0;
+    const double aij = inP ? P[pi * kGJB + pj] : 0.0;
+    __syncthreads();
+    if (inP) {
+      double v;
+      if (pi == k && pj == k) v = 1.0 / piv;
+      else if (pi == k) v = akj / piv;
+      else if (pj == k) v = -aik / piv;
+      else v = aij - aik * akj / piv;
+      P[pi * kGJB + pj] = v;
+    }
+    __syncthreads();
+  }
+  if (pi < nr && pj < bk) {
+    const int64_t row = r0 + pi;
+    double t = 0.0;
+    for (int l = 0; l < bk; ++l) t += A[row * n + k0 + l] * P[l * kGJB + pj];
+    T[pi * kGJB + pj] = t;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < nr * n; idx += 256) {
+    const int ri = idx / n, j = idx - ri * n;
+    const int row = r0 + ri;
+    const bool rk = row >= k0 && row < k0 + bk, ck = j >= k0 && j < k0 + bk;
+    double v;
+    if (rk && ck) {
+      v = P[(row - k0) * kGJB + (j - k0)];
+    } else if (rk) {
+      v = 0.0;
+      for (int l = 0; l < bk; ++l) v += P[(row - k0) * kGJB + l] * MK[l * n + j];
+    } else if (ck) {
+      v = -T[ri * kGJB + (j - k0)];
+    } else {
+      v = A[(int64_t)row * n + j];
+      for (int l = 0; l < bk; ++l) v -= T[ri * kGJB + l] * MK[l * n + j];
+    }
+    B[(int64_t)row * n + j] = v;
+  }
 }
 
 void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
                            int ny, int nz, hipStream_t s) {
   const int n = 3 * (nx + 1) * (ny + 1) * (nz + 1);
   const int64_t nn = (int64_t)n * n;
-  const unsigned g = (unsigned)((nn + 255) / 256);
-  hipLaunchKernelGGL(k_dense_assemble, dim3(g), dim3(256), 0, s, Ee, mask, ke, A, nx, ny, nz);
+  hipLaunchKernelGGL(k_dense_assemble, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, Ee, mask, ke, A, nx, ny,
+                     nz);
   ANNEAL_HIP(hipGetLastError());
+  const size_t lds = (size_t)(2 * kGJB * kGJB + kGJB * n) * sizeof(double);
+  ANNEAL_CHECK(lds <= 150 * 1024, "coarsest multigrid level too large for the dense inverse");
+  static bool attr = false;
+  if (!attr) {
+    ANNEAL_HIP(hipFuncSetAttribute((const void*)k_bgj_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   150 * 1024));
+    attr = true;
+  }
   double* src = A;
   double* dst = B;
-  for (int k = 0; k < n; ++k) {
-    hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, s, src, dst, n, k);
+  const unsigned g = (unsigned)((n + kGJB - 1) / kGJB);
+  for (int k0 = 0; k0 < n; k0 += kGJB) {
+    const int bk = std::min(kGJB, n - k0);
+    hipLaunchKernelGGL(k_bgj_step, dim3(g), dim3(256), lds, s, src, dst, n, k0, bk);
+    ANNEAL_HIP(hipGetLastError());
     std::swap(src, dst);
   }
-  ANNEAL_HIP(hipGetLastError());
   if (src != A) ANNEAL_HIP(hipMemcpyAsync(A, src, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
 }
 

@@ -17,7 +17,8 @@ struct ApplyArgs;
 
 template <typename T>
 void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s);
-int apply_blocks(int nx, int ny, int nz);
+int apply_blocks(int nx, int ny, int nz, int variant);
+int apply_variants();
 
 void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
                      hipStream_t s);

@@ -154,6 +154,44 @@ int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, do
   });
 }
 
+int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host) {
+  return guarded([&] {
+    FemHandle* h = fem_of(ctx);
+    const int c = h->fem->autotune(variant, us_host);
+    if (chosen_host) *chosen_host = c;
+    if (n_host) *n_host = apply_variants();
+    return 0;
+  });
+}
+
+int anneal_fem_set_option(void* ctx, const char* name, double value) {
+  return guarded([&] {
+    ANNEAL_CHECK(name, "null option name");
+    FemContext* f = fem_of(ctx)->fem;
+    const std::string n(name);
+    if (n == "graphs") {
+      f->use_graphs = value != 0.0;
+    } else if (n == "smoother_degree") {
+      ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree must be in [1, 15]");
+      f->nu_pre = (int)value;
+      f->autotune(f->tile(), nullptr);  // drops captured graphs
+    } else if (n == "cheb_ratio") {
+      ANNEAL_CHECK(value > 1.0, "cheb_ratio must exceed 1");
+      f->cheb_ratio = value;
+      f->autotune(f->tile(), nullptr);
+    } else if (n == "inner_tol_floor") {
+      ANNEAL_CHECK(value > 0.0 && value < 1.0, "inner_tol_floor must be in (0, 1)");
+      f->inner_tol_floor = value;
+    } else if (n == "inner_max") {
+      ANNEAL_CHECK(value >= 1, "inner_max must be >= 1");
+      f->inner_max = (int)value;
+    } else {
+      throw std::invalid_argument("unknown option: " + n);
+    }
+    return 0;
+  });
+}
+
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs) {
   return guarded([&] {
     ANNEAL_CHECK(r_dev && z_dev && r_dev != (const float*)z_dev, "vcycle needs distinct r and z");
@@ -227,12 +265,17 @@ int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const do
 
 int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
                      const double* phi_dev, double beta, double move, double lo, double hi, double vtarget,
-                     double* xnew_dev, int* steps_host) {
+                     double hint, double* xnew_dev, int* steps_host, int* evals_host, double* lam_last_host) {
   return guarded([&] {
     ANNEAL_CHECK(x_dev && dcx_dev && dvx_dev && phi_dev && xnew_dev, "null argument");
     ANNEAL_CHECK(lo > 0.0 && hi >= lo, "bisection bracket must satisfy 0 < lo <= hi");
-    const int s = filt_of(ctx)->oc_bisect(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, lo, hi, vtarget, xnew_dev);
+    int ev = 0;
+    double ll = 0.0;
+    const int s = filt_of(ctx)->oc_bisect(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, lo, hi, vtarget, xnew_dev,
+                                          hint, &ev, &ll);
     if (steps_host) *steps_host = s;
+    if (evals_host) *evals_host = ev;
+    if (lam_last_host) *lam_last_host = ll;
     return 0;
   });
 }

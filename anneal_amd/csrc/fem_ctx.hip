@@ -254,12 +254,15 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
     ANNEAL_HIP(hipMemset(pp_, 0, v32));
     ANNEAL_HIP(hipMemset(pq_, 0, v32));
     ANNEAL_HIP(hipMemset(r64_, 0, v64));
-    const int nblk = std::max(kDotBlocks, apply_blocks(g0.nx, g0.ny, g0.nz));
+    int nblk = kDotBlocks;
+    for (int v = 0; v < apply_variants(); ++v) nblk = std::max(nblk, apply_blocks(g0.nx, g0.ny, g0.nz, v));
     part_ = (double*)alloc((size_t)nblk * nrhs_max * sizeof(double));
     st_ = (double*)alloc(64 * kSt * sizeof(double));
     red_ = (double*)alloc(4 * 64 * sizeof(double));
     scale_ = (double*)alloc(64 * sizeof(double));
     ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
+    ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
+    vgraph_.assign(nrhs_max + 1, nullptr);
   } catch (...) {
     for (void* p : owned_) (void)hipFree(p);
     owned_.clear();
@@ -267,8 +270,47 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
   }
 }
 
+void FemContext::drop_graphs() {
+  for (auto& g : vgraph_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void FemContext::vcycle_pcg(int nrhs) {
+  if (!use_graphs) {
+    vcycle(0, pr_, pz_, nrhs);
+    return;
+  }
+  if (!vgraph_[nrhs]) {
+    hipStream_t saved = s_;
+    s_ = cap_s_;
+    hipGraph_t g = nullptr;
+    try {
+      ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
+      vcycle(0, pr_, pz_, nrhs);
+      ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
+    } catch (...) {
+      s_ = saved;
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(cap_s_, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    s_ = saved;
+    hipGraphExec_t exec = nullptr;
+    ANNEAL_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    vgraph_[nrhs] = exec;
+  }
+  ANNEAL_HIP(hipGraphLaunch(vgraph_[nrhs], s_));
+}
+
 FemContext::~FemContext() {
   (void)hipStreamSynchronize(s_);
+  drop_graphs();
+  if (cap_s_) (void)hipStreamDestroy(cap_s_);
   for (void* p : owned_) (void)hipFree(p);
   if (host_red_) (void)hipHostFree(host_red_);
 }
@@ -277,6 +319,7 @@ template <>
 ApplyArgs<float> FemContext::args<float>(int l) const {
   const Level& L = lv_[l];
   ApplyArgs<float> a{};
+  a.variant = tile_;
   a.Ee = L.Ee;
   a.mask = L.mask;
   a.dinv = L.dinv;
@@ -292,6 +335,7 @@ template <>
 ApplyArgs<double> FemContext::args<double>(int l) const {
   const Level& L = lv_[0];
   ApplyArgs<double> a{};
+  a.variant = 0;
   a.Ee = Ee64_;
   a.mask = L.mask;
   a.dinv = L.dinv;
@@ -311,6 +355,10 @@ void FemContext::set_Ee(const double* Ee_dev) {
   cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
   for (size_t l = 1; l < lv_.size(); ++l) launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, s_);
   for (size_t l = 0; l < lv_.size(); ++l) launch_inv_diag(lv_[l].Ee, lv_[l].mask, lv_[l].dinv, lv_[l].g, lv_[l].kdiag, s_);
+  if (!tuned_ && g0.nelem() >= 100000) {
+    tuned_ = true;
+    autotune(-1, nullptr);  // first operator on a large grid: pick the fastest tile shape
+  }
   const Level& C = lv_.back();
   dense_assemble_invert(C.Ee, C.mask, ke_coarse_dev_, Ainv_, Awork_, C.g.nx, C.g.ny, C.g.nz, s_);
 }
@@ -326,6 +374,48 @@ void FemContext::apply_f64(const double* u, double* y, int nrhs) {
   a.u = u;
   a.y = y;
   launch_apply(a, kApply, nrhs, s_);
+}
+
+int FemContext::autotune(int variant, float* us_out) {
+  const int nv = apply_variants();
+  drop_graphs();  // the tile shape is baked into captured V-cycles
+  if (variant >= 0) {
+    ANNEAL_CHECK(variant < nv, "unknown apply tile variant");
+    tile_ = variant;
+    return tile_;
+  }
+  // time the fine-level FP32 apply (one rhs) for every tile shape; keep the fastest
+  const GridDims& g0 = lv_[0].g;
+  hipEvent_t e0, e1;
+  ANNEAL_HIP(hipEventCreate(&e0));
+  ANNEAL_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int arg = 0;
+  for (int v = 0; v < nv; ++v) {
+    tile_ = v;
+    ApplyArgs<float> a = args<float>(0);
+    a.u = pp_;
+    a.y = pq_;
+    launch_apply(a, kApply, 1, s_);  // warm
+    ANNEAL_HIP(hipEventRecord(e0, s_));
+    const int reps = 10;
+    for (int i = 0; i < reps; ++i) launch_apply(a, kApply, 1, s_);
+    ANNEAL_HIP(hipEventRecord(e1, s_));
+    ANNEAL_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    ANNEAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const float us = 1e3f * ms / reps;
+    if (us_out) us_out[v] = us;
+    if (us < best) {
+      best = us;
+      arg = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)g0;
+  tile_ = arg;
+  return tile_;
 }
 
 void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
@@ -399,7 +489,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
   std::vector<int> its(nrhs, 0);
   std::vector<char> done(nrhs, 0), idone(nrhs, 0);
   std::vector<double> hst((size_t)nrhs * kSt, 0.0);
-  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz);
+  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, tile_);
   int status = 1;
   int total = 0;
   bool bad = false;
@@ -440,7 +530,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     ANNEAL_HIP(hipMemcpyAsync(st_, hst.data(), hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
-    vcycle(0, pr_, pz_, nrhs);
+    vcycle_pcg(nrhs);
     dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
     ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZ, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
                                 hipMemcpyDeviceToDevice, s_));
@@ -477,7 +567,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         ANNEAL_HIP(hipMemcpy2DAsync(st_ + kDone, kSt * sizeof(double), hst.data() + kDone, kSt * sizeof(double),
                                     sizeof(double), nrhs, hipMemcpyHostToDevice, s_));
       }
-      vcycle(0, pr_, pz_, nrhs);
+      vcycle_pcg(nrhs);
       dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
       ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZnew, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
                                   hipMemcpyDeviceToDevice, s_));

@@ -22,33 +22,77 @@
 
 namespace anneal {
 
+template <typename T>
+__device__ __forceinline__ KeCoef<T> scaled(const KeCoef<T>& k, T e) {
+  return KeCoef<T>{k.nAmB * e, k.nB * e, k.g * e, k.pc * e, k.pd * e, k.tAmB * e, k.tB * e, k.w * e};
+}
+
+// Block index -> (tile x, tile y, z chunk).  Consecutive workgroups are
+// dealt round-robin to the 8 XCDs; remap so that each XCD receives a
+// contiguous run of logical tiles (z chunks fastest), keeping the plane and
+// column halos that neighbouring tiles share inside one L2.
+__device__ __forceinline__ int xcd_logical(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = bid & 7;
+  return x * q + min(x, r) + (bid >> 3);
+}
+
 template <typename T, int TX, int TY, int ZC, int MODE>
 __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
-  constexpr int LX = TX + 1, LY = TY + 1, NT = TX * TY;
-  __shared__ T su[3][3][LY * LX];  // three planes: load p while p-1 is still read
+  constexpr int LX = TX + 1, LY = TY + 1, NT = TX * TY, NP = LX * LY;
+  constexpr int NLD = (NP + NT - 1) / NT;  // node loads per thread per plane
+  __shared__ T su[3][3][NP];               // three planes: p is written while p-1 is still read
   __shared__ T sp[12][NT];
   __shared__ double sred[NT / 64];
 
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  // blockIdx.x enumerates (tile x, tile y, z chunk); blockIdx.y = right-hand side.
-  int bid = blockIdx.x;
-  const int bx = bid % a.ntx;
-  bid /= a.ntx;
-  const int by = bid % a.nty;
-  const int bz = bid / a.nty;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int nb = a.ntx * a.nty * a.nzc;
+  const int lid = xcd_logical(blockIdx.x, nb);
+  const int bz = lid % a.nzc;
+  const int bxy = lid / a.nzc;
+  const int bx = bxy % a.ntx, by = bxy / a.ntx;
   const int r = blockIdx.y;
   const int X0 = bx * (TX - 1), Y0 = by * (TY - 1), Z0 = bz * ZC;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
+  const int sx = nx + 1;
+  const int64_t sxy = (int64_t)(nx + 1) * (ny + 1);
   const int64_t ldn = a.ldn;
   const T* __restrict__ u = a.u + (int64_t)3 * r * ldn;
   T* y = a.y + (int64_t)3 * r * ldn;  // not restrict: kCheb may write over prev
   const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
+  const T* pr = (MODE == kCheb && a.prev) ? a.prev + (int64_t)3 * r * ldn : nullptr;
 
+  // load slots of this thread in the (TX+1) x (TY+1) node tile
+  int64_t lbase[NLD];
+  bool lok[NLD];
+#pragma unroll
+  for (int k = 0; k < NLD; ++k) {
+    const int idx = tid + k * NT;
+    const int lx = idx % LX, ly = idx / LX;
+    const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
+    lok[k] = idx < NP && gx >= 0 && gx <= nx && gy >= 0 && gy <= ny;
+    lbase[k] = lok[k] ? gx + (int64_t)sx * gy : 0;
+  }
   const int ei = X0 - 1 + tx, ej = Y0 - 1 + ty;
   const bool ecol = (ei >= 0) && (ei < nx) && (ej >= 0) && (ej < ny);
+  const T* __restrict__ Ecol = a.Ee + (ecol ? ei + (int64_t)nx * ej : 0);
+  const int64_t exy = (int64_t)nx * ny;
   const int ni = X0 + tx, nj = Y0 + ty;
   const bool ncol = (tx < TX - 1) && (ty < TY - 1) && (ni <= nx) && (nj <= ny);
+  const int64_t nbase = ncol ? ni + (int64_t)sx * nj : 0;
+
+  T pre[NLD][3];
+  auto fetch = [&](int p) {
+    const bool pin = (p >= 0) && (p <= nz);
+#pragma unroll
+    for (int k = 0; k < NLD; ++k) {
+      const bool ok = pin && lok[k];
+      const int64_t n = lbase[k] + sxy * (ok ? p : 0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pre[k][c] = ok ? u[c * ldn + n] : T(0);
+    }
+  };
+  auto fetch_e = [&](int L) -> T { return (ecol && L >= 0 && L < nz) ? Ecol[exy * L] : T(0); };
 
   T Fprev[3][4], gup[3][4];
 #pragma unroll
@@ -59,50 +103,69 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
       gup[c][m] = T(0);
     }
   double dot = 0.0;
+  fetch(Z0 - 1);
+  T e_cur = T(0);
+  T e_nxt = fetch_e(Z0 - 1);
 
-  const int pend = Z0 + ZC;  // last plane to load (inclusive)
-  for (int p = Z0 - 1; p <= pend; ++p) {
-    T* buf = &su[(p - Z0 + 1) % 3][0][0];
-    const bool pin = (p >= 0) && (p <= nz);
-    for (int idx = threadIdx.x; idx < LX * LY; idx += NT) {
-      const int lx = idx % LX, ly = idx / LX;
-      const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
-      T v0 = T(0), v1 = T(0), v2 = T(0);
-      if (pin && gx >= 0 && gx <= nx && gy >= 0 && gy <= ny) {
-        const int64_t n = gx + (int64_t)sx * gy + (int64_t)sxy * p;
-        v0 = u[n];
-        v1 = u[ldn + n];
-        v2 = u[2 * ldn + n];
+#pragma unroll
+  for (int s = 0; s <= ZC + 1; ++s) {
+    const int p = Z0 - 1 + s;  // node plane staged in this step
+    T* buf = &su[s % 3][0][0];
+#pragma unroll
+    for (int k = 0; k < NLD; ++k) {
+      const int idx = tid + k * NT;
+      if (idx < NP) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[k][c];
       }
-      buf[idx] = v0;
-      buf[LX * LY + idx] = v1;
-      buf[2 * LX * LY + idx] = v2;
     }
+    if (s <= ZC) fetch(p + 1);  // in flight while this plane is processed
+    const T e_this = e_nxt;      // stiffness of layer p-1 (between planes p-1 and p)
+    if (s <= ZC) e_nxt = fetch_e(p);
+    // epilogue operands of the node finished in this step (plane p-1)
+    const int L = p - 1;
+    const bool outp = (s >= 2) && (L <= nz);
+    uint8_t mk = 0;
+    T bv[3] = {T(0), T(0), T(0)}, pv[3] = {T(0), T(0), T(0)};
+    float dv = 0.f;
+    int64_t nout = 0;
+    if (outp && ncol) {
+      nout = nbase + sxy * L;
+      mk = a.mask[nout];
+      if (MODE != kApply) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bv[c] = bb[c * ldn + nout];
+      }
+      if (MODE == kJacobi || MODE == kCheb) dv = a.dinv[nout];
+      if (MODE == kCheb && pr) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pv[c] = pr[c * ldn + nout];
+      }
+    }
+    e_cur = e_this;
     __syncthreads();
     T Fcur[3][4];
     {
       const int o = tx + LX * ty;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T* s = buf + c * LX * LY;
-        face_wht(s[o], s[o + 1], s[o + LX], s[o + LX + 1], Fcur[c]);
+        const T* sc = buf + c * NP;
+        face_wht(sc[o], sc[o + 1], sc[o + LX], sc[o + LX + 1], Fcur[c]);
       }
     }
-    if (p >= Z0) {
-      const int L = p - 1;  // element layer between planes p-1 and p
-      T e = T(0);
-      if (ecol && L >= 0 && L < nz) e = a.Ee[ei + (int64_t)nx * (ej + (int64_t)ny * L)];
+    if (s >= 1) {
+      // element layer L between planes p-1 and p
       T gb[3][4], gt[3][4];
-      if (e != T(0)) {
+      if (e_cur != T(0)) {
         T S[3][4], D[3][4];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
-            S[c][m] = e * (Fcur[c][m] + Fprev[c][m]);
-            D[c][m] = e * (Fcur[c][m] - Fprev[c][m]);
+            S[c][m] = Fcur[c][m] + Fprev[c][m];
+            D[c][m] = Fcur[c][m] - Fprev[c][m];
           }
-        element_walsh(S, D, a.k, gb, gt);
+        element_walsh(S, D, scaled(a.k, e_cur), gb, gt);
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -112,8 +175,8 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             gt[c][m] = T(0);
           }
       }
-      if (L >= Z0 && L <= nz) {
-        // plane L now has both its element layers: finish it
+      if (outp) {
+        // plane L has both of its element layers: finish it
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           T G[4];
@@ -121,40 +184,31 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
           for (int m = 0; m < 4; ++m) G[m] = gup[c][m] + gb[c][m];
           T c00, c10, c01, c11;
           face_iwht(G, c00, c10, c01, c11);
-          sp[4 * c + 0][threadIdx.x] = c00;
-          sp[4 * c + 1][threadIdx.x] = c10;
-          sp[4 * c + 2][threadIdx.x] = c01;
-          sp[4 * c + 3][threadIdx.x] = c11;
+          sp[4 * c + 0][tid] = c00;
+          sp[4 * c + 1][tid] = c10;
+          sp[4 * c + 2][tid] = c01;
+          sp[4 * c + 3][tid] = c11;
         }
         __syncthreads();
         if (ncol) {
-          const int t = threadIdx.x;
-          const int64_t n = ni + (int64_t)sx * nj + (int64_t)sxy * L;
-          const uint8_t mk = a.mask[n];
-          const T* prevbuf = &su[(p - Z0) % 3][0][0];
+          const T* prevbuf = &su[(s + 2) % 3][0][0];  // plane L
           const int o = (tx + 1) + LX * (ty + 1);
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            // corners: element (tx,ty) -> its (1,1) corner, (tx+1,ty) -> (0,1),
-            // (tx,ty+1) -> (1,0), (tx+1,ty+1) -> (0,0)
-            T v = sp[4 * c + 3][t] + sp[4 * c + 2][t + 1] + sp[4 * c + 1][t + TX] + sp[4 * c + 0][t + TX + 1];
+            // element (tx,ty) -> its (1,1) corner, (tx+1,ty) -> (0,1), (tx,ty+1) -> (1,0), (tx+1,ty+1) -> (0,0)
+            const T v = sp[4 * c + 3][tid] + sp[4 * c + 2][tid + 1] + sp[4 * c + 1][tid + TX] +
+                        sp[4 * c + 0][tid + TX + 1];
+            const T uc = prevbuf[c * NP + o];
             T out;
-            if (MODE == kApply) {
-              out = v;
-            } else if (MODE == kResid) {
-              out = bb[c * ldn + n] - v;
-            } else {
-              const T uc = prevbuf[c * LX * LY + o];
-              if (MODE == kJacobi) {
-                out = uc + a.omega * T(a.dinv[n]) * (bb[c * ldn + n] - v);
-              } else {
-                const T pv = a.prev ? a.prev[(int64_t)(3 * r + c) * ldn + n] : T(0);
-                out = uc + a.c1 * (uc - pv) + a.c2 * T(a.dinv[n]) * (bb[c * ldn + n] - v);
-              }
-            }
+            if (MODE == kApply) out = v;
+            else if (MODE == kResid) out = bv[c] - v;
+            else if (MODE == kJacobi) out = uc + a.omega * T(dv) * (bv[c] - v);
+            else out = uc + a.c1 * (uc - pv[c]) + a.c2 * T(dv) * (bv[c] - v);
             if (!((mk >> c) & 1)) out = T(0);
-            y[c * ldn + n] = out;
-            if (a.dot_part) dot += (double)prevbuf[c * LX * LY + o] * (double)out;
+            y[c * ldn + nout] = out;
+            if (MODE == kApply) {
+              if (a.dot_part) dot += (double)uc * (double)out;
+            }
           }
         }
       }
@@ -168,15 +222,15 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) Fprev[c][m] = Fcur[c][m];
   }
-  if (a.dot_part) {
+  if (MODE == kApply && a.dot_part) {
     // fixed-order block reduction
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
-    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = dot;
+    if ((tid & 63) == 0) sred[tid >> 6] = dot;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += sred[w];
-      a.dot_part[(int64_t)r * gridDim.x + blockIdx.x] = s;
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < NT / 64; ++w) sum += sred[w];
+      a.dot_part[(int64_t)r * gridDim.x + blockIdx.x] = sum;
     }
   }
 }
@@ -288,10 +342,40 @@ __global__ void k_prolong_add(const T* __restrict__ xc, T* __restrict__ xf, cons
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers
-template <typename T, int MODE>
-static void launch_apply_mode(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
-  constexpr int TX = 16, TY = 16, ZC = 8;
+// Host-side launchers.  Tile shapes (TX x TY element columns per workgroup,
+// ZC node planes per z chunk); the FP32 operator picks one per grid with a
+// timing sweep (FemContext::autotune), the FP64 operator uses the first.
+template <int TX, int TY, int ZC>
+struct Tile {
+  static constexpr int tx = TX, ty = TY, zc = ZC;
+};
+#define ANNEAL_TILES(X) X(0, 16, 16, 8) X(1, 16, 16, 16) X(2, 32, 8, 8) X(3, 32, 8, 16) X(4, 32, 16, 8) \
+  X(5, 16, 16, 4) X(6, 64, 4, 16)
+
+static void tile_dims(int variant, int& TX, int& TY, int& ZC) {
+  switch (variant) {
+#define ANNEAL_TILE_CASE(id, a, b, c) \
+  case id:                           \
+    TX = a;                          \
+    TY = b;                          \
+    ZC = c;                          \
+    return;
+    ANNEAL_TILES(ANNEAL_TILE_CASE)
+#undef ANNEAL_TILE_CASE
+    default: throw std::invalid_argument("unknown apply tile variant");
+  }
+}
+
+int apply_variants() { return 7; }
+
+int apply_blocks(int nx, int ny, int nz, int variant) {
+  int TX, TY, ZC;
+  tile_dims(variant, TX, TY, ZC);
+  return ((nx + 1 + TX - 2) / (TX - 1)) * ((ny + 1 + TY - 2) / (TY - 1)) * ((nz + 1 + ZC - 1) / ZC);
+}
+
+template <typename T, int MODE, int TX, int TY, int ZC>
+static void launch_tile(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
   ApplyArgs<T> a = a0;
   a.ntx = (a.nx + 1 + TX - 2) / (TX - 1);
   a.nty = (a.ny + 1 + TY - 2) / (TY - 1);
@@ -301,18 +385,28 @@ static void launch_apply_mode(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
   ANNEAL_HIP(hipGetLastError());
 }
 
-int apply_blocks(int nx, int ny, int nz) {
-  constexpr int TX = 16, TY = 16, ZC = 8;
-  return ((nx + 1 + TX - 2) / (TX - 1)) * ((ny + 1 + TY - 2) / (TY - 1)) * ((nz + 1 + ZC - 1) / ZC);
+template <typename T, int MODE>
+static void launch_apply_mode(const ApplyArgs<T>& a, int nrhs, int variant, hipStream_t s) {
+  if (sizeof(T) == 8) variant = 0;  // FP64: one shape (register pressure)
+  switch (variant) {
+#define ANNEAL_TILE_LAUNCH(id, a_, b_, c_) \
+  case id:                                 \
+    launch_tile<T, MODE, a_, b_, c_>(a, nrhs, s); \
+    return;
+    ANNEAL_TILES(ANNEAL_TILE_LAUNCH)
+#undef ANNEAL_TILE_LAUNCH
+    default: throw std::invalid_argument("unknown apply tile variant");
+  }
 }
 
 template <typename T>
 void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s) {
+  const int v = a.variant;
   switch (mode) {
-    case kApply: launch_apply_mode<T, kApply>(a, nrhs, s); break;
-    case kResid: launch_apply_mode<T, kResid>(a, nrhs, s); break;
-    case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, s); break;
-    case kCheb: launch_apply_mode<T, kCheb>(a, nrhs, s); break;
+    case kApply: launch_apply_mode<T, kApply>(a, nrhs, v, s); break;
+    case kResid: launch_apply_mode<T, kResid>(a, nrhs, v, s); break;
+    case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, v, s); break;
+    case kCheb: launch_apply_mode<T, kCheb>(a, nrhs, v, s); break;
     default: throw std::invalid_argument("bad apply mode");
   }
 }

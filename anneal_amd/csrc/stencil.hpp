@@ -32,7 +32,8 @@ struct ApplyArgs {
   T c1, c2;            // kCheb recurrence coefficients
   int nx, ny, nz;
   int64_t ldn;
-  int ntx, nty, nzc;   // tiles in x, y and z chunks
+  int ntx, nty, nzc;   // tiles in x, y and z chunks (set by the launcher)
+  int variant;         // tile shape id (host side)
 };
 
 // Face (2-D) Walsh transform of 4 corner values a=(i,j) b=(i+1,j) c=(i,j+1) d=(i+1,j+1)

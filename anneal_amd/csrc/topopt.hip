@@ -156,7 +156,29 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   }
 }
 
-// Sum the volume partials in a fixed order and advance the bisection.
+// Sum the volume partials in a fixed order, record the evaluation in the
+// bracket, replay every bisection trial it decides, and choose the next
+// multiplier to evaluate.
+__device__ void oc_record(double* oc, double lam, double f) {
+  if (f > 0.0) {
+    if (oc[kOcHasA] == 0.0 || lam > oc[kOcA]) {
+      oc[kOcA] = lam;
+      oc[kOcFA] = f;
+      oc[kOcHasA] = 1.0;
+      if (oc[kOcSide] > 0.0 && oc[kOcHasB] != 0.0) oc[kOcFB] *= 0.5;  // Illinois
+    }
+    oc[kOcSide] = 1.0;
+  } else {
+    if (oc[kOcHasB] == 0.0 || lam < oc[kOcB]) {
+      oc[kOcB] = lam;
+      oc[kOcFB] = f;
+      oc[kOcHasB] = 1.0;
+      if (oc[kOcSide] < 0.0 && oc[kOcHasA] != 0.0) oc[kOcFA] *= 0.5;
+    }
+    oc[kOcSide] = -1.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc) {
   if (oc[kOcDone] != 0.0) return;
   double s = 0.0;
@@ -165,20 +187,54 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const double vol = sh[0] + sh[1] + sh[2] + sh[3];
-    double lo = oc[kOcLo], hi = oc[kOcHi];
-    const double lam = oc[kOcLam];
-    if (vol > oc[kOcVt]) lo = lam;
-    else hi = lam;
-    oc[kOcLo] = lo;
-    oc[kOcHi] = hi;
-    oc[kOcLamLast] = lam;
-    oc[kOcVol] = vol;
-    oc[kOcSteps] += 1.0;
-    if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) oc[kOcDone] = 1.0;
-    else oc[kOcLam] = sqrt(lo * hi);
+  if (threadIdx.x != 0) return;
+  const double vol = sh[0] + sh[1] + sh[2] + sh[3];
+  oc[kOcVol] = vol;
+  oc[kOcEvals] += 1.0;
+  oc_record(oc, oc[kOcLam], vol - oc[kOcVt]);
+  double lo = oc[kOcLo], hi = oc[kOcHi];
+  for (int guard = 0; guard < 200; ++guard) {
+    const double lam = sqrt(lo * hi);  // the reference's next trial
+    int dec = 0;                        // +1: vol(lam) > vt, -1: vol(lam) <= vt
+    if (oc[kOcHasA] != 0.0 && lam <= oc[kOcA]) dec = 1;
+    else if (oc[kOcHasB] != 0.0 && lam >= oc[kOcB]) dec = -1;
+    if (dec != 0) {
+      if (dec > 0) lo = lam;
+      else hi = lam;
+      oc[kOcLamLast] = lam;
+      oc[kOcSteps] += 1.0;
+      oc[kOcTries] = 0.0;
+      if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) {
+        oc[kOcDone] = 1.0;
+        break;
+      }
+      continue;
+    }
+    // undecided: pick the next multiplier to evaluate
+    double next = lam;
+    const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
+    if (hasA && hasB) {
+      if (oc[kOcTries] < 4.0) {
+        const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
+        const double fa = oc[kOcFA], fb = oc[kOcFB];
+        double t = ta - fa * (tb - ta) / (fb - fa);
+        if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
+        const double c = exp(t);
+        if (c > oc[kOcA] && c < oc[kOcB]) next = c;
+        oc[kOcTries] += 1.0;
+      }
+    } else if (hasA || hasB) {
+      // one side known: step outward geometrically to find the other side
+      const double g = oc[kOcGrow];
+      const double c = hasA ? oc[kOcA] * g : oc[kOcB] / g;
+      if (c > lo && c < hi) next = c;
+      oc[kOcGrow] = g * g;
+    }
+    oc[kOcLam] = next;
+    break;
   }
+  oc[kOcLo] = lo;
+  oc[kOcHi] = hi;
 }
 
 __global__ void k_oc_final(const double* __restrict__ x, const double* __restrict__ dcx, const double* __restrict__ dvx,
@@ -256,7 +312,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&part_, std::max(nblk_, 6 * 512) * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&oc_, kOcLen * sizeof(double)));
-  ANNEAL_HIP(hipHostMalloc((void**)&host_, 16 * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_, kOcLen * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
   FilterArgs a{};
   a.flags = flags_;
@@ -327,12 +383,14 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
 }
 
 int FilterContext::oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
-                             double move, double lo, double hi, double vtarget, double* xnew) {
+                             double move, double lo, double hi, double vtarget, double* xnew, double hint,
+                             int* evals, double* lam_last) {
   double st[kOcLen] = {};
   st[kOcLo] = lo;
   st[kOcHi] = hi;
-  st[kOcLam] = std::sqrt(lo * hi);
+  st[kOcLam] = (hint > lo && hint < hi) ? hint : std::sqrt(lo * hi);
   st[kOcVt] = vtarget;
+  st[kOcGrow] = 1.05;
   ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
   FilterArgs a{};
   a.in = x;
@@ -351,22 +409,24 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   a.nz = g_.nz;
   int launched = 0;
   while (true) {
-    // the schedule from 1e16 down to 1+1e-9 takes 36 halvings; kernels after
-    // the stop are no-ops that read the device flag
-    const int batch = launched == 0 ? 37 : 8;
-    for (int i = 0; i < batch && launched < 100; ++i, ++launched) {
+    // evaluations launched after the stop are no-ops that read the device flag
+    const int batch = launched == 0 ? 12 : 8;
+    for (int i = 0; i < batch && launched < 240; ++i, ++launched) {
       launch_filter(kOc, R_, a, nblk_, s_);
       hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_, nblk_, oc_);
       ANNEAL_HIP(hipGetLastError());
     }
     ANNEAL_HIP(hipMemcpyAsync(host_, oc_, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
     ANNEAL_HIP(hipStreamSynchronize(s_));
-    if (host_[kOcDone] != 0.0 || launched >= 100) break;
+    if (host_[kOcDone] != 0.0) break;
+    if (launched >= 240) throw std::runtime_error("OC bisection did not terminate");
   }
   const int64_t n = g_.nelem();
   hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, x, dcx, dvx, flags_, oc_, beta,
                      move, n, xnew);
   ANNEAL_HIP(hipGetLastError());
+  if (evals) *evals = (int)host_[kOcEvals];
+  if (lam_last) *lam_last = host_[kOcLamLast];
   return (int)host_[kOcSteps];
 }
 

@@ -8,9 +8,13 @@ namespace anneal {
 // Element flags
 constexpr uint8_t kActive = 1, kFrozen = 2;
 
-// OC bisection state on the device (FP64).
-constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7;
-constexpr int kOcLen = 8;
+// OC bisection state on the device (FP64).  kOcLam is the multiplier the next
+// trial-volume kernel evaluates; (lo, hi) is the reference's bisection state;
+// [A, B] is a certified bracket of the root of vol(lam) = vt.
+constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7,
+              kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcTries = 15,
+              kOcEvals = 16, kOcGrow = 17;
+constexpr int kOcLen = 24;
 
 class FilterContext {
  public:
@@ -26,10 +30,16 @@ class FilterContext {
   // xPhys, dxp = project(forward(x)), frozen -> 1 (optimize.py:81-93).
   // Also returns sum(phi * xPhys) over active voxels in *vol_dev (device).
   void physical(const double* x, double beta, const double* phi, double* xphys, double* dxp, double* vol_dev);
-  // The whole geometric bisection (optimize.py:144-161) on the device.  Returns
-  // the number of trial steps; xnew receives the last trial design.
+  // The geometric bisection of optimize.py:144-161 on the device, replayed
+  // exactly (same trial multipliers, decisions and stop) from fewer volume
+  // evaluations: vol(lam) is non-increasing, so a certified bracket [A, B] of
+  // its root decides every trial outside it; inside, Illinois steps in log(lam)
+  // narrow the bracket first.  hint > 0 is a guess of the root (e.g. the
+  // previous iteration's) used for the first evaluation.  xnew receives the
+  // last trial design; returns the number of trials (the reference's count).
   int oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
-                double lo, double hi, double vtarget, double* xnew);
+                double lo, double hi, double vtarget, double* xnew, double hint = 0.0, int* evals = nullptr,
+                double* lam_last = nullptr);
 
   const GridDims& grid() const { return g_; }
   const uint8_t* flags() const { return flags_; }

@@ -203,6 +203,18 @@ class FEM:
         except Exception:
             pass
 
+    def set_option(self, name, value):
+        """Solver knobs of the device context: graphs, smoother_degree, cheb_ratio, inner_tol_floor, inner_max."""
+        _lib.call("anneal_fem_set_option", self._ctx, name.encode(), float(value))
+
+    def autotune(self, variant=-1):
+        """Pick the FP32 apply tile shape (variant < 0: time all on this grid); returns (chosen, us per shape)."""
+        chosen, n = _lib.ctypes.c_int(), _lib.ctypes.c_int()
+        us = (_lib.ctypes.c_float * 16)()
+        _lib.call("anneal_fem_autotune", self._ctx, int(variant), _lib.ctypes.byref(chosen), us,
+                  _lib.ctypes.byref(n))
+        return chosen.value, ([us[i] for i in range(n.value)] if variant < 0 else [])
+
     # -- layout helpers (AoS dof 3n+c on the host <-> SoA [c][n] on the device)
     def _to_device(self, vecs, reduced=True):
         torch = _torch()

@@ -77,11 +77,14 @@ class DeviceFilter:
         _lib.call("anneal_filter_physical", self._ctx, _lib.ptr(x), float(beta), _lib.ptr(phi), _lib.ptr(xphys),
                   _lib.ptr(dxp), _lib.ptr(vol))
 
-    def oc_bisect(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew):
-        steps = _lib.ctypes.c_int()
+    def oc_bisect(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint=0.0, info=None):
+        """Replayed OC bisection; returns the number of trials.  ``info`` (a dict) gets evals and lam_last."""
+        steps, evals, lam = _lib.ctypes.c_int(), _lib.ctypes.c_int(), _lib.c_double()
         _lib.call("anneal_oc_bisect", self._ctx, _lib.ptr(x), _lib.ptr(dcx), _lib.ptr(dvx), _lib.ptr(phi),
-                  float(beta), float(move), float(lo), float(hi), float(vtarget), _lib.ptr(xnew),
-                  _lib.ctypes.byref(steps))
+                  float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), _lib.ptr(xnew),
+                  _lib.ctypes.byref(steps), _lib.ctypes.byref(evals), _lib.ctypes.byref(lam))
+        if info is not None:
+            info["evals"], info["lam_last"] = evals.value, lam.value
         return steps.value
 
 
@@ -205,6 +208,8 @@ class SimpRun:
         self.beta_since = 0
         self.it = 0
         self.converged = False
+        self.oc_ratio = 0.0
+        self.oc_evals = 0
 
     def _tick(self):
         self.torch.cuda.synchronize()
@@ -270,8 +275,12 @@ class SimpRun:
         st = self.stats.cpu().numpy()
         ndes = st[3]
         scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
+        info = {}
+        hint = scale * self.oc_ratio if self.oc_ratio else 0.0   # previous root relative to its scale
         steps = filt.oc_bisect(self.x, self.dcx, self.dvx, self.phi, beta, self.move, scale * 1e-8, scale * 1e8,
-                               self.Vtarget, self.xnew)
+                               self.Vtarget, self.xnew, hint=hint, info=info)
+        self.oc_ratio = info["lam_last"] / scale
+        self.oc_evals += info["evals"]
         te = self._tick()
         T["oc"] += te - td
         filt.physical(self.xnew, beta, self.phi, self.xphys, self.dxp)
@@ -288,7 +297,7 @@ class SimpRun:
         h = {"iter": it, "objective": obj, "compliance": cs.tolist(), "p": p, "beta": beta,
              "volumeFraction": volfrac, "change": change, "grey": grey,
              "cgIters": fem.cg_iters[-ncase:], "amgRebuilt": bool(rebuild),
-             "seconds": time.time() - t0, "ocSteps": steps}
+             "seconds": time.time() - t0, "ocSteps": steps, "ocEvals": info["evals"]}
         self.hist.append(h)
         self.log(f"  it {it:3d}  p {p:.2f}  b {beta:.0f}  obj {obj:.4g}  vol {volfrac:.3f}  "
                  f"change {change:.3f}  grey {grey:.2f}  cg {fem.cg_iters[-1]}{' amg' if rebuild else ''}  "

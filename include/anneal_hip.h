@@ -69,6 +69,19 @@ int anneal_fem_apply_f64(void* ctx, const double* u_dev, double* y_dev, int nrhs
  * 4 = non-finite result (the reference raises SolverError for both). */
 int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, double rtol, int maxiter, int use_x0,
                      int* iters_host);
+/* Tile-shape autotuner of the FP32 apply (no reference counterpart: the
+ * reference has no kernels).  variant >= 0 selects a shape; variant < 0 times
+ * all shapes on this grid and keeps the fastest.  *chosen_host receives the
+ * shape in use; us_host (may be NULL; room for 16 floats) receives *n_host per-shape times in
+ * microseconds, where *n_host is the number of shapes. */
+int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host);
+/* Solver options (no reference counterpart; the reference's solver has none):
+ * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch
+ * eagerly), "smoother_degree" (Chebyshev degree per side, >= 1),
+ * "cheb_ratio" (lambda_max / lambda_min of the smoothed interval),
+ * "inner_tol_floor" (relative residual reduction asked of each FP32 inner
+ * PCG), "inner_max" (iterations per inner PCG).  Unknown names fail. */
+int anneal_fem_set_option(void* ctx, const char* name, double value);
 /* One V-cycle of the multigrid preconditioner on FP32 vectors (z = M r). */
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs);
 /* FEM.element_energy (fem.py:239-242): ce[r*nelem + e] = u_e^T ke u_e, E = 1. */
@@ -96,11 +109,16 @@ int anneal_filter_backward(void* ctx, const double* d_dev, double* y_dev);
 int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const double* phi_dev, double* xphys_dev,
                            double* dxp_dev, double* vol_dev);
 /* The OC bisection loop (optimize.py:144-161) on the device, starting from the
- * bracket [lo, hi]; xnew_dev receives the last trial design and *steps_host
- * the number of trials. */
+ * bracket [lo, hi].  The trial multipliers, the decisions vol > vtarget and
+ * the stop hi/lo < 1+1e-9 are the reference's; since vol(lam) is
+ * non-increasing, trials outside a certified bracket of its root are decided
+ * without a volume evaluation.  hint (0 = none) is a guess of the root for the
+ * first evaluation.  xnew_dev receives the last trial design, *steps_host the
+ * number of trials, *evals_host the volume evaluations spent (may be NULL) and
+ * *lam_last_host the last trial multiplier (may be NULL). */
 int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
                      const double* phi_dev, double beta, double move, double lo, double hi, double vtarget,
-                     double* xnew_dev, int* steps_host);
+                     double hint, double* xnew_dev, int* steps_host, int* evals_host, double* lam_last_host);
 
 /* ---- element-wise pieces of simp() ---------------------------------------- */
 

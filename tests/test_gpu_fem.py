@@ -147,3 +147,45 @@ def test_element_energy(torch):
     want = ref.element_energy(u)
     got = fem.element_energy(u)
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_graph_replay_matches_eager(torch):
+    """The captured V-cycle graph gives bit-identical solves to eager launches, replay after replay."""
+    g, active, fixed, Ee = random_problem((40, 20, 20), 9, 0.05)
+    fem = _product_fem(g, active, fixed)
+    K = fem.assemble(Ee[fem.act])
+    rng = np.random.default_rng(5)
+    fs = [rng.standard_normal(fem.nfree) for _ in range(3)]
+    fem.set_option("graphs", 0)
+    eager = [fem.solve(K, [f], rtol=1e-8)[0] for f in fs]
+    fem.set_option("graphs", 1)
+    for rep in range(2):
+        for f, want in zip(fs, eager):
+            got = fem.solve(K, [f], rtol=1e-8)[0]
+            torch.cuda.synchronize()
+            assert np.array_equal(got, want), f"replay {rep} differs"
+    # a new stiffness field read through the same graph
+    K2 = fem.assemble(Ee[fem.act] * rng.uniform(0.5, 2.0, fem.act.size))
+    fem.set_option("graphs", 0)
+    e2 = fem.solve(K2, [fs[0]], rtol=1e-8)[0]
+    fem.set_option("graphs", 1)
+    assert np.array_equal(fem.solve(K2, [fs[0]], rtol=1e-8)[0], e2)
+
+
+def test_autotune_shapes_agree(torch):
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem((30, 17, 12), 10, 0.1)
+    fem = _product_fem(g, active, fixed)
+    fem.assemble(Ee[fem.act])
+    U = torch.randn(1, 3, fem.ldn, dtype=torch.float32, device="cuda")
+    outs = []
+    for v in range(7):
+        fem.autotune(v)
+        Y = torch.zeros_like(U)
+        _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(Y), 1)
+        torch.cuda.synchronize()
+        outs.append(Y[..., :g.nnode].clone())
+    for o in outs[1:]:
+        assert torch.allclose(o, outs[0], rtol=2e-6, atol=1e-6 * outs[0].abs().max().item())
+    chosen, us = fem.autotune(-1)
+    assert 0 <= chosen < 7 and len(us) == 7

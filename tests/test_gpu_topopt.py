@@ -107,10 +107,15 @@ def test_oc_bisection(torch, beta):
         a[act] = v
         return torch.from_numpy(a).cuda()
     XN = torch.zeros(g.nelem, dtype=torch.float64, device="cuda")
-    steps = df.oc_bisect(full(x), full(dcx), full(dvx), full(phi), beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN)
-    got = XN.cpu().numpy()[act]
-    assert steps == steps_ref
-    assert np.abs(got - want).max() <= 1e-10
+    X, DCX, DVX, PHI = full(x), full(dcx), full(dvx), full(phi)
+    for hint in (0.0, scale * 0.37, scale * 1e6):
+        info = {}
+        steps = df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, hint=hint, info=info)
+        got = XN.cpu().numpy()[act]
+        print(f"oc beta={beta} hint={hint:.3g}: {steps} trials from {info['evals']} volume evaluations", flush=True)
+        assert steps == steps_ref
+        assert info["evals"] <= steps_ref
+        assert np.abs(got - want).max() <= 1e-10
 
 
 def test_sensitivity_kernel(torch):
