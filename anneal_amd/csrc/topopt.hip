@@ -159,6 +159,9 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
 // Sum the volume partials in a fixed order, record the evaluation in the
 // bracket, replay every bisection trial it decides, and choose the next
 // multiplier to evaluate.
+constexpr double kOcEps = 0.02;          // half-width of the warm bracket around a hint
+constexpr double kOcSecantWidth = 0.25;  // log-width below which the root is interpolated
+
 __device__ void oc_record(double* oc, double lam, double f) {
   if (f > 0.0) {
     if (oc[kOcHasA] == 0.0 || lam > oc[kOcA]) {
@@ -213,22 +216,28 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
     // undecided: pick the next multiplier to evaluate
     double next = lam;
     const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
-    if (hasA && hasB) {
-      if (oc[kOcTries] < 4.0) {
-        const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
-        const double fa = oc[kOcFA], fb = oc[kOcFB];
-        double t = ta - fa * (tb - ta) / (fb - fa);
-        if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
-        const double c = exp(t);
-        if (c > oc[kOcA] && c < oc[kOcB]) next = c;
-        oc[kOcTries] += 1.0;
-      }
-    } else if (hasA || hasB) {
-      // one side known: step outward geometrically to find the other side
+    const double stage = oc[kOcStage];
+    if (stage == 1.0) {
+      // second point of the warm bracket around the hint
+      const double c = oc[kOcHint] * (1.0 + kOcEps);
+      if (c > lo && c < hi) next = c;
+      oc[kOcStage] = 2.0;
+    } else if (stage == 2.0 && (hasA != hasB)) {
+      // one side known: step outward geometrically from it
       const double g = oc[kOcGrow];
       const double c = hasA ? oc[kOcA] * g : oc[kOcB] / g;
       if (c > lo && c < hi) next = c;
+      else oc[kOcStage] = 3.0;
       oc[kOcGrow] = g * g;
+    } else if (hasA && hasB && log(oc[kOcB] / oc[kOcA]) < kOcSecantWidth && oc[kOcTries] < 3.0) {
+      // narrow bracket: Illinois (regula falsi) step in log(lam)
+      const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
+      const double fa = oc[kOcFA], fb = oc[kOcFB];
+      double t = ta - fa * (tb - ta) / (fb - fa);
+      if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
+      const double c = exp(t);
+      if (c > oc[kOcA] && c < oc[kOcB]) next = c;
+      oc[kOcTries] += 1.0;
     }
     oc[kOcLam] = next;
     break;
@@ -388,9 +397,12 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   double st[kOcLen] = {};
   st[kOcLo] = lo;
   st[kOcHi] = hi;
-  st[kOcLam] = (hint > lo && hint < hi) ? hint : std::sqrt(lo * hi);
+  const bool warm = hint * (1.0 - kOcEps) > lo && hint * (1.0 + kOcEps) < hi;
+  st[kOcLam] = warm ? hint * (1.0 - kOcEps) : std::sqrt(lo * hi);
+  st[kOcStage] = warm ? 1.0 : 3.0;
+  st[kOcHint] = hint;
   st[kOcVt] = vtarget;
-  st[kOcGrow] = 1.05;
+  st[kOcGrow] = 1.25;
   ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
   FilterArgs a{};
   a.in = x;
