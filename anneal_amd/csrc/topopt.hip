@@ -229,6 +229,11 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
       if (c > lo && c < hi) next = c;
       else oc[kOcStage] = 3.0;
       oc[kOcGrow] = g * g;
+    } else if ((hasA != hasB) && oc[kOcSteps] >= 4.0 && oc[kOcEnds] == 0.0) {
+      // still one-sided after several trials: test the far end of the initial
+      // bracket; if it is on the same side, every remaining trial is decided
+      next = hasA ? oc[kOcHi0] : oc[kOcLo0];
+      oc[kOcEnds] = 1.0;
     } else if (hasA && hasB && log(oc[kOcB] / oc[kOcA]) < kOcSecantWidth && oc[kOcTries] < 3.0) {
       // narrow bracket: Illinois (regula falsi) step in log(lam)
       const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
@@ -401,6 +406,8 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   st[kOcLam] = warm ? hint * (1.0 - kOcEps) : std::sqrt(lo * hi);
   st[kOcStage] = warm ? 1.0 : 3.0;
   st[kOcHint] = hint;
+  st[kOcLo0] = lo;
+  st[kOcHi0] = hi;
   st[kOcVt] = vtarget;
   st[kOcGrow] = 1.25;
   ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));

@@ -13,7 +13,7 @@ constexpr uint8_t kActive = 1, kFrozen = 2;
 // [A, B] is a certified bracket of the root of vol(lam) = vt.
 constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7,
               kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcTries = 15,
-              kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19;
+              kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19, kOcLo0 = 20, kOcHi0 = 21, kOcEnds = 22;
 constexpr int kOcLen = 24;
 
 class FilterContext {

@@ -77,8 +77,10 @@ def test_physical_projection(torch, beta):
     assert abs(vol.item() - (phi_act * xp).sum()) <= 1e-12 * (phi_act * xp).sum()
 
 
-@pytest.mark.parametrize("beta", [0.0, 2.0])
-def test_oc_bisection(torch, beta):
+@pytest.mark.parametrize("beta,target", [(0.0, "inside"), (2.0, "inside"), (0.0, "unreachable")])
+def test_oc_bisection(torch, beta, target):
+    """Device OC update vs the oracle's 36-trial bisection (optimize.py:144-161): same trial
+    count and the same last trial design, from fewer volume evaluations."""
     from anneal_amd.optimize import DeviceFilter
     from anneal_amd.fem import Grid
     shape, r = (24, 12, 10), 1.5
@@ -91,15 +93,26 @@ def test_oc_bisection(torch, beta):
     dcx = -rng.random(act.size) * 10 ** rng.uniform(-2, 1, act.size)
     dvx = rng.uniform(0.1, 1.0, act.size)
     phi = rng.uniform(0.5, 1.0, act.size)
-    vt = 0.3 * phi.sum()
 
     def physical(xd):
         xp, _ = heaviside(ref.forward(xd), beta)
         xp = xp.copy()
         xp[frz] = 1.0
         return xp
-    want, steps_ref = oc_update(x, dcx, dvx, phi, frz, des, vt, 0.2, beta, physical)
     scale = max(float(np.mean(-dcx[des]) / max(np.mean(dvx[des]), 1e-30)), 1e-300)
+    mv = 0.2 if beta <= 1 else 0.1
+
+    def vol(lam):
+        xn = np.clip(x * np.sqrt(np.maximum(-dcx, 0) / np.maximum(dvx * lam, 1e-300)), np.maximum(0, x - mv),
+                     np.minimum(1, x + mv))
+        xn[frz] = 1.0
+        return (phi * physical(xn)).sum()
+    if target == "inside":       # a target the bracket contains, as in a real OC step
+        vt = 0.5 * (vol(scale * 1e-8) + vol(scale * 1e8))
+    else:                        # more volume than any trial reaches: every trial decides "lo"
+        vt = 0.3 * phi.sum()
+        assert vol(scale * 1e8) > vt
+    want, steps_ref = oc_update(x, dcx, dvx, phi, frz, des, vt, 0.2, beta, physical)
     df = DeviceFilter(Grid(shape, 1.0, (0, 0, 0)), active, r, frozen)
 
     def full(v):
@@ -112,10 +125,23 @@ def test_oc_bisection(torch, beta):
         info = {}
         steps = df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, hint=hint, info=info)
         got = XN.cpu().numpy()[act]
-        print(f"oc beta={beta} hint={hint:.3g}: {steps} trials from {info['evals']} volume evaluations", flush=True)
+        print(f"oc beta={beta} {target} hint={hint / scale:.3g}*scale: {steps} trials from {info['evals']} "
+              f"volume evaluations", flush=True)
         assert steps == steps_ref
-        assert info["evals"] <= steps_ref
         assert np.abs(got - want).max() <= 1e-10
+        assert info["evals"] < steps_ref
+    # a warm start at the previous root needs only a handful of evaluations
+    info = {}
+    df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, hint=info_lam(df, X, DCX, DVX,
+                 PHI, beta, scale, vt, XN), info=info)
+    if target == "inside":
+        assert info["evals"] <= 12
+
+
+def info_lam(df, X, DCX, DVX, PHI, beta, scale, vt, XN):
+    info = {}
+    df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, info=info)
+    return info["lam_last"] * 1.003
 
 
 def test_sensitivity_kernel(torch):
