@@ -24,6 +24,7 @@ SIGNATURES = {
     "anneal_fem_destroy": [c_void_p],
     "anneal_fem_info": [c_void_p, c_i64p, c_intp, c_void_p],
     "anneal_fem_set_Ee": [c_void_p, c_void_p],
+    "anneal_fem_rebuild": [c_void_p],
     "anneal_fem_apply_f32": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_apply_f64": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_solve": [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_intp],

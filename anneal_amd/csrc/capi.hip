@@ -118,6 +118,13 @@ int anneal_fem_set_Ee(void* ctx, const double* Ee_dev) {
   });
 }
 
+int anneal_fem_rebuild(void* ctx) {
+  return guarded([&] {
+    fem_of(ctx)->fem->rebuild_coarse();
+    return 0;
+  });
+}
+
 int anneal_fem_apply_f32(void* ctx, const float* u_dev, float* y_dev, int nrhs) {
   return guarded([&] {
     ANNEAL_CHECK(u_dev && y_dev && u_dev != (const float*)y_dev, "apply needs distinct u and y");
@@ -182,6 +189,8 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
     } else if (n == "inner_tol_floor") {
       ANNEAL_CHECK(value > 0.0 && value < 1.0, "inner_tol_floor must be in (0, 1)");
       f->inner_tol_floor = value;
+    } else if (n == "rebuild_each_solve") {
+      f->rebuild_each_solve = value != 0.0;
     } else if (n == "inner_max") {
       ANNEAL_CHECK(value >= 1, "inner_max must be >= 1");
       f->inner_max = (int)value;

@@ -359,8 +359,14 @@ void FemContext::set_Ee(const double* Ee_dev) {
     tuned_ = true;
     autotune(-1, nullptr);  // first operator on a large grid: pick the fastest tile shape
   }
+  coarse_stale_ = true;
+}
+
+void FemContext::rebuild_coarse() {
   const Level& C = lv_.back();
   dense_assemble_invert(C.Ee, C.mask, ke_coarse_dev_, Ainv_, Awork_, C.g.nx, C.g.ny, C.g.nz, s_);
+  coarse_stale_ = false;
+  coarse_valid_ = true;
 }
 
 void FemContext::apply_f32(const float* u, float* y, int nrhs) {
@@ -477,6 +483,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
 
 int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out) {
   ANNEAL_CHECK(nrhs >= 1 && nrhs <= nrhs_max_, "nrhs out of range for this context");
+  if (!coarse_valid_ || (coarse_stale_ && rebuild_each_solve)) rebuild_coarse();
   const GridDims& g0 = lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
   const size_t v64 = (size_t)nrhs * 3 * ld * sizeof(double);

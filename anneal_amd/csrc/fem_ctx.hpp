@@ -43,6 +43,14 @@ class FemContext {
   FemContext& operator=(const FemContext&) = delete;
 
   void set_Ee(const double* Ee_dev);
+  // Refresh the coarsest level's dense inverse for the current Ee (the
+  // expensive part of the hierarchy; the rediscretised levels and Jacobi
+  // diagonals follow every set_Ee).  A stale inverse is still a symmetric
+  // positive definite coarse solve, so PCG stays correct with it; only the
+  // iteration count can grow.  This mirrors the reference's "rebuild" of its
+  // AMG hierarchy (fem.py:206-216, optimize.py:120-124).
+  void rebuild_coarse();
+  bool rebuild_each_solve = true;  // solve() refreshes a stale coarse inverse
   void apply_f32(const float* u, float* y, int nrhs);
   void apply_f64(const double* u, double* y, int nrhs);
   // F, U: device, [nrhs][3][ldn] FP64; U holds x0 when use_x0.  Returns 0 on
@@ -104,6 +112,7 @@ class FemContext {
   double* host_red_ = nullptr;  // pinned
   int tile_ = 0;
   bool tuned_ = false;
+  bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   void drop_graphs();

@@ -272,73 +272,82 @@ __global__ void k_coarsen_Ee(const float* __restrict__ Ef, float* __restrict__ E
   Ec[e] = float(s * 0.125);
 }
 
-// Full-weighting restriction b_c = P^T r (P = trilinear prolongation).
+// 1-D stencil of the trilinear prolongation seen from coarse index I: fine
+// nodes 2I-1, 2I, 2I+1 with weights 1/2, 1, 1/2 (zero outside [0, n]);
+// out-of-range indices are clamped so every load stays inside the grid.
+__device__ __forceinline__ void fw_stencil(int I, int n, int (&idx)[3], float (&w)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int i = 2 * I - 1 + t;
+    const bool ok = i >= 0 && i <= n;
+    idx[t] = ok ? i : 0;
+    w[t] = ok ? (t == 1 ? 1.0f : 0.5f) : 0.0f;
+  }
+}
+
+// Full-weighting restriction b_c = P^T r (P = trilinear prolongation), one
+// coarse node per thread, all three components, fixed summation order.
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ maskc, int nx,
                            int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
   const int64_t nc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
   const int64_t N = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (N >= nc) return;
+  const int r = blockIdx.y;
   const int I = N % (cx + 1), J = (N / (cx + 1)) % (cy + 1), K = N / ((int64_t)(cx + 1) * (cy + 1));
   const uint8_t mk = maskc[N];
-  const int sx = nx + 1, sxy = (nx + 1) * (ny + 1);
-  for (int r = 0; r < nrhs; ++r) {
-    for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-      if ((mk >> c) & 1) {
-        const T* src = rf + (int64_t)(3 * r + c) * ldf;
-        for (int dk = -1; dk <= 1; ++dk) {
-          const int k = 2 * K + dk;
-          if (k < 0 || k > nz) continue;
-          const double wk = dk == 0 ? 1.0 : 0.5;
-          for (int dj = -1; dj <= 1; ++dj) {
-            const int j = 2 * J + dj;
-            if (j < 0 || j > ny) continue;
-            const double wj = wk * (dj == 0 ? 1.0 : 0.5);
-            for (int di = -1; di <= 1; ++di) {
-              const int i = 2 * I + di;
-              if (i < 0 || i > nx) continue;
-              s += wj * (di == 0 ? 1.0 : 0.5) * (double)src[i + (int64_t)sx * j + (int64_t)sxy * k];
-            }
-          }
-        }
+  int ix[3], iy[3], iz[3];
+  float wx[3], wy[3], wz[3];
+  fw_stencil(I, nx, ix, wx);
+  fw_stencil(J, ny, iy, wy);
+  fw_stencil(K, nz, iz, wz);
+  const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T* src = rf + (int64_t)(3 * r + c) * ldf;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const float w = wz[a] * wy[b];
+        const T* row = src + iz[a] * sxy + iy[b] * sx;
+        acc[c] += w * (wx[0] * (float)row[ix[0]] + wx[1] * (float)row[ix[1]] + wx[2] * (float)row[ix[2]]);
       }
-      bc[(int64_t)(3 * r + c) * ldc + N] = T(s);
-    }
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) bc[(int64_t)(3 * r + c) * ldc + N] = ((mk >> c) & 1) ? T(acc[c]) : T(0);
 }
 
-// Prolongation with correction: x_f += M_f P x_c.
+// Prolongation with correction: x_f += M_f P x_c (up to 8 coarse nodes per fine node).
 template <typename T>
 __global__ void k_prolong_add(const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ maskf, int nx,
                               int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
   const int64_t nf = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (n >= nf) return;
-  const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
+  const int r = blockIdx.y;
   const uint8_t mk = maskf[n];
   if (!mk) return;
-  const int csx = cx + 1, csxy = (cx + 1) * (cy + 1);
-  int I[2], J[2], K[2];
-  double wi[2], wj[2], wk[2];
-  int ni = 1, nj = 1, nk = 1;
-  I[0] = i >> 1; wi[0] = 1.0;
-  if (i & 1) { I[1] = I[0] + 1; wi[0] = wi[1] = 0.5; ni = 2; }
-  J[0] = j >> 1; wj[0] = 1.0;
-  if (j & 1) { J[1] = J[0] + 1; wj[0] = wj[1] = 0.5; nj = 2; }
-  K[0] = k >> 1; wk[0] = 1.0;
-  if (k & 1) { K[1] = K[0] + 1; wk[0] = wk[1] = 0.5; nk = 2; }
-  for (int r = 0; r < nrhs; ++r)
-    for (int c = 0; c < 3; ++c) {
-      if (!((mk >> c) & 1)) continue;
-      const T* src = xc + (int64_t)(3 * r + c) * ldc;
-      double s = 0.0;
-      for (int a = 0; a < nk; ++a)
-        for (int b = 0; b < nj; ++b)
-          for (int q = 0; q < ni; ++q)
-            s += wk[a] * wj[b] * wi[q] * (double)src[I[q] + (int64_t)csx * J[b] + (int64_t)csxy * K[a]];
-      xf[(int64_t)(3 * r + c) * ldf + n] += T(s);
-    }
+  const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
+  // coarse neighbours I0 = i/2 and I1 = I0 + (i odd) per axis; with weight 1/2
+  // on each of the pair (which coincide for even i) the 8-corner sum times
+  // 1/8 is exactly the trilinear interpolation
+  const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+  const int I1 = I0 + (i & 1), J1 = J0 + (j & 1), K1 = K0 + (k & 1);
+  const int64_t csx = cx + 1, csxy = (int64_t)(cx + 1) * (cy + 1);
+  const int64_t o00 = I0 + csx * J0 + csxy * K0, o01 = I1 + csx * J0 + csxy * K0;
+  const int64_t o10 = I0 + csx * J1 + csxy * K0, o11 = I1 + csx * J1 + csxy * K0;
+  const int64_t p00 = I0 + csx * J0 + csxy * K1, p01 = I1 + csx * J0 + csxy * K1;
+  const int64_t p10 = I0 + csx * J1 + csxy * K1, p11 = I1 + csx * J1 + csxy * K1;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((mk >> c) & 1)) continue;
+    const T* s = xc + (int64_t)(3 * r + c) * ldc;
+    const float v = (((float)s[o00] + (float)s[o01]) + ((float)s[o10] + (float)s[o11])) +
+                    (((float)s[p00] + (float)s[p01]) + ((float)s[p10] + (float)s[p11]));
+    xf[(int64_t)(3 * r + c) * ldf + n] += T(0.125f * v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,14 +437,14 @@ void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const Grid
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(c.nnode(), 256)), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(c.nnode(), 256), nrhs), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
                      c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
   ANNEAL_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(f.nnode(), 256)), dim3(256), 0, s, xc, xf, maskf, f.nx, f.ny, f.nz,
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(f.nnode(), 256), nrhs), dim3(256), 0, s, xc, xf, maskf, f.nx, f.ny, f.nz,
                      c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
   ANNEAL_HIP(hipGetLastError());
 }

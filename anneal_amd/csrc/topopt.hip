@@ -63,9 +63,9 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   const int by = bid % nty;
   const int bz = bid / nty;
   const int X0 = bx * FTX, Y0 = by * FTY, Z0 = bz * FTZ;
-  double lam = 0.0, mv = 0.0;
+  double rlam = 0.0, mv = 0.0;
   if (MODE == kOc) {
-    lam = a.oc[kOcLam];
+    rlam = 1.0 / sqrt(a.oc[kOcLam]);
     mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
   }
   for (int idx = threadIdx.x; idx < LX * LY * LZ; idx += FTX * FTY) {
@@ -82,13 +82,12 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
           v = a.in[e];
         } else if (MODE == kBwd) {
           v = a.in[e] * a.invHs[e];
-        } else {  // kOc: trial design of optimize.py:150-153
+        } else {  // kOc: trial design of optimize.py:150-153, with sqrt(c/dv) precomputed
           if (f & kFrozen) {
             v = 1.0;
           } else {
             const double x = a.in[e];
-            const double ratio = sqrt(fmax(-a.dcx[e], 0.0) / fmax(a.dvx[e] * lam, 1e-300));
-            v = fmin(fmax(x * ratio, fmax(0.0, x - mv)), fmin(1.0, x + mv));
+            v = fmin(fmax(x * (a.dcx[e] * rlam), fmax(0.0, x - mv)), fmin(1.0, x + mv));
           }
         }
       }
@@ -251,6 +250,16 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
   oc[kOcHi] = hi;
 }
 
+// sqrt(max(-dcx, 0) / dvx) per element, so a trial needs one multiply by
+// lam^-1/2 (dvx = 0 with -dcx > 0 gives +inf, which the move limit clips).
+__global__ void k_oc_prep(const double* __restrict__ dcx, const double* __restrict__ dvx, int64_t n,
+                          double* __restrict__ r0) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double c = fmax(-dcx[e], 0.0);
+  r0[e] = c > 0.0 ? sqrt(c / dvx[e]) : 0.0;
+}
+
 __global__ void k_oc_final(const double* __restrict__ x, const double* __restrict__ dcx, const double* __restrict__ dvx,
                            const uint8_t* __restrict__ flags, const double* __restrict__ oc, double beta, double move,
                            int64_t n, double* __restrict__ xnew) {
@@ -324,6 +333,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&flags_, n));
   ANNEAL_HIP(hipMalloc(&Hs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&r0_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&part_, std::max(nblk_, 6 * 512) * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&oc_, kOcLen * sizeof(double)));
   ANNEAL_HIP(hipHostMalloc((void**)&host_, kOcLen * sizeof(double), hipHostMallocDefault));
@@ -346,6 +356,7 @@ FilterContext::~FilterContext() {
   (void)hipFree(flags_);
   (void)hipFree(Hs_);
   (void)hipFree(invHs_);
+  (void)hipFree(r0_);
   (void)hipFree(part_);
   (void)hipFree(oc_);
   (void)hipHostFree(host_);
@@ -411,10 +422,13 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   st[kOcVt] = vtarget;
   st[kOcGrow] = 1.25;
   ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
+  const int64_t ne = g_.nelem();
+  hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_);
+  ANNEAL_HIP(hipGetLastError());
   FilterArgs a{};
   a.in = x;
-  a.dcx = dcx;
-  a.dvx = dvx;
+  a.dcx = r0_;
+  a.dvx = nullptr;
   a.flags = flags_;
   a.invHs = invHs_;
   a.phi = phi;
@@ -571,13 +585,17 @@ __global__ __launch_bounds__(256) void k_design_stats(const double* __restrict__
     part[q * gridDim.x + blockIdx.x] = v;
   }
 }
-__global__ __launch_bounds__(64) void k_design_stats_final(const double* __restrict__ part, int nblk,
-                                                           double* __restrict__ out) {
-  if (threadIdx.x >= 6) return;
-  const int q = threadIdx.x;
+// One wavefront per statistic, fixed-order tree over the block partials.
+__global__ __launch_bounds__(384) void k_design_stats_final(const double* __restrict__ part, int nblk,
+                                                            double* __restrict__ out) {
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double v = 0.0;
-  for (int b = 0; b < nblk; ++b) v = (q == 2) ? fmax(v, part[q * nblk + b]) : v + part[q * nblk + b];
-  out[q] = v;
+  for (int b = lane; b < nblk; b += 64) v = (q == 2) ? fmax(v, part[q * nblk + b]) : v + part[q * nblk + b];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off, 64);
+    v = (q == 2) ? fmax(v, o) : v + o;
+  }
+  if (lane == 0) out[q] = v;
 }
 
 void element_energy(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
@@ -609,7 +627,7 @@ void design_stats(const double* xold, const double* xnew, const double* xphys, c
   hipLaunchKernelGGL(k_design_stats, dim3(kStatBlocks), dim3(256), 0, s, xold, xnew, xphys, phi, dcx, dvx, flags, n,
                      part);
   ANNEAL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_design_stats_final, dim3(1), dim3(64), 0, s, part, kStatBlocks, out_dev);
+  hipLaunchKernelGGL(k_design_stats_final, dim3(1), dim3(384), 0, s, part, kStatBlocks, out_dev);
   ANNEAL_HIP(hipGetLastError());
 }
 

@@ -53,7 +53,7 @@ class FilterContext {
   double w_[125];
   hipStream_t s_;
   uint8_t* flags_ = nullptr;
-  double *Hs_ = nullptr, *invHs_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
+  double *Hs_ = nullptr, *invHs_ = nullptr, *r0_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
   int nblk_ = 0;
 };
 

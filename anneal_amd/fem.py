@@ -184,6 +184,7 @@ class FEM:
         red[self.free] = np.arange(self.nfree)
         self.red = red
         self.device = torch.device("cuda", torch.cuda.current_device())
+        self.set_option("rebuild_each_solve", 0)   # the coarse inverse follows solve(rebuild=...)
         self._active_dev = torch.from_numpy(act_u8).to(self.device)
         self._current = None
         self.amg_builds = 0
@@ -265,7 +266,7 @@ class FEM:
         """K u = f for every f in F_list (fem.py:206-232); returns free-dof float64 vectors."""
         self._activate(K)
         if rebuild or self.amg_builds == 0:
-            self.amg_builds += 1
+            self.rebuild()
         out = []
         for s in range(0, len(F_list), self.nrhs_max):
             Fs = [np.asarray(f, float) for f in F_list[s:s + self.nrhs_max]]
@@ -276,6 +277,11 @@ class FEM:
             self.cg_iters.extend(iters)
             out.extend(self._from_device(U, len(Fs)))
         return out
+
+    def rebuild(self):
+        """Refresh the multigrid hierarchy's coarse inverse for the current operator (fem.py:207-216)."""
+        _lib.call("anneal_fem_rebuild", self._ctx)
+        self.amg_builds += 1
 
     def solve_device(self, F, U, nrhs, rtol, maxiter, use_x0):
         """Device-resident solve on SoA FP64 blocks [nrhs, 3, ldn]; returns PCG iterations per rhs."""

@@ -240,7 +240,7 @@ class SimpRun:
         last = fem.cg_iters[-1] if fem.cg_iters else None
         rebuild = self.base_iters is None or (last is not None and last > 1.6 * self.base_iters)
         if rebuild:
-            fem.amg_builds += 1
+            fem.rebuild()
         its = fem.solve_device(self.F, self.U, self.ncase, self.rtol, self.maxiter, use_x0=self.have_u)
         fem.cg_iters.extend(its)
         self.have_u = True

@@ -58,6 +58,12 @@ int anneal_fem_info(void* ctx, int64_t* ldn, int* levels, uint8_t* mask_host);
  * scales (0 on inactive voxels).  No matrix is formed: this sets the operator
  * and rebuilds the multigrid levels. */
 int anneal_fem_set_Ee(void* ctx, const double* Ee_dev);
+/* Refresh the expensive part of the multigrid hierarchy (the coarsest
+ * level's dense inverse) for the current Ee: the counterpart of the
+ * reference's AMG rebuild (fem.py:207-216).  Between rebuilds the solve keeps
+ * the previous coarse inverse (still a valid SPD coarse solve), unless the
+ * option "rebuild_each_solve" is 1. */
+int anneal_fem_rebuild(void* ctx);
 /* y = K u on free dofs (fixed/unused dofs give 0): the product the reference
  * forms with its CSR matrix (fem.py:201), FP32 and FP64 variants. */
 int anneal_fem_apply_f32(void* ctx, const float* u_dev, float* y_dev, int nrhs);
@@ -80,7 +86,9 @@ int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host
  * eagerly), "smoother_degree" (Chebyshev degree per side, >= 1),
  * "cheb_ratio" (lambda_max / lambda_min of the smoothed interval),
  * "inner_tol_floor" (relative residual reduction asked of each FP32 inner
- * PCG), "inner_max" (iterations per inner PCG).  Unknown names fail. */
+ * PCG), "inner_max" (iterations per inner PCG), "rebuild_each_solve" (1 =
+ * every solve after a new Ee refreshes the coarse inverse, 0 = only
+ * anneal_fem_rebuild does).  Unknown names fail. */
 int anneal_fem_set_option(void* ctx, const char* name, double value);
 /* One V-cycle of the multigrid preconditioner on FP32 vectors (z = M r). */
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs);

@@ -189,3 +189,34 @@ def test_autotune_shapes_agree(torch):
         assert torch.allclose(o, outs[0], rtol=2e-6, atol=1e-6 * outs[0].abs().max().item())
     chosen, us = fem.autotune(-1)
     assert 0 <= chosen < 7 and len(us) == 7
+
+
+@pytest.mark.parametrize("shape,inactive", [((40, 20, 20), 0.0), ((33, 17, 11), 0.2)])
+def test_vcycle_is_symmetric_positive_definite(torch, shape, inactive):
+    """The preconditioner must be SPD for PCG: <r1, M r2> = <r2, M r1> and <r, M r> > 0.
+
+    Exercises restriction (P^T), prolongation (P), the Chebyshev smoother and the
+    coarse inverse together: a restriction that is not the transpose of the
+    prolongation breaks the symmetry at once.
+    """
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem(shape, 11, inactive)
+    fem = _product_fem(g, active, fixed)
+    fem.assemble(Ee[fem.act])
+    _lib.call("anneal_fem_rebuild", fem._ctx)
+    mask = np.zeros((3, fem.ldn))
+    mask[:, :g.nnode] = ((fem.dof_mask[None, :] >> np.arange(3)[:, None]) & 1)
+    rng = np.random.default_rng(3)
+    rs = [torch.from_numpy((rng.standard_normal((3, fem.ldn)) * mask)[None]).float().cuda() for _ in range(3)]
+    zs = []
+    for r in rs:
+        z = torch.zeros_like(r)
+        _lib.call("anneal_fem_vcycle", fem._ctx, _lib.ptr(r), _lib.ptr(z), 1)
+        zs.append(z)
+    torch.cuda.synchronize()
+    d = lambda a, b: float((a.double() * b.double()).sum())
+    for i in range(3):
+        assert d(rs[i], zs[i]) > 0
+        for j in range(i + 1, 3):
+            a, b = d(rs[i], zs[j]), d(rs[j], zs[i])
+            assert abs(a - b) <= 1e-5 * (abs(a) + abs(b)), (a, b)
