@@ -164,7 +164,7 @@ int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, do
 int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host) {
   return guarded([&] {
     FemHandle* h = fem_of(ctx);
-    const int c = h->fem->autotune(variant, us_host);
+    const int c = variant == -2 ? h->fem->tile() : h->fem->autotune(variant, us_host);
     if (chosen_host) *chosen_host = c;
     if (n_host) *n_host = apply_variants();
     return 0;

@@ -216,6 +216,12 @@ class FEM:
                   _lib.ctypes.byref(n))
         return chosen.value, ([us[i] for i in range(n.value)] if variant < 0 else [])
 
+    def tile(self):
+        """Tile shape id in use by the FP32 apply."""
+        chosen = _lib.ctypes.c_int()
+        _lib.call("anneal_fem_autotune", self._ctx, -2, _lib.ctypes.byref(chosen), None, None)
+        return chosen.value
+
     # -- layout helpers (AoS dof 3n+c on the host <-> SoA [c][n] on the device)
     def _to_device(self, vecs, reduced=True):
         torch = _torch()

@@ -195,6 +195,7 @@ class SimpRun:
         self.cs_dev = torch.zeros(self.ncase, **f64)
         self.coef_dev = torch.zeros(self.ncase, **f64)
         self.stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.timing = timers is not None      # per-phase timing synchronises the device five times per step
         self.T = timers if timers is not None else {}
         for k in ("physical", "solve", "sensitivity", "oc", "stats"):
             self.T.setdefault(k, 0.0)
@@ -212,7 +213,8 @@ class SimpRun:
         self.oc_evals = 0
 
     def _tick(self):
-        self.torch.cuda.synchronize()
+        if self.timing:
+            self.torch.cuda.synchronize()
         return time.perf_counter()
 
     def step(self):

@@ -76,8 +76,9 @@ int anneal_fem_apply_f64(void* ctx, const double* u_dev, double* y_dev, int nrhs
 int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, double rtol, int maxiter, int use_x0,
                      int* iters_host);
 /* Tile-shape autotuner of the FP32 apply (no reference counterpart: the
- * reference has no kernels).  variant >= 0 selects a shape; variant < 0 times
- * all shapes on this grid and keeps the fastest.  *chosen_host receives the
+ * reference has no kernels).  variant >= 0 selects a shape; variant == -2
+ * only reports the shape in use; any other negative value times all shapes
+ * on this grid and keeps the fastest.  *chosen_host receives the
  * shape in use; us_host (may be NULL; room for 16 floats) receives *n_host per-shape times in
  * microseconds, where *n_host is the number of shapes. */
 int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host);
