@@ -319,7 +319,7 @@ template <>
 ApplyArgs<float> FemContext::args<float>(int l) const {
   const Level& L = lv_[l];
   ApplyArgs<float> a{};
-  a.variant = tile_;
+  a.variant = L.tile;
   a.Ee = L.Ee;
   a.mask = L.mask;
   a.dinv = L.dinv;
@@ -355,7 +355,7 @@ void FemContext::set_Ee(const double* Ee_dev) {
   cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
   for (size_t l = 1; l < lv_.size(); ++l) launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, s_);
   for (size_t l = 0; l < lv_.size(); ++l) launch_inv_diag(lv_[l].Ee, lv_[l].mask, lv_[l].dinv, lv_[l].g, lv_[l].kdiag, s_);
-  if (!tuned_ && g0.nelem() >= 100000) {
+  if (!tuned_ && g0.nelem() >= 4096) {
     tuned_ = true;
     autotune(-1, nullptr);  // first operator on a large grid: pick the fastest tile shape
   }
@@ -384,44 +384,46 @@ void FemContext::apply_f64(const double* u, double* y, int nrhs) {
 
 int FemContext::autotune(int variant, float* us_out) {
   const int nv = apply_variants();
-  drop_graphs();  // the tile shape is baked into captured V-cycles
+  drop_graphs();  // tile shapes are baked into captured V-cycles
   if (variant >= 0) {
     ANNEAL_CHECK(variant < nv, "unknown apply tile variant");
-    tile_ = variant;
-    return tile_;
+    for (auto& L : lv_) L.tile = variant;
+    return variant;
   }
-  // time the fine-level FP32 apply (one rhs) for every tile shape; keep the fastest
-  const GridDims& g0 = lv_[0].g;
+  // time the FP32 apply (one rhs) of every tile shape on every level with a
+  // stencil; keep the fastest per level
   hipEvent_t e0, e1;
   ANNEAL_HIP(hipEventCreate(&e0));
   ANNEAL_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  int arg = 0;
-  for (int v = 0; v < nv; ++v) {
-    tile_ = v;
-    ApplyArgs<float> a = args<float>(0);
-    a.u = pp_;
-    a.y = pq_;
-    launch_apply(a, kApply, 1, s_);  // warm
-    ANNEAL_HIP(hipEventRecord(e0, s_));
-    const int reps = 10;
-    for (int i = 0; i < reps; ++i) launch_apply(a, kApply, 1, s_);
-    ANNEAL_HIP(hipEventRecord(e1, s_));
-    ANNEAL_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    ANNEAL_HIP(hipEventElapsedTime(&ms, e0, e1));
-    const float us = 1e3f * ms / reps;
-    if (us_out) us_out[v] = us;
-    if (us < best) {
-      best = us;
-      arg = v;
+  for (size_t l = 0; l + 1 < lv_.size() || (l == 0 && lv_.size() == 1); ++l) {
+    Level& L = lv_[l];
+    float best = 1e30f;
+    int arg = 0;
+    for (int v = 0; v < nv; ++v) {
+      L.tile = v;
+      ApplyArgs<float> a = args<float>((int)l);
+      a.u = l == 0 ? pp_ : L.x;
+      a.y = l == 0 ? pq_ : L.r;
+      launch_apply(a, kApply, 1, s_);  // warm
+      ANNEAL_HIP(hipEventRecord(e0, s_));
+      const int reps = 8;
+      for (int i = 0; i < reps; ++i) launch_apply(a, kApply, 1, s_);
+      ANNEAL_HIP(hipEventRecord(e1, s_));
+      ANNEAL_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      ANNEAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const float us = 1e3f * ms / reps;
+      if (l == 0 && us_out) us_out[v] = us;
+      if (us < best) {
+        best = us;
+        arg = v;
+      }
     }
+    L.tile = arg;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)g0;
-  tile_ = arg;
-  return tile_;
+  return lv_[0].tile;
 }
 
 void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
@@ -445,17 +447,30 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   // 2*nu_pre - 1 buffer swaps in total: start in x2 so the result lands in x.
   float* cur = L.x2;
   float* nxt = x;
-  jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
-  for (int k = 1; k < nu_pre; ++k) {
-    a.u = cur;
-    a.prev = (k == 1) ? nullptr : nxt;
+  if (nu_pre == 2) {
+    // steps 1 and 2 fused: x1 = omega D^-1 b is formed inside the kernel,
+    // x2 lands where the unfused sequence would put it
+    a.u = nullptr;
+    a.prev = nullptr;
     a.y = nxt;
-    a.c1 = float(c1[k]);
-    a.c2 = float(c2[k]);
-    launch_apply(a, kCheb, nrhs, s_);
+    a.omega = float(1.0 / th);
+    a.c1 = float(c1[1]);
+    a.c2 = float(c2[1]);
+    launch_apply(a, kCheb0, nrhs, s_);
     std::swap(cur, nxt);
+  } else {
+    jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
+    for (int k = 1; k < nu_pre; ++k) {
+      a.u = cur;
+      a.prev = (k == 1) ? nullptr : nxt;
+      a.y = nxt;
+      a.c1 = float(c1[k]);
+      a.c2 = float(c2[k]);
+      launch_apply(a, kCheb, nrhs, s_);
+      std::swap(cur, nxt);
+    }
   }
   a.u = cur;
   a.y = L.r;
@@ -496,7 +511,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
   std::vector<int> its(nrhs, 0);
   std::vector<char> done(nrhs, 0), idone(nrhs, 0);
   std::vector<double> hst((size_t)nrhs * kSt, 0.0);
-  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, tile_);
+  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile);
   int status = 1;
   int total = 0;
   bool bad = false;

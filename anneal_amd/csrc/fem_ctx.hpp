@@ -27,6 +27,7 @@ struct Level {
   float *x = nullptr, *x2 = nullptr, *b = nullptr, *r = nullptr;  // nrhs_max x 3 x ldn
   KeCoef<float> k{};
   double kdiag = 0.0;
+  int tile = 0;  // FP32 apply tile shape for this level (autotuned)
 };
 
 struct SolveStats {
@@ -57,16 +58,18 @@ class FemContext {
   // convergence, 1 when maxiter was exhausted, 2 on a non-finite result.
   int solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out);
   void vcycle(int l, const float* b, float* x, int nrhs);
-  // Tile shape of the FP32 apply: variant >= 0 sets it, variant < 0 times every
-  // shape on the fine grid and keeps the fastest.  us_out (may be null)
-  // receives the per-shape microseconds.  Returns the shape in use.
+  // Tile shape of the FP32 apply: variant >= 0 sets it on every level,
+  // variant < 0 times every shape on every multigrid level and keeps the
+  // fastest per level (coarse levels are latency-bound and prefer short z
+  // chunks).  us_out (may be null) receives the fine level's per-shape
+  // microseconds.  Returns the fine level's shape.
   int autotune(int variant, float* us_out);
   // The fine-level V-cycle z = M r on the PCG buffers, replayed from a hipGraph
   // captured once per rhs count (pointers and smoother coefficients are fixed;
   // Ee, dinv and the coarse inverse are read from fixed buffers).
   void vcycle_pcg(int nrhs);
   bool use_graphs = true;
-  int tile() const { return tile_; }
+  int tile() const { return lv_[0].tile; }
 
   const GridDims& grid() const { return lv_[0].g; }
   int levels() const { return (int)lv_.size(); }
@@ -110,7 +113,6 @@ class FemContext {
   double *x64_ = nullptr, *r64_ = nullptr;
   double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
   double* host_red_ = nullptr;  // pinned
-  int tile_ = 0;
   bool tuned_ = false;
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs

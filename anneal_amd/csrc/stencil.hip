@@ -88,8 +88,16 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     for (int k = 0; k < NLD; ++k) {
       const bool ok = pin && lok[k];
       const int64_t n = lbase[k] + sxy * (ok ? p : 0);
+      if (MODE == kCheb0) {
+        // first smoothing step from a zero guess: u = M omega D^-1 b
+        const uint8_t m = ok ? a.mask[n] : uint8_t(0);
+        const T d = ok ? a.omega * T(a.dinv[n]) : T(0);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pre[k][c] = ok ? u[c * ldn + n] : T(0);
+        for (int c = 0; c < 3; ++c) pre[k][c] = ((m >> c) & 1) ? d * bb[c * ldn + n] : T(0);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pre[k][c] = ok ? u[c * ldn + n] : T(0);
+      }
     }
   };
   auto fetch_e = [&](int L) -> T { return (ecol && L >= 0 && L < nz) ? Ecol[exy * L] : T(0); };
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) bv[c] = bb[c * ldn + nout];
       }
-      if (MODE == kJacobi || MODE == kCheb) dv = a.dinv[nout];
+      if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0) dv = a.dinv[nout];
       if (MODE == kCheb && pr) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) pv[c] = pr[c * ldn + nout];
@@ -203,6 +211,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             if (MODE == kApply) out = v;
             else if (MODE == kResid) out = bv[c] - v;
             else if (MODE == kJacobi) out = uc + a.omega * T(dv) * (bv[c] - v);
+            else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(dv) * (bv[c] - v);
             else out = uc + a.c1 * (uc - pv[c]) + a.c2 * T(dv) * (bv[c] - v);
             if (!((mk >> c) & 1)) out = T(0);
             y[c * ldn + nout] = out;
@@ -359,7 +368,7 @@ struct Tile {
   static constexpr int tx = TX, ty = TY, zc = ZC;
 };
 #define ANNEAL_TILES(X) X(0, 16, 16, 8) X(1, 16, 16, 16) X(2, 32, 8, 8) X(3, 32, 8, 16) X(4, 32, 16, 8) \
-  X(5, 16, 16, 4) X(6, 64, 4, 16)
+  X(5, 16, 16, 4) X(6, 64, 4, 16) X(7, 16, 16, 2) X(8, 8, 8, 2) X(9, 8, 8, 4) X(10, 16, 8, 2)
 
 static void tile_dims(int variant, int& TX, int& TY, int& ZC) {
   switch (variant) {
@@ -375,7 +384,7 @@ static void tile_dims(int variant, int& TX, int& TY, int& ZC) {
   }
 }
 
-int apply_variants() { return 7; }
+int apply_variants() { return 11; }
 
 int apply_blocks(int nx, int ny, int nz, int variant) {
   int TX, TY, ZC;
@@ -416,6 +425,7 @@ void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s) {
     case kResid: launch_apply_mode<T, kResid>(a, nrhs, v, s); break;
     case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, v, s); break;
     case kCheb: launch_apply_mode<T, kCheb>(a, nrhs, v, s); break;
+    case kCheb0: launch_apply_mode<T, kCheb0>(a, nrhs, v, s); break;
     default: throw std::invalid_argument("bad apply mode");
   }
 }

@@ -15,6 +15,7 @@ enum ApplyMode : int {
   kResid = 1,   // y = M (b - K u)
   kJacobi = 2,  // y = M (u + omega * dinv * (b - K u))
   kCheb = 3,    // y = M (u + c1 (u - prev) + c2 * dinv * (b - K u)); prev may be null (0) or alias y
+  kCheb0 = 4,   // kCheb with u = M omega dinv b formed on the fly and prev = 0 (first two smoothing steps fused)
 };
 
 template <typename T>

@@ -51,7 +51,8 @@ def apply_roofline(fem, reps=20, nbuf=8):
     import torch
     from anneal_amd import _lib
     g = fem.grid
-    shapes = ["16,16,8", "16,16,16", "32,8,8", "32,8,16", "32,16,8", "16,16,4", "64,4,16"]
+    shapes = ["16,16,8", "16,16,16", "32,8,8", "32,8,16", "32,16,8", "16,16,4", "64,4,16", "16,16,2", "8,8,2",
+              "8,8,4", "16,8,2"]
     tile = shapes[fem.tile()] + ",kApply"
     bufs = [(torch.randn(1, 3, fem.ldn, device="cuda", dtype=torch.float32),
              torch.empty(1, 3, fem.ldn, device="cuda", dtype=torch.float32)) for _ in range(nbuf)]

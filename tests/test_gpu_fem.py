@@ -179,7 +179,7 @@ def test_autotune_shapes_agree(torch):
     fem.assemble(Ee[fem.act])
     U = torch.randn(1, 3, fem.ldn, dtype=torch.float32, device="cuda")
     outs = []
-    for v in range(7):
+    for v in range(11):
         fem.autotune(v)
         Y = torch.zeros_like(U)
         _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(Y), 1)
@@ -188,7 +188,7 @@ def test_autotune_shapes_agree(torch):
     for o in outs[1:]:
         assert torch.allclose(o, outs[0], rtol=2e-6, atol=1e-6 * outs[0].abs().max().item())
     chosen, us = fem.autotune(-1)
-    assert 0 <= chosen < 7 and len(us) == 7
+    assert 0 <= chosen < 11 and len(us) == 11
 
 
 @pytest.mark.parametrize("shape,inactive", [((40, 20, 20), 0.0), ((33, 17, 11), 0.2)])
