@@ -243,6 +243,41 @@ void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* k
 // ---------------------------------------------------------------------------
 // PCG pieces (FP32 vectors, FP64 scalars on the device).
 //   st layout per rhs r: st[r*kSt + ...]
+// Reduce per-rhs partials (fixed order) straight into the PCG scalar block
+// and finish the scalar update: op 0 stores, op 1 stores pq and sets alpha,
+// op 2 takes rz_new and sets beta (and rz).
+__global__ __launch_bounds__(kRedThreads) void k_finalize_st(const double* __restrict__ part, int nblk,
+                                                             double* __restrict__ st, int field, int op) {
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedThreads) s += part[(int64_t)r * nblk + i];
+  __shared__ double sh[kRedThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
+  double* q = st + r * kSt;
+  q[field] = t;
+  const bool live = q[kDone] == 0.0;
+  if (op == 1) {
+    q[kAlpha] = (live && t > 0.0 && isfinite(t)) ? q[kRZ] / t : 0.0;
+  } else if (op == 2) {
+    q[kBeta] = (live && q[kRZ] > 0.0) ? t / q[kRZ] : 0.0;
+    if (live) q[kRZ] = t;
+  }
+}
+void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, int op, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_st, dim3(nrhs), dim3(kRedThreads), 0, s, part, nblk, st, field, op);
+  ANNEAL_HIP(hipGetLastError());
+}
+void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s) {
+  hipLaunchKernelGGL((k_dot_partial<float, float>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld,
+                     part);
+  ANNEAL_HIP(hipGetLastError());
+}
+
 __global__ void k_pcg_alpha(double* st, int nrhs) {
   const int r = threadIdx.x;
   if (r >= nrhs) return;

@@ -181,14 +181,17 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
     } else if (n == "smoother_degree") {
       ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree must be in [1, 15]");
       f->nu_pre = (int)value;
-      f->autotune(f->tile(), nullptr);  // drops captured graphs
+      f->invalidate_graphs();
     } else if (n == "cheb_ratio") {
       ANNEAL_CHECK(value > 1.0, "cheb_ratio must exceed 1");
       f->cheb_ratio = value;
-      f->autotune(f->tile(), nullptr);
+      f->invalidate_graphs();
     } else if (n == "inner_tol_floor") {
       ANNEAL_CHECK(value > 0.0 && value < 1.0, "inner_tol_floor must be in (0, 1)");
       f->inner_tol_floor = value;
+    } else if (n == "fuse_first_step") {
+      f->fuse_first_step = value != 0.0;
+      f->invalidate_graphs();
     } else if (n == "rebuild_each_solve") {
       f->rebuild_each_solve = value != 0.0;
     } else if (n == "inner_max") {

@@ -405,14 +405,17 @@ int FemContext::autotune(int variant, float* us_out) {
       a.u = l == 0 ? pp_ : L.x;
       a.y = l == 0 ? pq_ : L.r;
       launch_apply(a, kApply, 1, s_);  // warm
-      ANNEAL_HIP(hipEventRecord(e0, s_));
-      const int reps = 8;
-      for (int i = 0; i < reps; ++i) launch_apply(a, kApply, 1, s_);
-      ANNEAL_HIP(hipEventRecord(e1, s_));
-      ANNEAL_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      ANNEAL_HIP(hipEventElapsedTime(&ms, e0, e1));
-      const float us = 1e3f * ms / reps;
+      float us = 1e30f;
+      for (int round = 0; round < 3; ++round) {  // best of three: small grids are noisy
+        ANNEAL_HIP(hipEventRecord(e0, s_));
+        const int reps = 8;
+        for (int i = 0; i < reps; ++i) launch_apply(a, kApply, 1, s_);
+        ANNEAL_HIP(hipEventRecord(e1, s_));
+        ANNEAL_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        ANNEAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        us = std::min(us, 1e3f * ms / reps);
+      }
       if (l == 0 && us_out) us_out[v] = us;
       if (us < best) {
         best = us;
@@ -449,7 +452,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   float* nxt = x;
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
-  if (nu_pre == 2) {
+  if (nu_pre == 2 && fuse_first_step) {
     // steps 1 and 2 fused: x1 = omega D^-1 b is formed inside the kernel,
     // x2 lands where the unfused sequence would put it
     a.u = nullptr;
@@ -553,9 +556,8 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
     vcycle_pcg(nrhs);
-    dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
-    ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZ, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
-                                hipMemcpyDeviceToDevice, s_));
+    dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+    finalize_st(part_, kDotBlocks, nrhs, st_, kRZ, 0, s_);
     pcg_dir(pp_, pz_, st_, n, ld, nrhs, 1, s_);
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
       ApplyArgs<float> af = args<float>(0);
@@ -563,10 +565,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       af.y = pq_;
       af.dot_part = part_;
       launch_apply(af, kApply, nrhs, s_);
-      finalize(part_, nblk_apply, nrhs, red_, s_);
-      ANNEAL_HIP(hipMemcpy2DAsync(st_ + kPQ, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
-                                  hipMemcpyDeviceToDevice, s_));
-      pcg_alpha(st_, nrhs, s_);
+      finalize_st(part_, nblk_apply, nrhs, st_, kPQ, 1, s_);  // pq, alpha
       pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_);
       ANNEAL_HIP(hipMemcpyAsync(host_red_ + 64, red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
       ANNEAL_HIP(hipStreamSynchronize(s_));
@@ -590,10 +589,8 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
                                     sizeof(double), nrhs, hipMemcpyHostToDevice, s_));
       }
       vcycle_pcg(nrhs);
-      dot(pr_, pz_, n, ld, nrhs, part_, red_, s_);
-      ANNEAL_HIP(hipMemcpy2DAsync(st_ + kRZnew, kSt * sizeof(double), red_, sizeof(double), sizeof(double), nrhs,
-                                  hipMemcpyDeviceToDevice, s_));
-      pcg_beta(st_, nrhs, s_);
+      dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+      finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
       pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
     }
     if (bad) break;

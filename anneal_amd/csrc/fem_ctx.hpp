@@ -69,6 +69,8 @@ class FemContext {
   // Ee, dinv and the coarse inverse are read from fixed buffers).
   void vcycle_pcg(int nrhs);
   bool use_graphs = true;
+  void invalidate_graphs() { drop_graphs(); }
+  bool fuse_first_step = false;  // form x1 = omega D^-1 b inside the first Chebyshev kernel
   int tile() const { return lv_[0].tile; }
 
   const GridDims& grid() const { return lv_[0].g; }
