@@ -262,6 +262,7 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
     scale_ = (double*)alloc(64 * sizeof(double));
     ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
     ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
+    ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
     vgraph_.assign(nrhs_max + 1, nullptr);
   } catch (...) {
     for (void* p : owned_) (void)hipFree(p);
@@ -311,6 +312,7 @@ FemContext::~FemContext() {
   (void)hipStreamSynchronize(s_);
   drop_graphs();
   if (cap_s_) (void)hipStreamDestroy(cap_s_);
+  if (ev_) (void)hipEventDestroy(ev_);
   for (void* p : owned_) (void)hipFree(p);
   if (host_red_) (void)hipHostFree(host_red_);
 }
@@ -559,6 +561,11 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
     finalize_st(part_, kDotBlocks, nrhs, st_, kRZ, 0, s_);
     pcg_dir(pp_, pz_, st_, n, ld, nrhs, 1, s_);
+    // rr of the last two iterations (per rhs) to predict convergence: when the
+    // next residual is not expected to meet the tolerance, the next V-cycle,
+    // dot and direction update (they read r and write z, p, rz, beta only) are
+    // queued before the host waits for rr, so the device never idles on it.
+    std::vector<double> rr1(nrhs, 0.0), rr2(nrhs, 0.0);
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
       ApplyArgs<float> af = args<float>(0);
       af.u = pp_;
@@ -568,13 +575,29 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       finalize_st(part_, nblk_apply, nrhs, st_, kPQ, 1, s_);  // pq, alpha
       pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_);
       ANNEAL_HIP(hipMemcpyAsync(host_red_ + 64, red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
-      ANNEAL_HIP(hipStreamSynchronize(s_));
+      ANNEAL_HIP(hipEventRecord(ev_, s_));
+      bool speculate = it >= 2;
+      for (int r = 0; r < nrhs && speculate; ++r) {
+        if (idone[r]) continue;
+        const double rate = rr2[r] > 0.0 ? rr1[r] / rr2[r] : 1.0;
+        if (rr1[r] * rate <= 2.0 * tol2[r]) speculate = false;  // may converge now: do not queue work
+      }
+      auto next_direction = [&]() {
+        vcycle_pcg(nrhs);
+        dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+        finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
+        pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+      };
+      if (speculate) next_direction();
+      ANNEAL_HIP(hipEventSynchronize(ev_));
       ++total;
       bool iall = true, changed = false;
       for (int r = 0; r < nrhs; ++r) {
         if (idone[r]) continue;
         ++its[r];
         const double rr = host_red_[64 + r];
+        rr2[r] = rr1[r];
+        rr1[r] = rr;
         if (!std::isfinite(rr)) bad = true;
         if (rr <= tol2[r]) {
           idone[r] = 1;
@@ -588,10 +611,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         ANNEAL_HIP(hipMemcpy2DAsync(st_ + kDone, kSt * sizeof(double), hst.data() + kDone, kSt * sizeof(double),
                                     sizeof(double), nrhs, hipMemcpyHostToDevice, s_));
       }
-      vcycle_pcg(nrhs);
-      dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
-      finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
-      pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+      if (!speculate) next_direction();
     }
     if (bad) break;
     for (int r = 0; r < nrhs; ++r) sc[r] = done[r] ? 0.0 : rnorm[r];

@@ -118,6 +118,7 @@ class FemContext {
   bool tuned_ = false;
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
+  hipEvent_t ev_ = nullptr;                 // marks the residual-norm copy of a PCG step
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   void drop_graphs();
 };
