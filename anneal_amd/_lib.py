@@ -23,6 +23,7 @@ SIGNATURES = {
                           c_int, c_void_p],
     "anneal_fem_destroy": [c_void_p],
     "anneal_fem_info": [c_void_p, c_i64p, c_intp, c_void_p],
+    "anneal_fem_level_info": [c_void_p, c_int, c_intp, c_intp, c_intp, c_intp],
     "anneal_fem_set_Ee": [c_void_p, c_void_p],
     "anneal_fem_rebuild": [c_void_p],
     "anneal_fem_apply_f32": [c_void_p, c_void_p, c_void_p, c_int],

@@ -110,6 +110,19 @@ int anneal_fem_info(void* ctx, int64_t* ldn, int* levels, uint8_t* mask_host) {
   });
 }
 
+int anneal_fem_level_info(void* ctx, int level, int* nx, int* ny, int* nz, int* tile) {
+  return guarded([&] {
+    FemContext* f = fem_of(ctx)->fem;
+    ANNEAL_CHECK(level >= 0 && level < f->levels(), "level out of range");
+    const Level& L = f->level(level);
+    if (nx) *nx = L.g.nx;
+    if (ny) *ny = L.g.ny;
+    if (nz) *nz = L.g.nz;
+    if (tile) *tile = L.tile;
+    return 0;
+  });
+}
+
 int anneal_fem_set_Ee(void* ctx, const double* Ee_dev) {
   return guarded([&] {
     ANNEAL_CHECK(Ee_dev, "null Ee");

@@ -216,6 +216,15 @@ class FEM:
                   _lib.ctypes.byref(n))
         return chosen.value, ([us[i] for i in range(n.value)] if variant < 0 else [])
 
+    def level_info(self):
+        """[(nx, ny, nz, tile shape id)] per multigrid level, finest first."""
+        out = []
+        for l in range(self.levels):
+            v = [_lib.ctypes.c_int() for _ in range(4)]
+            _lib.call("anneal_fem_level_info", self._ctx, l, *[_lib.ctypes.byref(x) for x in v])
+            out.append(tuple(x.value for x in v))
+        return out
+
     def tile(self):
         """Tile shape id in use by the FP32 apply."""
         chosen = _lib.ctypes.c_int()

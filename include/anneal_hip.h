@@ -54,6 +54,8 @@ int anneal_fem_destroy(void* ctx);
  * free-dof mask (nnode bytes, bit c = dof 3n+c free) copied to mask_host if
  * not NULL. */
 int anneal_fem_info(void* ctx, int64_t* ldn, int* levels, uint8_t* mask_host);
+/* Multigrid level l (0 = finest): grid dimensions and FP32 apply tile shape id. */
+int anneal_fem_level_info(void* ctx, int level, int* nx, int* ny, int* nz, int* tile);
 /* FEM.assemble(Ee) (fem.py:198-201): Ee_dev holds nelem FP64 stiffness
  * scales (0 on inactive voxels).  No matrix is formed: this sets the operator
  * and rebuilds the multigrid levels. */
