@@ -27,6 +27,32 @@ __device__ __forceinline__ KeCoef<T> scaled(const KeCoef<T>& k, T e) {
   return KeCoef<T>{k.nAmB * e, k.nB * e, k.g * e, k.pc * e, k.pd * e, k.tAmB * e, k.tB * e, k.w * e};
 }
 
+// Raw buffer access.  A resource covers one node plane (or one element
+// layer) of one array; lanes outside the tile's valid region use an
+// out-of-range offset, so their loads return 0 and their stores are dropped
+// by the hardware range check, with no per-lane selects or branches.
+constexpr uint32_t kOob = 0xFFFFFFF0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, 0, 0);
+  } else {
+    using V2 = __attribute__((ext_vector_type(2))) uint32_t;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, 0, 0);
+  }
+}
+
 // Block index -> (tile x, tile y, z chunk).  Consecutive workgroups are
 // dealt round-robin to the 8 XCDs; remap so that each XCD receives a
 // contiguous run of logical tiles (z chunks fastest), keeping the plane and
@@ -62,45 +88,63 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
   const T* pr = (MODE == kCheb && a.prev) ? a.prev + (int64_t)3 * r * ldn : nullptr;
 
-  // load slots of this thread in the (TX+1) x (TY+1) node tile
-  int64_t lbase[NLD];
-  bool lok[NLD];
+  // Addressing: every access goes through a buffer resource for one plane
+  // (uniform, scalar registers) plus a per-thread byte offset that is fixed
+  // along z; out-of-range lanes carry kOob.
+  uint32_t lvo[NLD];  // byte offsets of this thread's load slots in a node plane
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
     const int idx = tid + k * NT;
     const int lx = idx % LX, ly = idx / LX;
     const int gx = X0 - 1 + lx, gy = Y0 - 1 + ly;
-    lok[k] = idx < NP && gx >= 0 && gx <= nx && gy >= 0 && gy <= ny;
-    lbase[k] = lok[k] ? gx + (int64_t)sx * gy : 0;
+    const bool ok = idx < NP && gx >= 0 && gx <= nx && gy >= 0 && gy <= ny;
+    lvo[k] = ok ? uint32_t(gx + sx * gy) * uint32_t(sizeof(T)) : kOob;
   }
   const int ei = X0 - 1 + tx, ej = Y0 - 1 + ty;
   const bool ecol = (ei >= 0) && (ei < nx) && (ej >= 0) && (ej < ny);
-  const T* __restrict__ Ecol = a.Ee + (ecol ? ei + (int64_t)nx * ej : 0);
+  const uint32_t evo = ecol ? uint32_t(ei + nx * ej) * uint32_t(sizeof(T)) : kOob;
   const int64_t exy = (int64_t)nx * ny;
   const int ni = X0 + tx, nj = Y0 + ty;
   const bool ncol = (tx < TX - 1) && (ty < TY - 1) && (ni <= nx) && (nj <= ny);
-  const int64_t nbase = ncol ? ni + (int64_t)sx * nj : 0;
+  const uint32_t nvo = ncol ? uint32_t(ni + sx * nj) : kOob;   // element index (scaled per array below)
+  const uint32_t pbytes = uint32_t(sxy) * uint32_t(sizeof(T));
 
   T pre[NLD][3];
   auto fetch = [&](int p) {
     const bool pin = (p >= 0) && (p <= nz);
+    const int64_t pb = sxy * (pin ? p : 0);
+    const uint32_t nb = pin ? pbytes : 0u;
+    if (MODE == kCheb0) {
+      // first smoothing step from a zero guess: u = M omega D^-1 b
+      const __amdgpu_buffer_rsrc_t rm = rsrc(a.mask + pb, pin ? uint32_t(sxy) : 0u);
+      const __amdgpu_buffer_rsrc_t rd = rsrc(a.dinv + pb, pin ? uint32_t(sxy) * 4u : 0u);
 #pragma unroll
-    for (int k = 0; k < NLD; ++k) {
-      const bool ok = pin && lok[k];
-      const int64_t n = lbase[k] + sxy * (ok ? p : 0);
-      if (MODE == kCheb0) {
-        // first smoothing step from a zero guess: u = M omega D^-1 b
-        const uint8_t m = ok ? a.mask[n] : uint8_t(0);
-        const T d = ok ? a.omega * T(a.dinv[n]) : T(0);
+      for (int k = 0; k < NLD; ++k) {
+        const uint32_t node = lvo[k] == kOob ? kOob : lvo[k] / uint32_t(sizeof(T));
+        const uint8_t m = __builtin_amdgcn_raw_buffer_load_b8(rm, node, 0, 0);
+        const T d = a.omega * T(bload<float>(rd, node == kOob ? kOob : node * 4u));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pre[k][c] = ((m >> c) & 1) ? d * bb[c * ldn + n] : T(0);
-      } else {
+        for (int c = 0; c < 3; ++c) {
+          const T bv = bload<T>(rsrc(bb + c * ldn + pb, nb), lvo[k]);
+          pre[k][c] = ((m >> c) & 1) ? d * bv : T(0);
+        }
+      }
+    } else {
+      const __amdgpu_buffer_rsrc_t r0 = rsrc(u + pb, nb), r1 = rsrc(u + ldn + pb, nb),
+                                   r2 = rsrc(u + 2 * ldn + pb, nb);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pre[k][c] = ok ? u[c * ldn + n] : T(0);
+      for (int k = 0; k < NLD; ++k) {
+        pre[k][0] = bload<T>(r0, lvo[k]);
+        pre[k][1] = bload<T>(r1, lvo[k]);
+        pre[k][2] = bload<T>(r2, lvo[k]);
       }
     }
   };
-  auto fetch_e = [&](int L) -> T { return (ecol && L >= 0 && L < nz) ? Ecol[exy * L] : T(0); };
+  auto fetch_e = [&](int L) -> T {
+    const bool lin = L >= 0 && L < nz;
+    const __amdgpu_buffer_rsrc_t re = rsrc(a.Ee + exy * (lin ? L : 0), lin ? uint32_t(exy) * uint32_t(sizeof(T)) : 0u);
+    return bload<T>(re, evo);
+  };
 
   T Fprev[3][4], gup[3][4];
 #pragma unroll
@@ -136,18 +180,20 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     uint8_t mk = 0;
     T bv[3] = {T(0), T(0), T(0)}, pv[3] = {T(0), T(0), T(0)};
     float dv = 0.f;
-    int64_t nout = 0;
-    if (outp && ncol) {
-      nout = nbase + sxy * L;
-      mk = a.mask[nout];
+    const int64_t lb = sxy * (outp ? L : 0);  // uniform base of the output plane
+    const uint32_t nb_out = outp ? pbytes : 0u;
+    const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
+    if (outp) {
+      mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(a.mask + lb, uint32_t(sxy)), nvo, 0, 0);
       if (MODE != kApply) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bv[c] = bb[c * ldn + nout];
+        for (int c = 0; c < 3; ++c) bv[c] = bload<T>(rsrc(bb + c * ldn + lb, nb_out), ovo);
       }
-      if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0) dv = a.dinv[nout];
+      if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
+        dv = bload<float>(rsrc(a.dinv + lb, uint32_t(sxy) * 4u), nvo == kOob ? kOob : nvo * 4u);
       if (MODE == kCheb && pr) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pv[c] = pr[c * ldn + nout];
+        for (int c = 0; c < 3; ++c) pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
       }
     }
     e_cur = e_this;
@@ -163,8 +209,10 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     }
     if (s >= 1) {
       // element layer L between planes p-1 and p
+      // no branch on Ee == 0: lanes of one wave would run both paths, and
+      // the scaled coefficients make an empty element contribute zeros anyway
       T gb[3][4], gt[3][4];
-      if (e_cur != T(0)) {
+      {
         T S[3][4], D[3][4];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -174,14 +222,6 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             D[c][m] = Fcur[c][m] - Fprev[c][m];
           }
         element_walsh(S, D, scaled(a.k, e_cur), gb, gt);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            gb[c][m] = T(0);
-            gt[c][m] = T(0);
-          }
       }
       if (outp) {
         // plane L has both of its element layers: finish it
@@ -198,7 +238,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
           sp[4 * c + 3][tid] = c11;
         }
         __syncthreads();
-        if (ncol) {
+        {
           const T* prevbuf = &su[(s + 2) % 3][0][0];  // plane L
           const int o = (tx + 1) + LX * (ty + 1);
 #pragma unroll
@@ -214,9 +254,9 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(dv) * (bv[c] - v);
             else out = uc + a.c1 * (uc - pv[c]) + a.c2 * T(dv) * (bv[c] - v);
             if (!((mk >> c) & 1)) out = T(0);
-            y[c * ldn + nout] = out;
+            bstore<T>(out, rsrc(y + c * ldn + lb, nb_out), ovo);  // dropped for lanes without a node
             if (MODE == kApply) {
-              if (a.dot_part) dot += (double)uc * (double)out;
+              if (a.dot_part && ncol) dot += (double)uc * (double)out;
             }
           }
         }
