@@ -109,8 +109,10 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   const uint32_t nvo = ncol ? uint32_t(ni + sx * nj) : kOob;   // element index (scaled per array below)
   const uint32_t pbytes = uint32_t(sxy) * uint32_t(sizeof(T));
 
-  T pre[NLD][3];
-  auto fetch = [&](int p) {
+  // Node planes are prefetched two steps ahead (two register slots), so the
+  // load latency of a plane is covered by two steps of compute.
+  T pre[2][NLD][3];
+  auto fetch = [&](int p, int slot) {
     const bool pin = (p >= 0) && (p <= nz);
     const int64_t pb = sxy * (pin ? p : 0);
     const uint32_t nb = pin ? pbytes : 0u;
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T bv = bload<T>(rsrc(bb + c * ldn + pb, nb), lvo[k]);
-          pre[k][c] = ((m >> c) & 1) ? d * bv : T(0);
+          pre[slot][k][c] = ((m >> c) & 1) ? d * bv : T(0);
         }
       }
     } else {
@@ -134,9 +136,9 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
                                    r2 = rsrc(u + 2 * ldn + pb, nb);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
-        pre[k][0] = bload<T>(r0, lvo[k]);
-        pre[k][1] = bload<T>(r1, lvo[k]);
-        pre[k][2] = bload<T>(r2, lvo[k]);
+        pre[slot][k][0] = bload<T>(r0, lvo[k]);
+        pre[slot][k][1] = bload<T>(r1, lvo[k]);
+        pre[slot][k][2] = bload<T>(r2, lvo[k]);
       }
     }
   };
@@ -144,6 +146,37 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     const bool lin = L >= 0 && L < nz;
     const __amdgpu_buffer_rsrc_t re = rsrc(a.Ee + exy * (lin ? L : 0), lin ? uint32_t(exy) * uint32_t(sizeof(T)) : 0u);
     return bload<T>(re, evo);
+  };
+
+  // Operands of the output node in plane L (mask, b, D^-1, previous iterate),
+  // also fetched one step before they are used.
+  struct Epi {
+    uint8_t mk;
+    float dv;
+    T bv[3], pv[3];
+  };
+  const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
+  auto fetch_epi = [&](int L, Epi& E) {
+    const bool ok = L >= Z0 && L <= nz && L < Z0 + ZC;
+    const int64_t lb = sxy * (ok ? L : 0);
+    const uint32_t nb_out = ok ? pbytes : 0u;
+    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(a.mask + lb, ok ? uint32_t(sxy) : 0u), nvo, 0, 0);
+    E.dv = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      E.bv[c] = T(0);
+      E.pv[c] = T(0);
+    }
+    if (MODE != kApply) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rsrc(bb + c * ldn + lb, nb_out), ovo);
+    }
+    if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
+      E.dv = bload<float>(rsrc(a.dinv + lb, ok ? uint32_t(sxy) * 4u : 0u), nvo == kOob ? kOob : nvo * 4u);
+    if (MODE == kCheb && pr) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
+    }
   };
 
   T Fprev[3][4], gup[3][4];
@@ -155,9 +188,11 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
       gup[c][m] = T(0);
     }
   double dot = 0.0;
-  fetch(Z0 - 1);
-  T e_cur = T(0);
-  T e_nxt = fetch_e(Z0 - 1);
+  fetch(Z0 - 1, 0);
+  fetch(Z0, 1);
+  T eq[2] = {fetch_e(Z0 - 1), fetch_e(Z0)};  // layers used at steps 1 and 2
+  Epi ep[2];
+  fetch_epi(Z0, ep[0]);  // plane finished at step 2
 
 #pragma unroll
   for (int s = 0; s <= ZC + 1; ++s) {
@@ -168,35 +203,26 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
       const int idx = tid + k * NT;
       if (idx < NP) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[k][c];
+        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[s & 1][k][c];
       }
     }
-    if (s <= ZC) fetch(p + 1);  // in flight while this plane is processed
-    const T e_this = e_nxt;      // stiffness of layer p-1 (between planes p-1 and p)
-    if (s <= ZC) e_nxt = fetch_e(p);
-    // epilogue operands of the node finished in this step (plane p-1)
+    if (s + 2 <= ZC + 1) fetch(p + 2, s & 1);  // in flight for the next two steps
+    T e_cur = T(0);
+    if (s >= 1) {
+      e_cur = eq[(s - 1) & 1];  // layer p-1
+      if (s + 2 <= ZC + 1) eq[(s - 1) & 1] = fetch_e(p + 1);
+    }
+    // the node finished in this step lies in plane L = p - 1 (steps s >= 2)
     const int L = p - 1;
     const bool outp = (s >= 2) && (L <= nz);
-    uint8_t mk = 0;
-    T bv[3] = {T(0), T(0), T(0)}, pv[3] = {T(0), T(0), T(0)};
-    float dv = 0.f;
+    Epi E;
+    if (s >= 2) {
+      E = ep[s & 1];
+      if (s + 1 <= ZC + 1) fetch_epi(L + 1, ep[(s + 1) & 1]);
+    }
     const int64_t lb = sxy * (outp ? L : 0);  // uniform base of the output plane
     const uint32_t nb_out = outp ? pbytes : 0u;
-    const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
-    if (outp) {
-      mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(a.mask + lb, uint32_t(sxy)), nvo, 0, 0);
-      if (MODE != kApply) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) bv[c] = bload<T>(rsrc(bb + c * ldn + lb, nb_out), ovo);
-      }
-      if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
-        dv = bload<float>(rsrc(a.dinv + lb, uint32_t(sxy) * 4u), nvo == kOob ? kOob : nvo * 4u);
-      if (MODE == kCheb && pr) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
-      }
-    }
-    e_cur = e_this;
+
     __syncthreads();
     T Fcur[3][4];
     {
@@ -249,11 +275,11 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             const T uc = prevbuf[c * NP + o];
             T out;
             if (MODE == kApply) out = v;
-            else if (MODE == kResid) out = bv[c] - v;
-            else if (MODE == kJacobi) out = uc + a.omega * T(dv) * (bv[c] - v);
-            else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(dv) * (bv[c] - v);
-            else out = uc + a.c1 * (uc - pv[c]) + a.c2 * T(dv) * (bv[c] - v);
-            if (!((mk >> c) & 1)) out = T(0);
+            else if (MODE == kResid) out = E.bv[c] - v;
+            else if (MODE == kJacobi) out = uc + a.omega * T(E.dv) * (E.bv[c] - v);
+            else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(E.dv) * (E.bv[c] - v);
+            else out = uc + a.c1 * (uc - E.pv[c]) + a.c2 * T(E.dv) * (E.bv[c] - v);
+            if (!((E.mk >> c) & 1)) out = T(0);
             bstore<T>(out, rsrc(y + c * ldn + lb, nb_out), ovo);  // dropped for lanes without a node
             if (MODE == kApply) {
               if (a.dot_part && ncol) dot += (double)uc * (double)out;
