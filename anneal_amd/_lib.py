@@ -47,6 +47,8 @@ SIGNATURES = {
     "anneal_design_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
                             c_void_p, c_void_p, c_void_p],
     "anneal_dot_f64": [c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p, c_void_p],
+    "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
+                                    c_void_p, c_int64, c_void_p, c_void_p],
 }
 _RESTYPE = {"anneal_last_error": ctypes.c_char_p}
 

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "fem_ctx.hpp"
+#include "fields.hpp"
 #include "topopt.hpp"
 
 namespace anneal {
@@ -330,6 +331,20 @@ int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int6
   return guarded([&] {
     ANNEAL_CHECK(a_dev && b_dev && out_dev && work_dev && nrhs >= 1, "null argument");
     dot(a_dev, b_dev, nnode, ldn, nrhs, work_dev, out_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_fields_corner_values(const float* u_dev, const float* vm_dev, int nx, int ny, int nz, double h,
+                                const double* origin_host, const double* tri_dev, const double* nrm_dev,
+                                int64_t nface, float* out_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && h > 0.0, "need a non-empty grid and h > 0");
+    ANNEAL_CHECK(nface >= 0, "nface must be >= 0");
+    ANNEAL_CHECK(origin_host, "null origin");
+    ANNEAL_CHECK(nface == 0 || (u_dev && vm_dev && tri_dev && nrm_dev && out_dev), "null argument");
+    corner_values(u_dev, vm_dev, nx, ny, nz, h, origin_host, tri_dev, nrm_dev, nface, out_dev,
+                  (hipStream_t)stream);
     return 0;
   });
 }

@@ -149,6 +149,23 @@ int anneal_design_stats(const double* xold_dev, const double* xnew_dev, const do
 int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs, double* out_dev,
                    double* work_dev, void* stream);
 
+/* ---- result fields on a display mesh (anneal/fields.py) ------------------ */
+
+/* fields.corner_values (fields.py:17-37) after the npz has been read
+ * (fields.load, fields.py:10-14): out_dev[3*nface][4] float32 =
+ * [dx, dy, dz, vonMises] per triangle corner, in triangle order.
+ *   u_dev   [nnode][3] float32 nodal displacement ("u::case", node index
+ *           i + (nx+1)(j + (ny+1)k));
+ *   vm_dev  [nelem] float32 element von Mises ("vm::case");
+ *   origin_host  3 doubles ("origin"); h ("h"); nx, ny, nz ("shape");
+ *   tri_dev [nface][3][3] float64 corner points (mesh.triangles);
+ *   nrm_dev [nface][3] float64 face normals (mesh.face_normals).
+ * Displacement: trilinear, coordinates clamped to the node box; stress: 2x2x2
+ * grey dilation of vm, order-0 lookup half a voxel inside the surface. */
+int anneal_fields_corner_values(const float* u_dev, const float* vm_dev, int nx, int ny, int nz, double h,
+                                const double* origin_host, const double* tri_dev, const double* nrm_dev,
+                                int64_t nface, float* out_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
