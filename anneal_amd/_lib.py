@@ -47,6 +47,22 @@ SIGNATURES = {
     "anneal_design_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
                             c_void_p, c_void_p, c_void_p],
     "anneal_dot_f64": [c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p, c_void_p],
+    "anneal_unet_create": [ctypes.POINTER(c_void_p), c_int, c_int, c_intp, c_int, c_void_p],
+    "anneal_unet_destroy": [c_void_p],
+    "anneal_unet_nparams": [c_void_p, c_i64p],
+    "anneal_unet_get_params": [c_void_p, c_void_p],
+    "anneal_unet_set_params": [c_void_p, c_void_p],
+    "anneal_unet_get_grads": [c_void_p, c_void_p],
+    "anneal_unet_forward": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p],
+    "anneal_unet_backward": [c_void_p, c_void_p],
+    "anneal_unet_zero_grad": [c_void_p],
+    "anneal_unet_adam": [c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int,
+                         ctypes.c_float],
+    "anneal_unet_rel_mse": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                            c_void_p],
+    "anneal_unet_restrict": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, ctypes.c_float, c_void_p],
+    "anneal_unet_prolong": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int, c_int,
+                            ctypes.c_float, c_int, c_void_p],
     "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p],
 }

@@ -4,10 +4,12 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "fem_ctx.hpp"
 #include "fields.hpp"
 #include "topopt.hpp"
+#include "unet.hpp"
 
 namespace anneal {
 static thread_local std::string g_err;
@@ -41,6 +43,10 @@ struct FemHandle {
 inline FemHandle* fem_of(void* p) {
   if (!p) throw std::invalid_argument("null FEM context");
   return static_cast<FemHandle*>(p);
+}
+inline UNet* net_of(void* p) {
+  if (!p) throw std::invalid_argument("null U-Net context");
+  return static_cast<UNet*>(p);
 }
 inline FilterContext* filt_of(void* p) {
   if (!p) throw std::invalid_argument("null filter context");
@@ -345,6 +351,117 @@ int anneal_fields_corner_values(const float* u_dev, const float* vm_dev, int nx,
     ANNEAL_CHECK(nface == 0 || (u_dev && vm_dev && tri_dev && nrm_dev && out_dev), "null argument");
     corner_values(u_dev, vm_dev, nx, ny, nz, h, origin_host, tri_dev, nrm_dev, nface, out_dev,
                   (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_unet_create(void** ctx, int cin, int cout, const int* widths, int levels, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(ctx && widths && levels >= 2 && levels <= 6, "bad U-Net arguments");
+    *ctx = nullptr;
+    std::vector<int> w(widths, widths + levels);
+    *ctx = new UNet(cin, cout, w, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_unet_destroy(void* ctx) {
+  return guarded([&] {
+    delete static_cast<UNet*>(ctx);
+    return 0;
+  });
+}
+
+int anneal_unet_nparams(void* ctx, int64_t* n) {
+  return guarded([&] {
+    ANNEAL_CHECK(n, "null argument");
+    *n = net_of(ctx)->nparams();
+    return 0;
+  });
+}
+
+int anneal_unet_get_params(void* ctx, float* host) {
+  return guarded([&] {
+    ANNEAL_CHECK(host, "null argument");
+    net_of(ctx)->get_params(host);
+    return 0;
+  });
+}
+
+int anneal_unet_set_params(void* ctx, const float* host) {
+  return guarded([&] {
+    ANNEAL_CHECK(host, "null argument");
+    net_of(ctx)->set_params(host);
+    return 0;
+  });
+}
+
+int anneal_unet_get_grads(void* ctx, float* host) {
+  return guarded([&] {
+    ANNEAL_CHECK(host, "null argument");
+    net_of(ctx)->get_grads(host);
+    return 0;
+  });
+}
+
+int anneal_unet_forward(void* ctx, const uint16_t* in_dev, int B, int nx, int ny, int nz, float* out_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(in_dev && out_dev, "null argument");
+    ANNEAL_CHECK(B >= 1 && nx >= 1 && ny >= 1 && nz >= 1, "empty input");
+    net_of(ctx)->forward(in_dev, NetDims{B, nx, ny, nz}, out_dev);
+    return 0;
+  });
+}
+
+int anneal_unet_backward(void* ctx, const float* dout_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(dout_dev, "null argument");
+    net_of(ctx)->backward(dout_dev);
+    return 0;
+  });
+}
+
+int anneal_unet_zero_grad(void* ctx) {
+  return guarded([&] {
+    net_of(ctx)->zero_grad();
+    return 0;
+  });
+}
+
+int anneal_unet_adam(void* ctx, float lr, float beta1, float beta2, float eps, int step, float weight_decay) {
+  return guarded([&] {
+    net_of(ctx)->adam(lr, beta1, beta2, eps, step, weight_decay);
+    return 0;
+  });
+}
+
+int anneal_unet_rel_mse(const float* o_dev, const float* t_dev, int B, int nx, int ny, int nz, int c, float* dout_dev,
+                        double* loss_dev, double* work_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(o_dev && t_dev && loss_dev && work_dev, "null argument");
+    ANNEAL_CHECK(B >= 1 && B <= 64 && c >= 1, "need 1 <= B <= 64 samples");
+    net_rel_mse(o_dev, t_dev, NetDims{B, nx, ny, nz}, c, dout_dev, loss_dev, work_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_unet_restrict(const uint16_t* x_dev, int B, int fx, int fy, int fz, int c, uint16_t* y_dev, float s,
+                         void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && y_dev && c % 8 == 0 && c > 0, "bad restrict arguments");
+    const NetDims f{B, fx, fy, fz}, co{B, coarse_n(fx), coarse_n(fy), coarse_n(fz)};
+    net_restrict(View{(bf16*)x_dev, c, c}, f, View{y_dev, c, c}, co, s, nullptr, false, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_unet_prolong(const uint16_t* x_dev, int B, int cx, int cy, int cz, int c, uint16_t* y_dev, int fx, int fy,
+                        int fz, float s, int accumulate, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && y_dev && c % 8 == 0 && c > 0, "bad prolong arguments");
+    ANNEAL_CHECK(cx == coarse_n(fx) && cy == coarse_n(fy) && cz == coarse_n(fz), "coarse dims must be n/2 + 1");
+    net_prolong(View{(bf16*)x_dev, c, c}, NetDims{B, cx, cy, cz}, View{y_dev, c, c}, NetDims{B, fx, fy, fz}, s,
+                nullptr, accumulate != 0, (hipStream_t)stream);
     return 0;
   });
 }

@@ -166,6 +166,36 @@ int anneal_fields_corner_values(const float* u_dev, const float* vm_dev, int nx,
                                 const double* origin_host, const double* tri_dev, const double* nrm_dev,
                                 int64_t nface, float* out_dev, void* stream);
 
+/* ---- learned preconditioner: 3-D U-Net (no counterpart in the reference) ---
+ * Channels-last bf16 tensors on node grids (voxel i + nx (j + ny k), sample
+ * stride nx ny nz); convolutions are BF16 MFMA implicit GEMMs with FP32
+ * accumulation.  Flat parameter order: for the encoder levels (conv a, conv b),
+ * then the decoder levels from the finest (conv a, conv b), then the 1x1 output
+ * convolution; each layer W[co][tap][ci] (tap = dx+1 + 3(dy+1) + 9(dz+1))
+ * followed by b[co].  Used by anneal_amd/learned.py. */
+int anneal_unet_create(void** ctx, int cin, int cout, const int* widths, int levels, void* stream);
+int anneal_unet_destroy(void* ctx);
+int anneal_unet_nparams(void* ctx, int64_t* n);
+int anneal_unet_get_params(void* ctx, float* host);
+int anneal_unet_set_params(void* ctx, const float* host);
+int anneal_unet_get_grads(void* ctx, float* host);
+/* out_dev[B][V][cout] FP32 = net(in_dev[B][V][cin] bf16), V = nx ny nz nodes. */
+int anneal_unet_forward(void* ctx, const uint16_t* in_dev, int B, int nx, int ny, int nz, float* out_dev);
+/* Accumulate parameter gradients of the last forward for dL/dout (FP32). */
+int anneal_unet_backward(void* ctx, const float* dout_dev);
+int anneal_unet_zero_grad(void* ctx);
+int anneal_unet_adam(void* ctx, float lr, float beta1, float beta2, float eps, int step, float weight_decay);
+/* loss_dev = mean_b sum (o - t)^2 / sum t^2 over each sample; dout_dev (may be
+ * null) its gradient.  work_dev: 130 B doubles. */
+int anneal_unet_rel_mse(const float* o_dev, const float* t_dev, int B, int nx, int ny, int nz, int c, float* dout_dev,
+                        double* loss_dev, double* work_dev, void* stream);
+/* Node-aligned grid transfer of bf16 channels-last tensors (c a multiple of 8):
+ * restrict y[I] = s sum_d w(d) x[2I+d]; prolong y[i] (+)= s sum_parents w x. */
+int anneal_unet_restrict(const uint16_t* x_dev, int B, int fx, int fy, int fz, int c, uint16_t* y_dev, float s,
+                         void* stream);
+int anneal_unet_prolong(const uint16_t* x_dev, int B, int cx, int cy, int cz, int c, uint16_t* y_dev, int fx, int fy,
+                        int fz, float s, int accumulate, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
