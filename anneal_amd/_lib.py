@@ -63,6 +63,9 @@ SIGNATURES = {
     "anneal_unet_restrict": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, ctypes.c_float, c_void_p],
     "anneal_unet_prolong": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int, c_int,
                             ctypes.c_float, c_int, c_void_p],
+    "anneal_fem_set_learned": [c_void_p, c_void_p, c_int, c_double, c_int],
+    "anneal_fem_learned_stage": [c_void_p, c_void_p, c_void_p, c_int],
+    "anneal_fem_learned_pair": [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
     "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p],
 }

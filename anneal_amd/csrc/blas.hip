@@ -266,6 +266,11 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize_st(const double* __res
   } else if (op == 2) {
     q[kBeta] = (live && q[kRZ] > 0.0) ? t / q[kRZ] : 0.0;
     if (live) q[kRZ] = t;
+  } else if (op == 3) {
+    const double pq = q[kPQ];
+    q[kBeta] = (live && pq > 0.0 && isfinite(t)) ? -t / pq : 0.0;
+  } else if (op == 4) {
+    if (live) q[kRZ] = t;
   }
 }
 void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, int op, hipStream_t s) {

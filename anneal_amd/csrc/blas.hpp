@@ -8,7 +8,7 @@ constexpr int kDotBlocks = 512;  // fixed reduction grid: results do not depend 
 
 // Per right-hand-side PCG scalars on the device (FP64).
 constexpr int kSt = 8;
-constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kRR = 3, kAlpha = 4, kBeta = 5, kDone = 6;
+constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kRR = 3, kAlpha = 4, kBeta = 5, kDone = 6, kZQ = 7;
 
 template <typename T>
 struct KeCoef;
@@ -39,6 +39,8 @@ void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_
 void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
                            int ny, int nz, hipStream_t s);
 void pcg_alpha(double* st, int nrhs, hipStream_t s);
+// st[field] = sum of the partials, then op: 0 none; 1 alpha = rz / t; 2 beta = t / rz, rz = t;
+// 3 (flexible) beta = -t / pq with t = <z_new, K p>; 4 rz = t.
 void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, int op, hipStream_t s);
 void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s);
 void pcg_beta(double* st, int nrhs, hipStream_t s);

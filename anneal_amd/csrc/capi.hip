@@ -466,4 +466,32 @@ int anneal_unet_prolong(const uint16_t* x_dev, int B, int cx, int cy, int cz, in
   });
 }
 
+int anneal_fem_set_learned(void* fem, void* unet, int level, double e_ref, int mode) {
+  return guarded([&] {
+    FemHandle* h = fem_of(fem);
+    h->fem->set_learned(unet ? net_of(unet) : nullptr, level, e_ref, mode);
+    h->fem->invalidate_graphs();
+    return 0;
+  });
+}
+
+int anneal_fem_learned_stage(void* fem, const float* r_dev, float* z_dev, int nrhs) {
+  return guarded([&] {
+    ANNEAL_CHECK(r_dev && z_dev && nrhs >= 1, "null argument");
+    FemHandle* h = fem_of(fem);
+    ANNEAL_CHECK(h->fem->learned_mode() != 0, "no network attached (anneal_fem_set_learned)");
+    h->fem->learned_stage(r_dev, z_dev, nrhs);
+    return 0;
+  });
+}
+
+int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev, int nrhs, int level, double e_ref,
+                            uint16_t* in_dev, float* tgt_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(F_dev && U_dev && in_dev && tgt_dev, "null argument");
+    fem_of(fem)->fem->learned_pair(F_dev, U_dev, nrhs, level, e_ref, in_dev, tgt_dev);
+    return 0;
+  });
+}
+
 }  // extern "C"

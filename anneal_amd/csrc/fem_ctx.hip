@@ -1,5 +1,7 @@
 // Multigrid hierarchy and mixed-precision PCG driver (host side of the solve).
 #include "fem_ctx.hpp"
+#include "learned.hpp"
+#include "unet.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -277,7 +279,112 @@ void FemContext::drop_graphs() {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  for (auto& kv : vgraph_bx_)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
+  vgraph_bx_.clear();
 }
+
+void FemContext::vcycle_graph(const float* b, float* x, int nrhs) {
+  if (!use_graphs) {
+    vcycle(0, b, x, nrhs);
+    return;
+  }
+  const auto key = std::make_tuple(nrhs, (const void*)b, (void*)x);
+  auto it = vgraph_bx_.find(key);
+  if (it == vgraph_bx_.end()) {
+    hipStream_t saved = s_;
+    s_ = cap_s_;
+    hipGraph_t g = nullptr;
+    try {
+      ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
+      vcycle(0, b, x, nrhs);
+      ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
+    } catch (...) {
+      s_ = saved;
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(cap_s_, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    s_ = saved;
+    hipGraphExec_t exec = nullptr;
+    ANNEAL_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    it = vgraph_bx_.emplace(key, exec).first;
+  }
+  ANNEAL_HIP(hipGraphLaunch(it->second, s_));
+}
+
+void FemContext::set_learned(UNet* net, int level, double e_ref, int mode) {
+  ANNEAL_CHECK(mode >= 0 && mode <= 2, "learned mode must be 0, 1 or 2");
+  if (!net || mode == 0) {
+    net_ = nullptr;
+    net_mode_ = 0;
+    return;
+  }
+  ANNEAL_CHECK(level >= 1 && level < (int)lv_.size(), "the network level must be a coarse multigrid level");
+  ANNEAL_CHECK(e_ref > 0.0 && std::isfinite(e_ref), "e_ref must be positive");
+  ANNEAL_CHECK(net->cin() == kNetCin && net->cout() == kNetCout, "the network must map 8 input to 3 output channels");
+  const GridDims& g = lv_[level].g;
+  if (!net_in_ || level != net_level_) {
+    net_in_ = (uint16_t*)alloc((size_t)nrhs_max_ * g.nnode() * kNetCin * sizeof(uint16_t));
+    net_out_ = (float*)alloc((size_t)nrhs_max_ * g.nnode() * kNetCout * sizeof(float));
+  }
+  if (!net_nrm_) net_nrm_ = (double*)alloc(4 * 64 * sizeof(double));
+  if (!pz1_) {
+    const size_t v32 = (size_t)nrhs_max_ * 3 * lv_[0].g.ldn() * sizeof(float);
+    pz1_ = (float*)alloc(v32);
+    pt_ = (float*)alloc(v32);
+  }
+  net_ = net;
+  net_level_ = level;
+  net_eref_ = e_ref;
+  net_mode_ = mode;
+}
+
+void FemContext::learned_stage(const float* r, float* z, int nrhs) {
+  ANNEAL_CHECK(net_, "no network attached");
+  const float* src = r;
+  for (int l = 1; l <= net_level_; ++l) {
+    launch_restrict(src, lv_[l].b, lv_[l].mask, lv_[l - 1].g, lv_[l].g, nrhs, s_);
+    src = lv_[l].b;
+  }
+  Level& N = lv_[net_level_];
+  dot_partial(N.b, N.b, N.g.nnode(), N.g.ldn(), nrhs, part_, s_);
+  finalize(part_, kDotBlocks, nrhs, net_nrm_, s_);
+  net_pack(N.Ee, N.mask, N.b, net_nrm_, N.g, nrhs, net_eref_, net_in_, s_);
+  net_->forward(net_in_, NetDims{nrhs, N.g.nx + 1, N.g.ny + 1, N.g.nz + 1}, net_out_);
+  net_unpack(net_out_, N.mask, net_nrm_, N.g, nrhs, net_eref_, N.x, s_);
+  for (int l = net_level_; l >= 1; --l) {
+    float* dst = l == 1 ? z : lv_[l - 1].x;
+    fill_zero(dst, (int64_t)nrhs * 3 * lv_[l - 1].g.ldn(), s_);
+    launch_prolong_add(lv_[l].x, dst, lv_[l - 1].mask, lv_[l - 1].g, lv_[l].g, nrhs, s_);
+  }
+}
+
+void FemContext::learned_pair(const double* F, const double* U, int nrhs, int level, double e_ref, uint16_t* in,
+                              float* tgt) {
+  ANNEAL_CHECK(nrhs >= 1 && nrhs <= nrhs_max_, "nrhs out of range for this context");
+  ANNEAL_CHECK(level >= 1 && level < (int)lv_.size(), "the network level must be a coarse multigrid level");
+  ANNEAL_CHECK(e_ref > 0.0, "e_ref must be positive");
+  if (!net_nrm_) net_nrm_ = (double*)alloc(4 * 64 * sizeof(double));
+  const GridDims& g0 = lv_[0].g;
+  for (int r = 0; r < nrhs; ++r)
+    for (int c = 0; c < 3; ++c) cast_f64_f32(F + (size_t)(3 * r + c) * g0.ldn(), pr_ + (size_t)(3 * r + c) * g0.ldn(),
+                                             g0.nnode(), s_);
+  const float* src = pr_;
+  for (int l = 1; l <= level; ++l) {
+    launch_restrict(src, lv_[l].b, lv_[l].mask, lv_[l - 1].g, lv_[l].g, nrhs, s_);
+    src = lv_[l].b;
+  }
+  Level& N = lv_[level];
+  dot_partial(N.b, N.b, N.g.nnode(), N.g.ldn(), nrhs, part_, s_);
+  finalize(part_, kDotBlocks, nrhs, net_nrm_, s_);
+  net_pack(N.Ee, N.mask, N.b, net_nrm_, N.g, nrhs, e_ref, in, s_);
+  net_target(U, g0, level, N.g, net_nrm_, nrhs, e_ref, tgt, s_);
+}
+
+
 
 void FemContext::vcycle_pcg(int nrhs) {
   if (!use_graphs) {
@@ -557,7 +664,8 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     ANNEAL_HIP(hipMemcpyAsync(st_, hst.data(), hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
-    vcycle_pcg(nrhs);
+    if (net_ && net_mode_ == 1) learned_warm_start(nrhs);
+    precondition(nrhs);
     dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
     finalize_st(part_, kDotBlocks, nrhs, st_, kRZ, 0, s_);
     pcg_dir(pp_, pz_, st_, n, ld, nrhs, 1, s_);
@@ -582,10 +690,17 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         const double rate = rr2[r] > 0.0 ? rr1[r] / rr2[r] : 1.0;
         if (rr1[r] * rate <= 2.0 * tol2[r]) speculate = false;  // may converge now: do not queue work
       }
+      const bool flexible = net_ && net_mode_ == 2;
       auto next_direction = [&]() {
-        vcycle_pcg(nrhs);
+        precondition(nrhs);
         dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
-        finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
+        if (flexible) {
+          finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 4, s_);  // rz for the next alpha
+          dot_partial(pz_, pq_, n, ld, nrhs, part_, s_);
+          finalize_st(part_, kDotBlocks, nrhs, st_, kZQ, 3, s_);  // beta = -<z, K p> / <p, K p>
+        } else {
+          finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
+        }
         pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
       };
       if (speculate) next_direction();
@@ -622,6 +737,44 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
   ANNEAL_HIP(hipStreamSynchronize(s_));
   for (int r = 0; r < nrhs; ++r) iters_out[r] = its[r];
   return status;
+}
+
+// z1 = gamma P N(R r) with gamma = <r, z1> / <z1, K z1> (the energy-optimal
+// length of the network step), t = r - K z1.  Reads r, writes z1 and t.
+void FemContext::scaled_network_step(const float* r, float* z1, float* t, int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  learned_stage(r, z1, nrhs);
+  ApplyArgs<float> a = args<float>(0);
+  a.u = z1;
+  a.y = pt_;
+  a.dot_part = part_;
+  launch_apply(a, kApply, nrhs, s_);  // pt = K z1, partials of <z1, K z1>
+  const int nblk = apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile);
+  finalize(part_, nblk, nrhs, net_nrm_ + 64, s_);
+  dot_partial(r, z1, n, ld, nrhs, part_, s_);
+  finalize(part_, kDotBlocks, nrhs, net_nrm_ + 128, s_);
+  net_gamma(net_nrm_ + 128, net_nrm_ + 64, nrhs, net_nrm_ + 192, s_);
+  net_scaled_resid(r, pt_, z1, t, net_nrm_ + 192, ld, nrhs, s_);  // t = r - gamma K z1, z1 *= gamma
+}
+
+void FemContext::precondition(int nrhs) {
+  if (!net_ || net_mode_ != 2) {
+    vcycle_pcg(nrhs);
+    return;
+  }
+  // multiplicative two-stage preconditioner: z = z1 + V(r - K z1)
+  const int64_t nv = (int64_t)nrhs * 3 * lv_[0].g.ldn();
+  scaled_network_step(pr_, pz1_, pt_, nrhs);
+  vcycle_graph(pt_, pz_, nrhs);
+  add_f32(pz_, pz1_, nv, s_);
+}
+
+void FemContext::learned_warm_start(int nrhs) {
+  // x = z1, r = r - K z1 before the (plain) inner PCG starts
+  const size_t bytes = (size_t)nrhs * 3 * lv_[0].g.ldn() * sizeof(float);
+  scaled_network_step(pr_, pz1_, pr_, nrhs);
+  ANNEAL_HIP(hipMemcpyAsync(px_, pz1_, bytes, hipMemcpyDeviceToDevice, s_));
 }
 
 }  // namespace anneal

@@ -5,10 +5,14 @@
 // PCG preconditioned by a rediscretised geometric V-cycle, with FP32 vectors,
 // FP64 dot products and FP64 true-residual refinement to the requested rtol.
 #pragma once
+#include <map>
+#include <tuple>
 #include <vector>
 #include "stencil.hpp"
 
 namespace anneal {
+
+class UNet;
 
 // 24x24 hex8 stiffness for E = 1, cube of side h, 2x2x2 Gauss quadrature.
 // Corner q = a + 2b + 4c for the corner at (a, b, c) in {0,1}^3, dof 3q + comp.
@@ -68,7 +72,29 @@ class FemContext {
   // captured once per rhs count (pointers and smoother coefficients are fixed;
   // Ee, dinv and the coarse inverse are read from fixed buffers).
   void vcycle_pcg(int nrhs);
+  // Same on other buffers (x = M b), one graph per (nrhs, b, x).
+  void vcycle_graph(const float* b, float* x, int nrhs);
   bool use_graphs = true;
+
+  // Learned stage (a U-Net N on the node grid of multigrid level `level`,
+  // z1 = gamma P N(R r) with the energy-optimal gamma = <r, z1> / <z1, K z1>):
+  //   mode 0: off (plain multigrid PCG);
+  //   mode 1: warm start: each inner solve starts from x = z1, r - K z1;
+  //   mode 2: every iteration z = z1 + V(r - K z1) (two-stage, multiplicative),
+  //           with flexible CG (beta = -<z, K p> / <p, K p>) because the stage
+  //           is nonlinear in r.
+  // gamma makes the network step a line search, so it never raises the energy
+  // error; the FP64 true-residual refinement decides convergence as before.
+  // e_ref scales the stiffness feature (cbrt(Ee / e_ref)) and the output (1 / e_ref).
+  void set_learned(UNet* net, int level, double e_ref, int mode);
+  int learned_mode() const { return net_ ? net_mode_ : 0; }
+  int learned_level() const { return net_level_; }
+  // z = P^l N(R^l r) on [nrhs][3][ldn] FP32 buffers.
+  void learned_stage(const float* r, float* z, int nrhs);
+  // Training pair for right-hand sides F and solutions U (FP64, [nrhs][3][ldn]):
+  // in[b][V][8] the network input for r = F, tgt[b][V][3] the solution injected
+  // onto the network level in output units.
+  void learned_pair(const double* F, const double* U, int nrhs, int level, double e_ref, uint16_t* in, float* tgt);
   void invalidate_graphs() { drop_graphs(); }
   bool fuse_first_step = false;  // form x1 = omega D^-1 b inside the first Chebyshev kernel
   int tile() const { return lv_[0].tile; }
@@ -120,7 +146,18 @@ class FemContext {
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
   hipEvent_t ev_ = nullptr;                 // marks the residual-norm copy of a PCG step
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
+  std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
   void drop_graphs();
+  void precondition(int nrhs);  // pz_ = M pr_
+  void scaled_network_step(const float* r, float* z1, float* t, int nrhs);
+  void learned_warm_start(int nrhs);
+  UNet* net_ = nullptr;
+  int net_level_ = 1, net_mode_ = 0;
+  double net_eref_ = 1.0;
+  uint16_t* net_in_ = nullptr;
+  float* net_out_ = nullptr;
+  double* net_nrm_ = nullptr;
+  float *pz1_ = nullptr, *pt_ = nullptr;
 };
 
 }  // namespace anneal

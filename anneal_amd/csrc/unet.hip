@@ -21,6 +21,29 @@
 #include <cstring>
 
 namespace anneal {
+
+struct ConvK {
+  const uint16_t* x;
+  int xpitch, cin, taps;
+  const uint16_t* w;  // [n][kp]
+  int kp, nchunk;
+  const float* bias;  // [n] or null
+  uint16_t* y;        // bf16 output view or null
+  int ypitch;
+  float* yf;  // FP32 output [v][cf] (channels < cf) or null
+  int cf;
+  const uint16_t* mask;  // zero the output where mask <= 0 (or null)
+  int mpitch;
+  int relu, accum;
+  int nx, ny, nz;
+  int64_t nvox;
+  // split-K: blockIdx.z covers chunks [z * kper, (z + 1) * kper); with
+  // ksplit > 1 the raw sums go to part[z][v][n] and k_conv_finish applies the
+  // epilogue (fixed summation order over z)
+  int ksplit, kper, nout;
+  float* part;
+};
+
 namespace {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -57,22 +80,6 @@ __device__ __forceinline__ void tap_offsets(int tap, int taps, int& dx, int& dy,
   }
 }
 
-struct ConvK {
-  const uint16_t* x;
-  int xpitch, cin, taps;
-  const uint16_t* w;  // [n][kp]
-  int kp, nchunk;
-  const float* bias;  // [n] or null
-  uint16_t* y;        // bf16 output view or null
-  int ypitch;
-  float* yf;  // FP32 output [v][cf] (channels < cf) or null
-  int cf;
-  const uint16_t* mask;  // zero the output where mask <= 0 (or null)
-  int mpitch;
-  int relu, accum;
-  int nx, ny, nz;
-  int64_t nvox;
-};
 
 template <int NB, int MB>
 __global__ void __launch_bounds__(256) k_conv(ConvK a) {
@@ -101,14 +108,15 @@ __global__ void __launch_bounds__(256) k_conv(ConvK a) {
     for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
   // this lane group's k = 8 g + 32 c  ->  (tap, c0), advanced by 32 per chunk
-  int tap = (8 * g) / a.cin, c0 = (8 * g) % a.cin;
+  const int cbeg = blockIdx.z * a.kper, cend = min(a.nchunk, cbeg + a.kper);
+  int tap = (32 * cbeg + 8 * g) / a.cin, c0 = (32 * cbeg + 8 * g) % a.cin;
   int dx, dy, dz;
   tap_offsets(tap, a.taps, dx, dy, dz);
-  for (int c = 0; c < a.nchunk; ++c) {
+  // fragments of chunk c are loaded one chunk ahead of their MFMAs
+  auto load = [&](int c, bf16x8_t (&A)[MB], bf16x8_t (&Bv)[NB]) {
     const int k0 = 32 * c + 8 * g;
     const bool kv = tap < a.taps;
     const int64_t off = dx + (int64_t)dy * a.nx + (int64_t)dz * nxy;
-    bf16x8_t A[MB], Bv[NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
       const bool ok = kv && vin[mb] && unsigned(ci[mb] + dx) < unsigned(a.nx) && unsigned(cj[mb] + dy) < unsigned(a.ny) &&
@@ -120,10 +128,6 @@ __global__ void __launch_bounds__(256) k_conv(ConvK a) {
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
       Bv[nb] = __builtin_bit_cast(bf16x8_t, ld16(a.w + (int64_t)(n0 + 16 * nb + r) * a.kp + k0));
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mb], Bv[nb], acc[mb][nb], 0, 0, 0);
     c0 += 32;
     if (c0 >= a.cin) {
       while (c0 >= a.cin) {
@@ -132,6 +136,31 @@ __global__ void __launch_bounds__(256) k_conv(ConvK a) {
       }
       tap_offsets(tap, a.taps, dx, dy, dz);
     }
+  };
+  bf16x8_t A0[MB], B0[NB], A1[MB], B1[NB];
+  if (cbeg < cend) load(cbeg, A0, B0);
+  for (int c = cbeg; c < cend; ++c) {
+    if (c + 1 < cend) load(c + 1, A1, B1);
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0[mb], B0[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) A0[mb] = A1[mb];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) B0[nb] = B1[nb];
+  }
+  if (a.ksplit > 1) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t v = vb + 16 * mb + 4 * g + i;
+          if (v < a.nvox) a.part[((int64_t)blockIdx.z * a.nvox + v) * a.nout + n0 + 16 * nb + r] = acc[mb][nb][i];
+        }
+    return;
   }
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
@@ -150,6 +179,21 @@ __global__ void __launch_bounds__(256) k_conv(ConvK a) {
         if (a.y) a.y[v * a.ypitch + co] = f2bf(val);
         if (a.yf && co < a.cf) a.yf[v * a.cf + co] = val;
       }
+}
+
+__global__ void __launch_bounds__(256) k_conv_finish(ConvK a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nvox * a.nout) return;
+  const int64_t v = t / a.nout;
+  const int co = int(t - v * a.nout);
+  float val = 0.f;
+  for (int z = 0; z < a.ksplit; ++z) val += a.part[((int64_t)z * a.nvox + v) * a.nout + co];
+  if (a.bias) val += a.bias[co];
+  if (a.accum) val += bf2f(a.y[v * a.ypitch + co]);
+  if (a.relu) val = fmaxf(val, 0.f);
+  if (a.mask && !(bf2f(a.mask[v * a.mpitch + co]) > 0.f)) val = 0.f;
+  if (a.y) a.y[v * a.ypitch + co] = f2bf(val);
+  if (a.yf && co < a.cf) a.yf[v * a.cf + co] = val;
 }
 
 struct WgradK {
@@ -590,6 +634,7 @@ UNet::UNet(int cin, int cout, const std::vector<int>& widths, hipStream_t s)
 
 UNet::~UNet() {
   free_plan();
+  if (kpart_) (void)hipFree(kpart_);
   for (void* p : owned_) (void)hipFree(p);
 }
 
@@ -779,17 +824,44 @@ static void pick_nb(int n16, int& NB, int& MB) {
 template <int NB, int MB>
 static void launch_conv(const ConvK& a, int n16, hipStream_t s) {
   const int64_t per_block = 64 * MB;
-  dim3 grid(unsigned((a.nvox + per_block - 1) / per_block), unsigned(n16 / NB));
+  dim3 grid(unsigned((a.nvox + per_block - 1) / per_block), unsigned(n16 / NB), unsigned(a.ksplit));
   hipLaunchKernelGGL((k_conv<NB, MB>), grid, dim3(256), 0, s, a);
 }
 
-static void conv_dispatch(const ConvK& a, int n16, hipStream_t s) {
+// Grids smaller than the machine split the K loop over blockIdx.z (at least 4
+// chunks each) so enough waves overlap their load latency.
+void UNet::conv_dispatch(ConvK a, int n16) {
   int NB, MB;
   pick_nb(n16, NB, MB);
-  if (NB == 4) launch_conv<4, 2>(a, n16, s);
-  else if (NB == 3) launch_conv<3, 2>(a, n16, s);
-  else if (NB == 2) launch_conv<2, 4>(a, n16, s);
-  else launch_conv<1, 4>(a, n16, s);
+  const int64_t blocks = ((a.nvox + 64 * MB - 1) / (64 * MB)) * (n16 / NB);
+  a.ksplit = 1;
+  a.kper = a.nchunk;
+  a.nout = n16 * 16;
+  if (blocks < 512 && a.nchunk >= 8) {
+    int ks = (int)std::min<int64_t>(a.nchunk / 4, (1024 + blocks - 1) / blocks);
+    if (ks > 1) {
+      a.kper = (a.nchunk + ks - 1) / ks;
+      a.ksplit = (a.nchunk + a.kper - 1) / a.kper;
+    }
+  }
+  if (a.ksplit > 1) {
+    const size_t need = (size_t)a.ksplit * a.nvox * a.nout;
+    if (need > kpart_elems_) {
+      if (kpart_) ANNEAL_HIP(hipFree(kpart_));
+      kpart_ = nullptr;
+      ANNEAL_HIP(hipMalloc(&kpart_, need * sizeof(float)));
+      kpart_elems_ = need;
+    }
+    a.part = kpart_;
+  }
+  if (NB == 4) launch_conv<4, 2>(a, n16, s_);
+  else if (NB == 3) launch_conv<3, 2>(a, n16, s_);
+  else if (NB == 2) launch_conv<2, 4>(a, n16, s_);
+  else launch_conv<1, 4>(a, n16, s_);
+  if (a.ksplit > 1) {
+    const int64_t n = a.nvox * a.nout;
+    hipLaunchKernelGGL(k_conv_finish, dim3(blocks_for(n)), dim3(256), 0, s_, a);
+  }
   ANNEAL_HIP(hipGetLastError());
 }
 
@@ -813,7 +885,7 @@ void UNet::conv_fwd(ConvLayer& Ly, const View& x, const NetDims& d, const View& 
   a.ny = d.ny;
   a.nz = d.nz;
   a.nvox = d.vox();
-  conv_dispatch(a, Ly.cout_pad / 16, s_);
+  conv_dispatch(a, Ly.cout_pad / 16);
 }
 
 void UNet::conv_dgrad(ConvLayer& Ly, const View& dy, const NetDims& d, const View& dx, const View* mask, bool accum) {
@@ -835,7 +907,7 @@ void UNet::conv_dgrad(ConvLayer& Ly, const View& dy, const NetDims& d, const Vie
   a.ny = d.ny;
   a.nz = d.nz;
   a.nvox = d.vox();
-  conv_dispatch(a, Ly.cin / 16, s_);
+  conv_dispatch(a, Ly.cin / 16);
 }
 
 void UNet::conv_wgrad(ConvLayer& Ly, const View& x, const View& dy, const NetDims& d) {

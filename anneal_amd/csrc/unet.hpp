@@ -46,6 +46,8 @@ struct ConvLayer {
   int64_t nparams() const { return (int64_t)cout * taps * cin + cout; }
 };
 
+struct ConvK;  // convolution launch arguments (unet.hip)
+
 class UNet {
  public:
   // cin input channels (multiple of 8), cout output channels (<= 16),
@@ -95,6 +97,9 @@ class UNet {
   void conv_dgrad(ConvLayer& L, const View& dy, const NetDims& d, const View& dx, const View* mask, bool accum);
   void conv_wgrad(ConvLayer& L, const View& x, const View& dy, const NetDims& d);
   void refresh_bf16(ConvLayer& L);
+  void conv_dispatch(ConvK a, int n16);
+  float* kpart_ = nullptr;  // split-K partial sums
+  size_t kpart_elems_ = 0;
 
   hipStream_t s_;
   int cin_, cout_;

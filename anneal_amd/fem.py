@@ -225,6 +225,36 @@ class FEM:
             out.append(tuple(x.value for x in v))
         return out
 
+    def set_learned(self, net, level=1, e_ref=1.0, mode=2):
+        """Attach a U-Net (anneal_amd.learned.UNet) as a preconditioner stage of every solve.
+
+        mode 1: warm start x = z1 (z1 = gamma P N(R r), gamma energy-optimal) before plain PCG;
+        mode 2: z = z1 + V(r - K z1) in every flexible-CG iteration;
+        mode 0 or net None: plain multigrid PCG.  The network runs on the node
+        grid of multigrid level ``level``; e_ref is the stiffness scale (E0).
+        """
+        if net is None or mode == 0:
+            _lib.call("anneal_fem_set_learned", self._ctx, None, 1, 1.0, 0)
+            self._net = None
+            return
+        _lib.call("anneal_fem_set_learned", self._ctx, net._ctx, int(level), float(e_ref), int(mode))
+        self._net = net          # the context must outlive its use here
+
+    def learned_stage(self, r, z, nrhs):
+        """z = P^l N(R^l r) on FP32 SoA device blocks [nrhs, 3, ldn]."""
+        _lib.call("anneal_fem_learned_stage", self._ctx, _lib.ptr(r), _lib.ptr(z), int(nrhs))
+
+    def learned_pair(self, F, U, nrhs, level=1, e_ref=1.0):
+        """(input bf16 [nrhs, V, 8], target FP32 [nrhs, V, 3]) for loads F and solutions U (FP64 SoA device)."""
+        torch = _torch()
+        nx, ny, nz, _ = self.level_info()[level]
+        V = (nx + 1) * (ny + 1) * (nz + 1)
+        x = torch.empty(nrhs, V, 8, dtype=torch.bfloat16, device=self.device)
+        t = torch.empty(nrhs, V, 3, dtype=torch.float32, device=self.device)
+        _lib.call("anneal_fem_learned_pair", self._ctx, _lib.ptr(F), _lib.ptr(U), int(nrhs), int(level),
+                  float(e_ref), _lib.ptr(x), _lib.ptr(t))
+        return x, t, (nx + 1, ny + 1, nz + 1)
+
     def tile(self):
         """Tile shape id in use by the FP32 apply."""
         chosen = _lib.ctypes.c_int()

@@ -137,3 +137,135 @@ class UNet:
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
         return cls(int(z["cin"]), int(z["cout"]), tuple(int(w) for w in z["widths"]), params=z["params"])
+
+
+# ---- training data from SIMP runs ------------------------------------------------
+
+TRAIN_SHAPE = (48, 24, 24)
+
+
+def training_problem(seed, shape=TRAIN_SHAPE):
+    """Seeded cantilever of the benchmark's family (the x = 0 face clamped).
+
+    A unit point load sits on the x = nx face at a node drawn from the middle
+    half of y and z, with a direction uniform on the sphere; the volume fraction
+    is drawn from U(0.2, 0.4).  Returns (grid, fill, active, frozen, fixed, F, vf).
+    """
+    from .fem import Grid
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nx, ny, nz = shape
+    g = Grid(shape, 1.0, (0, 0, 0))
+    n = np.arange(g.nnode)
+    clamp = n[n % (nx + 1) == 0]
+    fixed = (3 * clamp[:, None] + np.arange(3)).ravel()
+    j = int(rng.integers(ny // 4, 3 * ny // 4 + 1))
+    k = int(rng.integers(nz // 4, 3 * nz // 4 + 1))
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    node = nx + (nx + 1) * (j + (ny + 1) * k)
+    F = np.zeros(3 * g.nnode)
+    F[3 * node:3 * node + 3] = d
+    vf = float(rng.uniform(0.2, 0.4))
+    return g, np.ones(g.nelem), np.ones(g.nelem, bool), np.zeros(g.nelem, bool), fixed, F, vf
+
+
+def generate_pairs(n_runs, iters=50, shape=TRAIN_SHAPE, seed=0, level=1, r_mm=1.5, log=None):
+    """(X bf16 [N, V, 8], T FP32 [N, V, 3], node dims) from every solve of n_runs seeded SIMP runs.
+
+    Each SIMP iteration contributes the (density, load, displacement) triple of
+    its solve, mapped onto multigrid level ``level`` by FEM.learned_pair.
+    """
+    torch = _torch()
+    from .optimize import SimpRun
+    xs, ts, dims = [], [], None
+
+    def capture(run):
+        nonlocal dims
+        x, t, dims = run.fem.learned_pair(run.F, run.U, run.ncase, level=level, e_ref=run.E0)
+        xs.append(x)
+        ts.append(t)
+
+    for r in range(n_runs):
+        g, fill, active, frozen, fixed, F, vf = training_problem(seed + r, shape)
+        run = SimpRun(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, vf, iters=iters, r_mm=r_mm,
+                      tol=0.0, emin=1e-9, rtol=1e-6, log=lambda m: None, on_solve=capture)
+        for _ in range(iters):
+            run.step()
+        run.fem.close()
+        if log and (r + 1) % max(1, n_runs // 10) == 0:
+            log(f"pairs: {r + 1}/{n_runs} runs, {sum(x.shape[0] for x in xs)} samples")
+    torch.cuda.synchronize()
+    return torch.cat(xs), torch.cat(ts), dims
+
+
+def train(net, X, T, dims, steps, batch=32, lr=2e-3, lr_final=1e-4, seed=0, val=None, log=None, log_every=100):
+    """Adam on the per-sample relative squared error; cosine learning-rate decay.
+
+    val: optional (Xv, Tv) evaluated every log_every steps.  Returns the list of
+    (step, train loss, validation loss or None).
+    """
+    torch = _torch()
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    N = X.shape[0]
+    V = X.shape[1]
+    dout = torch.empty(batch, V, net.cout, dtype=torch.float32, device=X.device)
+    hist = []
+    for s in range(1, steps + 1):
+        idx = torch.randint(0, N, (batch,), generator=gen).to(X.device)
+        xb = X.index_select(0, idx).contiguous()
+        tb = T.index_select(0, idx).contiguous()
+        out = net.forward(xb, dims)
+        loss = net.rel_mse(out, tb, dims, dout)
+        net.zero_grad()
+        net.backward(dout)
+        c = 0.5 * (1.0 + np.cos(np.pi * (s - 1) / max(1, steps - 1)))
+        net.adam(lr_final + (lr - lr_final) * c)
+        if s % log_every == 0 or s == steps:
+            tl = float(loss.item())
+            vl = evaluate(net, *val, dims) if val is not None else None
+            hist.append((s, tl, vl))
+            if log:
+                log(f"train step {s}: loss {tl:.4f}" + (f", validation {vl:.4f}" if vl is not None else ""))
+    return hist
+
+
+def evaluate(net, X, T, dims, batch=64):
+    """Mean per-sample relative squared error over (X, T)."""
+    torch = _torch()
+    tot, n = 0.0, 0
+    for i in range(0, X.shape[0], batch):
+        xb, tb = X[i:i + batch].contiguous(), T[i:i + batch].contiguous()
+        out = net.forward(xb, dims)
+        tot += float(net.rel_mse(out, tb, dims).item()) * xb.shape[0]
+        n += xb.shape[0]
+    return tot / max(n, 1)
+
+
+def main():
+    import argparse
+    import sys
+    import time
+    ap = argparse.ArgumentParser(description="Train the learned-preconditioner U-Net on SIMP solves")
+    ap.add_argument("--runs", type=int, default=400)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--val-runs", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    t0 = time.time()
+    X, T, dims = generate_pairs(a.runs, a.iters, seed=a.seed, log=log)
+    Xv, Tv, _ = generate_pairs(a.val_runs, a.iters, seed=10_000 + a.seed)
+    log(f"data: {X.shape[0]} training / {Xv.shape[0]} validation pairs on {dims} nodes in {time.time() - t0:.1f} s")
+    net = UNet(seed=a.seed)
+    t1 = time.time()
+    hist = train(net, X, T, dims, a.steps, a.batch, a.lr, seed=a.seed, val=(Xv, Tv), log=log)
+    log(f"trained {a.steps} steps in {time.time() - t1:.1f} s; final validation {hist[-1][2]:.4f}")
+    net.save(a.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,7 +132,8 @@ class SimpRun:
 
     def __init__(self, grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
                  iters=80, r_mm=None, p_final=3.0, move=0.2, combine="sum", log=print, tol=0.01, x0=None,
-                 beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None):
+                 beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None, learned=None,
+                 learned_mode=2, learned_level=1, on_solve=None):
         torch = _torch()
         self.t_start = time.time()
         self.torch = torch
@@ -149,6 +150,9 @@ class SimpRun:
         if fem is None:
             fem = FEM(grid, active, fixed, nu, log=log, nrhs_max=max(1, len(self.cases)))
         self.fem = fem
+        if learned is not None:
+            fem.set_learned(learned, level=learned_level, e_ref=float(E0), mode=learned_mode)
+        self.on_solve = on_solve          # called as on_solve(run) after every solve (e.g. to collect training pairs)
         act = fem.act
         dev = fem.device
         phi_h = fill[act]
@@ -248,6 +252,8 @@ class SimpRun:
         self.have_u = True
         if rebuild:
             self.base_iters = fem.cg_iters[-1]
+        if self.on_solve is not None:
+            self.on_solve(self)
         _lib.call("anneal_dot_f64", _lib.ptr(self.F), _lib.ptr(self.U), self.grid.nnode, fem.ldn, self.ncase,
                   _lib.ptr(self.cs_dev), _lib.ptr(self.work), stream)
         cs = self.cs_dev.cpu().numpy()
@@ -323,17 +329,21 @@ class SimpRun:
 
 def simp(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
          iters=80, r_mm=None, p_final=3.0, move=0.2, combine="sum", log=print, tol=0.01, x0=None,
-         beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None):
+         beta_max=BETA_MAX, *, emin=None, rtol=1e-6, maxiter=2000, fem=None, timers=None, learned=None,
+         learned_mode=2, learned_level=1):
     """SIMP loop of optimize.py:50-180 on the GPU; returns the reference's result dict.
 
     Extra keyword-only arguments (defaults reproduce the reference): ``emin``
     (module ``EMIN`` when None, optimize.py:18), ``rtol`` of the inner solves
     (optimize.py:122 uses 1e-6), ``maxiter`` (fem.py:206), an existing ``fem``
-    context to reuse, and ``timers`` (a dict that receives per-phase seconds).
+    context to reuse, ``timers`` (a dict that receives per-phase seconds) and
+    ``learned`` (an anneal_amd.learned.UNet used as a flexible-PCG stage with
+    ``learned_mode`` / ``learned_level``, see FEM.set_learned).
     """
     return SimpRun(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf, iters=iters, r_mm=r_mm,
                    p_final=p_final, move=move, combine=combine, log=log, tol=tol, x0=x0, beta_max=beta_max,
-                   emin=emin, rtol=rtol, maxiter=maxiter, fem=fem, timers=timers).run()
+                   emin=emin, rtol=rtol, maxiter=maxiter, fem=fem, timers=timers, learned=learned,
+                   learned_mode=learned_mode, learned_level=learned_level).run()
 
 
 def benchmark_cantilever(log=lambda *a: None):

@@ -196,6 +196,23 @@ int anneal_unet_restrict(const uint16_t* x_dev, int B, int fx, int fy, int fz, i
 int anneal_unet_prolong(const uint16_t* x_dev, int B, int cx, int cy, int cz, int c, uint16_t* y_dev, int fx, int fy,
                         int fz, float s, int accumulate, void* stream);
 
+/* Attach a U-Net (anneal_unet_create with 8 input and 3 output channels) as a
+ * stage of the solve (no counterpart in the reference).  With
+ * z1 = gamma P N(R r), gamma = <r, z1> / <z1, K z1> (energy-optimal):
+ * mode 0 off; 1 warm start (each inner PCG starts from x = z1); 2 two-stage
+ * preconditioner z = z1 + V(r - K z1) in every iteration, with flexible CG.
+ * The network runs on the node grid of multigrid level `level` (>= 1).  The network context must outlive its
+ * use by the FEM context.  e_ref scales the stiffness input and the output. */
+int anneal_fem_set_learned(void* fem, void* unet, int level, double e_ref, int mode);
+/* z_dev = P^level N(R^level r_dev) on FP32 [nrhs][3][ldn] buffers (diagnostics). */
+int anneal_fem_learned_stage(void* fem, const float* r_dev, float* z_dev, int nrhs);
+/* Training pair for FP64 loads F_dev and solutions U_dev ([nrhs][3][ldn]) at
+ * the current Ee: in_dev[nrhs][V][8] bf16 network input for r = F and
+ * tgt_dev[nrhs][V][3] FP32 the solution injected onto the level, in output
+ * units (V = nodes of that level). */
+int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev, int nrhs, int level, double e_ref,
+                            uint16_t* in_dev, float* tgt_dev);
+
 #ifdef __cplusplus
 }
 #endif
