@@ -128,6 +128,45 @@ def apply_roofline(fem, reps=20, nbuf=8):
             "dof_applies_per_s": 3 * g.nnode / (ms * 1e-3)}
 
 
+def learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, plain_ms, runs=200, train_steps=2000,
+                mode=1, log=None):
+    """Train the U-Net on the GPU (pairs from seeded 48x24x24 cantilever SIMP runs) and time the same SIMP
+    schedule with the network as warm start (mode 1) or two-stage preconditioner (mode 2)."""
+    import torch
+    from anneal_amd import learned as Lm
+    from anneal_amd.optimize import SimpRun
+    t0 = time.perf_counter()
+    X, T, dims = Lm.generate_pairs(runs, 50, seed=0)
+    Xv, Tv, _ = Lm.generate_pairs(10, 50, seed=10_000)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = Lm.UNet(seed=0)
+    t1 = time.perf_counter()
+    Lm.train(net, X, T, dims, train_steps, batch=32, lr=2e-3, seed=0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    val = Lm.evaluate(net, Xv, Tv, dims)
+    del X, T
+    if log:
+        log(f"bench: learned: {Xv.shape[0]} validation pairs, relative squared error {val:.3f}; data {t_data:.1f} s, "
+            f"training {t_train:.1f} s")
+    run = SimpRun(g, fill, active, frozen, fixed, loads, weights, 1.0, 0.3, 0.3, iters=iters, r_mm=3.0, tol=0.0,
+                  emin=1e-9, rtol=1e-6, log=lambda m: None, learned=net, learned_mode=mode)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        run.step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t2) / steps
+    cg = run.fem.cg_iters
+    return {"value": round(1e3 / ms, 4), "unit": "SIMP iter/s", "ms_per_step": round(ms, 3),
+            "speedup_vs_plain": round(plain_ms / ms, 4), "mode": "warm start" if mode == 1 else "two-stage FPCG",
+            "pcg_iters_total": int(sum(cg)), "final_compliance": run.hist[-1]["compliance"],
+            "training": {"pairs": int(runs * 50), "grid": "48x24x24 cantilevers (network on 25x13x13 nodes)",
+                         "steps": train_steps, "batch": 32, "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                         "validation_rel_sq_error": round(val, 4)}}
+
+
 def cpu_baseline(budget_s=20.0):
     """Oracle SIMP iterations at 48x24x24 on one host core, scaled by element count to 192x96x96."""
     import numpy as np
@@ -156,6 +195,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed SIMP iterations (default: the whole schedule)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--workload", choices=["cantilever", "large"], default="cantilever",
                     help="cantilever: 192x96x96, 100 iterations (the headline); large: 384x192x192 with "
@@ -208,6 +249,12 @@ def main():
           f"{h[-1]['compliance']}", file=sys.stderr, flush=True)
     roof = apply_roofline(run.fem, reps=args.apply_reps)
     print(f"bench: apply {roof['us_per_launch']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
+    lrn = None
+    if not args.no_learned and args.workload == "cantilever":
+        lrn = learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, ms, mode=args.learned_mode,
+                          log=log)
+        print(f"bench: learned {lrn['ms_per_step']} ms/iter ({lrn['speedup_vs_plain']}x plain), PCG iterations "
+              f"{lrn['pcg_iters_total']}", file=sys.stderr, flush=True)
     cpu = None if (args.no_cpu or args.workload != "cantilever") else cpu_baseline()
     sx, sy, sz = shape
     out = {
@@ -219,7 +266,8 @@ def main():
         "config": {"workload": workload, "solver": "plain MG-PCG", "iterations_timed": f"1-{steps}",
                    "setup_s": round(setup_s, 3), "pcg_iters_total": int(sum(cg)), "pcg_iters_max": int(max(cg)),
                    "oc_volume_evals": int(run.oc_evals), "oc_trials": int(sum(x["ocSteps"] for x in h)),
-                   "final_compliance": h[-1]["compliance"], "final_volume_fraction": h[-1]["volumeFraction"]},
+                   "final_compliance": h[-1]["compliance"], "final_volume_fraction": h[-1]["volumeFraction"],
+                   "learned": lrn},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -119,3 +119,67 @@ def test_flexible_pcg_with_network_stage_converges(mode):
         res = f - K @ b_
         assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(f) * 1.0001
     fem.close()
+
+
+def test_trained_network_gain():
+    """What training buys: energy error of the network's warm start and PCG iterations, trained vs not.
+
+    Trained on pairs from seeded 48x24x24 cantilever SIMP runs, evaluated on a
+    held-out run's designs (cold solves from x = 0 to rtol 1e-8).  The warm start
+    is z1 = gamma P N(R F) with the energy-optimal gamma, so its relative energy
+    error is sqrt(1 - <F, z1>^2 / (<z1, K z1> <F, u>)); a network that learned
+    nothing scores about 1.
+    """
+    from anneal_amd import learned as Lm
+    from anneal_amd.optimize import SimpRun
+    X, T, dims = Lm.generate_pairs(24, 30, seed=0)
+    Xv, Tv, _ = Lm.generate_pairs(2, 30, seed=5000)
+    trained = Lm.UNet(seed=0)
+    untrained = Lm.UNet(seed=0)
+    Lm.train(trained, X, T, dims, 600, batch=32, lr=2e-3, seed=0)
+    lv_t, lv_u = Lm.evaluate(trained, Xv, Tv, dims), Lm.evaluate(untrained, Xv, Tv, dims)
+    print(f"validation relative squared error: trained {lv_t:.3f}, untrained {lv_u:.3f}", flush=True)
+    assert lv_t < 0.8 * lv_u
+
+    # held-out designs: snapshots of a SIMP run with another seed
+    g, fill, active, frozen, fixed, F, vf = Lm.training_problem(7777)
+    designs = []
+    run = SimpRun(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, vf, iters=30, r_mm=1.5, tol=0.0,
+                  emin=1e-9, rtol=1e-6, log=lambda m: None)
+    for it in range(30):
+        run.step()
+        if it in (0, 9, 19, 29):
+            designs.append(run.Ee.clone())
+    fem = run.fem
+    ld = fem.ldn
+    res = {}
+    for name, net in (("plain", None), ("untrained", untrained), ("trained", trained)):
+        iters, ratios = [], []
+        for Ee in designs:
+            fem.assemble_device(Ee)
+            fem.rebuild()
+            fem.set_learned(None)
+            U = torch.zeros_like(run.F)
+            its = fem.solve_device(run.F, U, 1, 1e-10, 2000, use_x0=False)
+            if net is not None:
+                fem.set_learned(net, level=1, e_ref=1.0, mode=1)
+                r32 = run.F.float().contiguous()
+                z = torch.zeros_like(r32)
+                fem.learned_stage(r32, z, 1)
+                z64 = z.double()
+                Kz = torch.empty_like(z64)
+                from anneal_amd import _lib
+                _lib.call("anneal_fem_apply_f64", fem._ctx, _lib.ptr(z64), _lib.ptr(Kz), 1)
+                fz, zkz, fu = float((run.F * z64).sum()), float((z64 * Kz).sum()), float((run.F * U).sum())
+                ratios.append(np.sqrt(max(0.0, 1.0 - fz * fz / (zkz * fu))))
+                U2 = torch.zeros_like(run.F)
+                its = fem.solve_device(run.F, U2, 1, 1e-8, 2000, use_x0=False)
+                assert float((U2 - U).norm() / U.norm()) < 1e-6
+            else:
+                U2 = torch.zeros_like(run.F)
+                its = fem.solve_device(run.F, U2, 1, 1e-8, 2000, use_x0=False)
+            iters.append(its[0])
+        res[name] = (iters, ratios)
+        print(f"{name}: cold-solve PCG iterations {iters}, warm-start energy error {np.round(ratios, 3)}", flush=True)
+    fem.set_learned(None)
+    assert np.mean(res["trained"][1]) < np.mean(res["untrained"][1])
