@@ -66,6 +66,10 @@ SIGNATURES = {
     "anneal_fem_set_learned": [c_void_p, c_void_p, c_int, c_double, c_int],
     "anneal_fem_learned_stage": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_learned_pair": [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
+    "anneal_batch_create": [ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int],
+    "anneal_batch_destroy": [c_void_p],
+    "anneal_batch_solve": [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_intp, c_double, c_int, c_int,
+                           c_intp, c_intp],
     "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p],
 }

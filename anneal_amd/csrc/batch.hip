@@ -1,0 +1,276 @@
+// Lock-step mixed-precision PCG over several independent FEM contexts (grids of
+// different sizes, each with its own block of right-hand sides).
+//
+// One inner PCG iteration of every active problem is recorded as ONE hipGraph:
+// the problems' kernels sit on parallel branches (one capture stream each,
+// forked from and joined back to the launch stream), and the host waits once
+// per iteration for the whole batch instead of once per problem.  The graph is
+// recorded once per refinement pass for the problems that need inner
+// iterations; a problem (or right-hand side) that meets its tolerance earlier
+// keeps its device "done" flags, so its branch leaves x, r and p untouched
+// until the pass ends.  The FP64 true-residual refinement and its decisions are
+// the same as in FemContext::solve.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace anneal {
+
+BatchSolver::BatchSolver(const std::vector<FemContext*>& fems) : f_(fems) {
+  ANNEAL_CHECK(!f_.empty() && f_.size() <= 64, "a batch holds 1..64 FEM contexts");
+  s_ = f_[0]->s_;
+  for (FemContext* f : f_) {
+    ANNEAL_CHECK(f != nullptr, "null FEM context in the batch");
+    ANNEAL_CHECK(f->s_ == s_, "the contexts of a batch must share one stream");
+  }
+  ANNEAL_HIP(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+  br_.resize(f_.size());
+  join_.resize(f_.size());
+  for (size_t i = 0; i < f_.size(); ++i) {
+    ANNEAL_HIP(hipStreamCreateWithFlags(&br_[i], hipStreamNonBlocking));
+    ANNEAL_HIP(hipEventCreateWithFlags(&join_[i], hipEventDisableTiming));
+  }
+  ANNEAL_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+  ANNEAL_HIP(hipEventCreateWithFlags(&done_ev_, hipEventDisableTiming));
+}
+
+BatchSolver::~BatchSolver() {
+  (void)hipStreamSynchronize(s_);
+  drop_graph();
+  for (auto s : br_) (void)hipStreamDestroy(s);
+  for (auto e : join_) (void)hipEventDestroy(e);
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (done_ev_) (void)hipEventDestroy(done_ev_);
+  if (cap_) (void)hipStreamDestroy(cap_);
+}
+
+void BatchSolver::drop_graph() {
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  graph_ = nullptr;
+  graph_key_.clear();
+}
+
+// One inner iteration of problem p (the body of FemContext::solve's inner loop
+// without the host-side speculation), issued on p's current stream.
+void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
+  const GridDims& g0 = f.lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, f.lv_[0].tile);
+  ApplyArgs<float> af = f.args<float>(0);
+  af.u = f.pp_;
+  af.y = f.pq_;
+  af.dot_part = f.part_;
+  launch_apply(af, kApply, nrhs, f.s_);
+  finalize_st(f.part_, nblk_apply, nrhs, f.st_, kPQ, 1, f.s_);
+  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_);
+  ANNEAL_HIP(hipMemcpyAsync(f.host_red_ + 64, f.red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, f.s_));
+  f.vcycle(0, f.pr_, f.pz_, nrhs);
+  dot_partial(f.pr_, f.pz_, n, ld, nrhs, f.part_, f.s_);
+  finalize_st(f.part_, kDotBlocks, nrhs, f.st_, kRZnew, 2, f.s_);
+  pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nrhs, 0, f.s_);
+}
+
+void BatchSolver::record(const std::vector<int>& active, const std::vector<int>& nrhs) {
+  std::vector<int> key;
+  for (int p : active) {
+    key.push_back(p);
+    key.push_back(nrhs[p]);
+  }
+  if (graph_ && key == graph_key_) return;
+  drop_graph();
+  std::vector<hipStream_t> saved(f_.size());
+  for (size_t i = 0; i < f_.size(); ++i) saved[i] = f_[i]->s_;
+  hipGraph_t g = nullptr;
+  try {
+    ANNEAL_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+    ANNEAL_HIP(hipEventRecord(fork_, cap_));
+    for (int p : active) {
+      ANNEAL_HIP(hipStreamWaitEvent(br_[p], fork_, 0));
+      f_[p]->s_ = br_[p];
+      issue_iteration(*f_[p], nrhs[p]);
+      ANNEAL_HIP(hipEventRecord(join_[p], br_[p]));
+      ANNEAL_HIP(hipStreamWaitEvent(cap_, join_[p], 0));
+    }
+    ANNEAL_HIP(hipStreamEndCapture(cap_, &g));
+  } catch (...) {
+    for (size_t i = 0; i < f_.size(); ++i) f_[i]->s_ = saved[i];
+    hipGraph_t dummy = nullptr;
+    (void)hipStreamEndCapture(cap_, &dummy);
+    if (dummy) (void)hipGraphDestroy(dummy);
+    throw;
+  }
+  for (size_t i = 0; i < f_.size(); ++i) f_[i]->s_ = saved[i];
+  ANNEAL_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  graph_key_ = key;
+}
+
+void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<double*>& U,
+                        const std::vector<int>& nrhs, double rtol, int maxiter, bool use_x0, int* status,
+                        int* iters /* [nprob][64] */) {
+  const int np = (int)f_.size();
+  ANNEAL_CHECK((int)F.size() == np && (int)U.size() == np && (int)nrhs.size() == np, "one F, U, nrhs per problem");
+  struct St {
+    std::vector<double> fnorm, rnorm, tol2, sc, hst;
+    std::vector<int> its;
+    std::vector<char> done, idone;
+    int status = 1, total = 0;
+    bool finished = false, bad = false, inner = false;
+  };
+  std::vector<St> S(np);
+  for (int p = 0; p < np; ++p) {
+    FemContext& f = *f_[p];
+    const int nr = nrhs[p];
+    ANNEAL_CHECK(nr >= 1 && nr <= f.nrhs_max_ && nr <= 64, "nrhs out of range for a batch member");
+    ANNEAL_CHECK(f.learned_mode() == 0, "batched solves use the plain multigrid preconditioner");
+    if (!f.coarse_valid_ || (f.coarse_stale_ && f.rebuild_each_solve)) f.rebuild_coarse();
+    const GridDims& g0 = f.lv_[0].g;
+    if (!use_x0) ANNEAL_HIP(hipMemsetAsync(U[p], 0, (size_t)nr * 3 * g0.ldn() * sizeof(double), s_));
+    dot(F[p], F[p], g0.nnode(), g0.ldn(), nr, f.part_, f.red_, s_);
+    ANNEAL_HIP(hipMemcpyAsync(f.host_red_, f.red_, nr * sizeof(double), hipMemcpyDeviceToHost, s_));
+    St& st = S[p];
+    st.fnorm.assign(nr, 0.0);
+    st.rnorm.assign(nr, 0.0);
+    st.tol2.assign(nr, 0.0);
+    st.sc.assign(nr, 0.0);
+    st.hst.assign((size_t)nr * kSt, 0.0);
+    st.its.assign(nr, 0);
+    st.done.assign(nr, 0);
+    st.idone.assign(nr, 0);
+  }
+  ANNEAL_HIP(hipStreamSynchronize(s_));
+  for (int p = 0; p < np; ++p)
+    for (int r = 0; r < nrhs[p]; ++r) S[p].fnorm[r] = std::sqrt(f_[p]->host_red_[r]);
+
+  for (int outer = 0; outer < 50; ++outer) {
+    // FP64 true residuals of every unfinished problem, one wait
+    for (int p = 0; p < np; ++p) {
+      if (S[p].finished) continue;
+      FemContext& f = *f_[p];
+      const GridDims& g0 = f.lv_[0].g;
+      ApplyArgs<double> a = f.args<double>(0);
+      a.u = U[p];
+      a.b = F[p];
+      a.y = f.r64_;
+      launch_apply(a, kResid, nrhs[p], s_);
+      dot(f.r64_, f.r64_, g0.nnode(), g0.ldn(), nrhs[p], f.part_, f.red_, s_);
+      ANNEAL_HIP(hipMemcpyAsync(f.host_red_, f.red_, nrhs[p] * sizeof(double), hipMemcpyDeviceToHost, s_));
+    }
+    ANNEAL_HIP(hipStreamSynchronize(s_));
+    std::vector<int> active;
+    for (int p = 0; p < np; ++p) {
+      St& st = S[p];
+      st.inner = false;
+      if (st.finished) continue;
+      FemContext& f = *f_[p];
+      const int nr = nrhs[p];
+      bool all = true;
+      for (int r = 0; r < nr; ++r) {
+        st.rnorm[r] = std::sqrt(f.host_red_[r]);
+        if (!std::isfinite(st.rnorm[r])) st.bad = true;
+        st.done[r] = (st.rnorm[r] <= rtol * st.fnorm[r]) ? 1 : 0;
+        all = all && st.done[r];
+      }
+      if (st.bad) {
+        st.status = 2;
+        st.finished = true;
+        continue;
+      }
+      if (all) {
+        st.status = 0;
+        st.finished = true;
+        continue;
+      }
+      if (st.total >= maxiter) {
+        st.status = 1;
+        st.finished = true;
+        continue;
+      }
+      // inner FP32 PCG set-up (as in FemContext::solve)
+      const GridDims& g0 = f.lv_[0].g;
+      const int64_t n = g0.nnode(), ld = g0.ldn();
+      for (int r = 0; r < nr; ++r) {
+        st.sc[r] = st.done[r] ? 0.0 : 1.0 / st.rnorm[r];
+        const double want = st.done[r] ? 1.0 : 0.5 * rtol * st.fnorm[r] / st.rnorm[r];
+        const double t = std::max(want, f.inner_tol_floor);
+        st.tol2[r] = t * t;
+        for (int k = 0; k < kSt; ++k) st.hst[r * kSt + k] = 0.0;
+        st.hst[r * kSt + kDone] = st.done[r] ? 1.0 : 0.0;
+        st.idone[r] = st.done[r];
+      }
+      ANNEAL_HIP(hipMemcpyAsync(f.scale_, st.sc.data(), nr * sizeof(double), hipMemcpyHostToDevice, s_));
+      ANNEAL_HIP(hipMemcpyAsync(f.st_, st.hst.data(), st.hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+      to_f32_scaled(f.r64_, f.pr_, f.scale_, n, ld, nr, s_);
+      ANNEAL_HIP(hipMemsetAsync(f.px_, 0, (size_t)nr * 3 * ld * sizeof(float), s_));
+      f.vcycle_pcg(nr);
+      dot_partial(f.pr_, f.pz_, n, ld, nr, f.part_, s_);
+      finalize_st(f.part_, kDotBlocks, nr, f.st_, kRZ, 0, s_);
+      pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nr, 1, s_);
+      st.inner = true;
+      active.push_back(p);
+    }
+    if (active.empty()) break;
+    record(active, nrhs);
+    int inner_max = 0;
+    for (int p : active) inner_max = std::max(inner_max, f_[p]->inner_max);
+    for (int it = 0; it < inner_max; ++it) {
+      bool any = false;
+      for (int p : active) any = any || (S[p].inner && S[p].total < maxiter);
+      if (!any) break;
+      ANNEAL_HIP(hipGraphLaunch(graph_, s_));
+      ANNEAL_HIP(hipEventRecord(done_ev_, s_));
+      ANNEAL_HIP(hipEventSynchronize(done_ev_));
+      for (int p : active) {
+        St& st = S[p];
+        if (!st.inner) continue;
+        FemContext& f = *f_[p];
+        ++st.total;
+        bool iall = true, changed = false;
+        for (int r = 0; r < nrhs[p]; ++r) {
+          if (st.idone[r]) continue;
+          ++st.its[r];
+          const double rr = f.host_red_[64 + r];
+          if (!std::isfinite(rr)) st.bad = true;
+          if (rr <= st.tol2[r]) {
+            st.idone[r] = 1;
+            changed = true;
+          }
+          iall = iall && st.idone[r];
+        }
+        if (st.bad || iall || st.total >= maxiter) {
+          // freeze this problem's branch: every rhs done on the device
+          for (int r = 0; r < nrhs[p]; ++r) st.hst[r * kSt + kDone] = 1.0;
+          changed = true;
+          st.inner = false;
+        }
+        if (changed) {
+          for (int r = 0; r < nrhs[p]; ++r)
+            if (st.idone[r]) st.hst[r * kSt + kDone] = 1.0;
+          ANNEAL_HIP(hipMemcpy2DAsync(f.st_ + kDone, kSt * sizeof(double), st.hst.data() + kDone, kSt * sizeof(double),
+                                      sizeof(double), nrhs[p], hipMemcpyHostToDevice, s_));
+        }
+      }
+    }
+    for (int p : active) {
+      St& st = S[p];
+      FemContext& f = *f_[p];
+      if (st.bad) {
+        st.status = 2;
+        st.finished = true;
+        continue;
+      }
+      const GridDims& g0 = f.lv_[0].g;
+      for (int r = 0; r < nrhs[p]; ++r) st.sc[r] = st.done[r] ? 0.0 : st.rnorm[r];
+      ANNEAL_HIP(hipMemcpyAsync(f.scale_, st.sc.data(), nrhs[p] * sizeof(double), hipMemcpyHostToDevice, s_));
+      axpy_f32_to_f64(f.px_, U[p], f.scale_, g0.nnode(), g0.ldn(), nrhs[p], s_);
+    }
+  }
+  ANNEAL_HIP(hipStreamSynchronize(s_));
+  for (int p = 0; p < np; ++p) {
+    status[p] = S[p].finished ? S[p].status : 1;
+    for (int r = 0; r < 64; ++r) iters[p * 64 + r] = r < nrhs[p] ? S[p].its[r] : 0;
+  }
+}
+
+}  // namespace anneal

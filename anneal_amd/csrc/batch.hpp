@@ -1,0 +1,38 @@
+// Several independent FEM problems (any grid sizes) solved together.
+#pragma once
+#include <vector>
+
+#include "fem_ctx.hpp"
+
+namespace anneal {
+
+class BatchSolver {
+ public:
+  // The contexts must share one stream; they stay owned by the caller.
+  explicit BatchSolver(const std::vector<FemContext*>& fems);
+  ~BatchSolver();
+  BatchSolver(const BatchSolver&) = delete;
+  BatchSolver& operator=(const BatchSolver&) = delete;
+
+  // Per problem p: F[p], U[p] FP64 [nrhs[p]][3][ldn_p] (U holds x0 when
+  // use_x0).  status[p]: 0 converged, 1 maxiter exhausted, 2 non-finite;
+  // iters[p * 64 + r]: PCG iterations of rhs r.
+  void solve(const std::vector<const double*>& F, const std::vector<double*>& U, const std::vector<int>& nrhs,
+             double rtol, int maxiter, bool use_x0, int* status, int* iters);
+  int size() const { return (int)f_.size(); }
+
+ private:
+  void issue_iteration(FemContext& f, int nrhs);
+  void record(const std::vector<int>& active, const std::vector<int>& nrhs);
+  void drop_graph();
+
+  std::vector<FemContext*> f_;
+  hipStream_t s_ = nullptr, cap_ = nullptr;
+  std::vector<hipStream_t> br_;
+  std::vector<hipEvent_t> join_;
+  hipEvent_t fork_ = nullptr, done_ev_ = nullptr;
+  hipGraphExec_t graph_ = nullptr;
+  std::vector<int> graph_key_;
+};
+
+}  // namespace anneal

@@ -1,11 +1,13 @@
 // extern "C" boundary of libanneal_hip.so (declared in include/anneal_hip.h).
 #include "../../include/anneal_hip.h"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "fem_ctx.hpp"
 #include "fields.hpp"
 #include "topopt.hpp"
@@ -491,6 +493,47 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
     ANNEAL_CHECK(F_dev && U_dev && in_dev && tgt_dev, "null argument");
     fem_of(fem)->fem->learned_pair(F_dev, U_dev, nrhs, level, e_ref, in_dev, tgt_dev);
     return 0;
+  });
+}
+
+int anneal_batch_create(void** batch, void* const* fems, int nprob) {
+  return guarded([&] {
+    ANNEAL_CHECK(batch && fems && nprob >= 1 && nprob <= 64, "need 1..64 FEM contexts");
+    *batch = nullptr;
+    std::vector<FemContext*> v;
+    for (int i = 0; i < nprob; ++i) v.push_back(fem_of(fems[i])->fem);
+    *batch = new BatchSolver(v);
+    return 0;
+  });
+}
+
+int anneal_batch_destroy(void* batch) {
+  return guarded([&] {
+    delete static_cast<BatchSolver*>(batch);
+    return 0;
+  });
+}
+
+int anneal_batch_solve(void* batch, const double* const* F_dev, double* const* U_dev, const int* nrhs, double rtol,
+                       int maxiter, int use_x0, int* status, int* iters) {
+  return guarded([&] {
+    ANNEAL_CHECK(batch && F_dev && U_dev && nrhs && status && iters, "null argument");
+    ANNEAL_CHECK(rtol > 0.0 && maxiter >= 1, "need rtol > 0 and maxiter >= 1");
+    BatchSolver* b = static_cast<BatchSolver*>(batch);
+    const int np = b->size();
+    std::vector<const double*> F(F_dev, F_dev + np);
+    std::vector<double*> U(U_dev, U_dev + np);
+    std::vector<int> nr(nrhs, nrhs + np);
+    for (int p = 0; p < np; ++p) ANNEAL_CHECK(F[p] && U[p], "null F or U");
+    std::vector<int> st(np);
+    b->solve(F, U, nr, rtol, maxiter, use_x0 != 0, st.data(), iters);
+    int worst = 0;
+    for (int p = 0; p < np; ++p) {
+      status[p] = st[p] == 0 ? 0 : (st[p] == 1 ? 3 : 4);
+      worst = std::max(worst, status[p]);
+    }
+    if (worst) set_error("some problems of the batch did not converge (see status)");
+    return worst;
   });
 }
 

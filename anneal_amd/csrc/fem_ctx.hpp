@@ -122,6 +122,7 @@ class FemContext {
   int inner_max = 400;
 
  private:
+  friend class BatchSolver;
   template <typename T>
   ApplyArgs<T> args(int l) const;
   void* alloc(size_t bytes);

@@ -223,10 +223,16 @@ class SimpRun:
 
     def step(self):
         """One SIMP iteration (optimize.py:105-175); returns its history entry."""
+        self._pre_solve()
+        its = self.fem.solve_device(self.F, self.U, self.ncase, self.rtol, self.maxiter, use_x0=self.have_u)
+        return self._post_solve(its)
+
+    def _pre_solve(self):
+        """Continuation, physical density, stiffness law and the rebuild decision (optimize.py:105-124)."""
         torch, fem, filt, T = self.torch, self.fem, self.filt, self.T
         self.it += 1
         it = self.it
-        t0 = time.time()
+        self._t0 = time.time()
         p = _schedule_p(it, self.iters, self.p_final, self.warm)
         if p >= self.p_final and not self.warm and self.beta_max > 0:        # optimize.py:111-116
             if self.beta == 0:
@@ -234,6 +240,7 @@ class SimpRun:
             elif self.beta_since >= self.beta_step and self.beta < self.beta_max:
                 self.beta, self.beta_since = min(self.beta_max, 2 * self.beta), 0
             self.beta_since += 1
+        self._p = p
         beta = self.beta
         nel, flags, stream = self.nel, filt.flags, self.stream
         ta = self._tick()
@@ -244,10 +251,16 @@ class SimpRun:
         tb = self._tick()
         T["physical"] += tb - ta
         last = fem.cg_iters[-1] if fem.cg_iters else None
-        rebuild = self.base_iters is None or (last is not None and last > 1.6 * self.base_iters)
-        if rebuild:
+        self._rebuild = self.base_iters is None or (last is not None and last > 1.6 * self.base_iters)
+        if self._rebuild:
             fem.rebuild()
-        its = fem.solve_device(self.F, self.U, self.ncase, self.rtol, self.maxiter, use_x0=self.have_u)
+        self._tb = tb
+
+    def _post_solve(self, its):
+        """Compliance, sensitivities, OC update and the history entry (optimize.py:125-175)."""
+        torch, fem, filt, T = self.torch, self.fem, self.filt, self.T
+        it, p, beta, rebuild, t0, tb = self.it, self._p, self.beta, self._rebuild, self._t0, self._tb
+        nel, flags, stream = self.nel, filt.flags, self.stream
         fem.cg_iters.extend(its)
         self.have_u = True
         if rebuild:

@@ -96,6 +96,95 @@ def large_part(shape=LARGE_SHAPE, seed=LARGE_SEED, keep_frac=0.05, solid_frac=0.
     return g, np.ones(g.nelem), active, frozen, fixed, loads, weights
 
 
+BATCH_SEED = 3
+BATCH_SHAPES = ((64, 64, 64), (96, 48, 48), (128, 64, 64))
+
+
+def batch_problems(seed=BATCH_SEED, n=16, ncase=4, iters=40, shapes=BATCH_SHAPES, r_mm=1.5):
+    """Independent parts of uneven sizes, each with a block of load cases.
+
+    Per problem (seeded): a grid drawn from ``shapes``; one of the six faces
+    clamped; ``ncase`` unit point loads at nodes of the opposite face (middle
+    60 % of its extent) with directions uniform on the sphere; volume fraction
+    U(0.2, 0.4).  Returns a list of SimpRun argument dicts (BatchSimp input).
+    """
+    import numpy as np
+    from anneal_amd.fem import Grid
+    rng = np.random.Generator(np.random.PCG64(seed))
+    probs = []
+    for _ in range(n):
+        shape = tuple(shapes[int(rng.integers(len(shapes)))])
+        nx, ny, nz = shape
+        dims = (nx, ny, nz)
+        g = Grid(shape, 1.0, (0, 0, 0))
+        nodes = np.arange(g.nnode)
+        coord = (nodes % (nx + 1), (nodes // (nx + 1)) % (ny + 1), nodes // ((nx + 1) * (ny + 1)))
+        axis, side = int(rng.integers(3)), int(rng.integers(2))
+        clamp = nodes[coord[axis] == (0 if side == 0 else dims[axis])]
+        fixed = (3 * clamp[:, None] + np.arange(3)).ravel()
+        far = dims[axis] if side == 0 else 0
+        loads, weights = {}, {}
+        others = [a for a in range(3) if a != axis]
+        for c in range(ncase):
+            pos = [0, 0, 0]
+            pos[axis] = far
+            for a in others:
+                pos[a] = int(rng.integers(int(0.2 * dims[a]), int(0.8 * dims[a]) + 1))
+            node = pos[0] + (nx + 1) * (pos[1] + (ny + 1) * pos[2])
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            F = np.zeros(3 * g.nnode)
+            F[3 * node:3 * node + 3] = d
+            loads[f"case{c}"] = F
+            weights[f"case{c}"] = 1.0
+        probs.append(dict(grid=g, fill=np.ones(g.nelem), active=np.ones(g.nelem, bool),
+                          frozen=np.zeros(g.nelem, bool), fixed=fixed, loads=loads, weights=weights, E0=1.0, nu=0.3,
+                          vf=float(rng.uniform(0.2, 0.4)), iters=iters, r_mm=r_mm, tol=0.0, emin=1e-9, rtol=1e-6))
+    return probs
+
+
+def batch_bench(steps, seq_steps, log):
+    """Lock-step batch of 16 problems vs the same problems stepped one after another."""
+    import torch
+    from anneal_amd.batch import BatchSimp
+    from anneal_amd.optimize import SimpRun
+    probs = batch_problems()
+    b = BatchSimp(probs)
+    b.step()                                     # first iteration (cold solves, graph recording) untimed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cg = sum(sum(r.fem.cg_iters) for r in b.runs)
+    comp = [r.hist[-1]["compliance"] for r in b.runs]
+    b.close()
+    del b
+    # unbatched: each problem on its own, same iterations
+    seq = [SimpRun(*[p[k] for k in ("grid", "fill", "active", "frozen", "fixed", "loads", "weights", "E0", "nu", "vf")],
+                   **{k: v for k, v in p.items() if k not in ("grid", "fill", "active", "frozen", "fixed", "loads",
+                                                               "weights", "E0", "nu", "vf")}, log=lambda m: None)
+           for p in probs]
+    for r in seq:
+        r.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(seq_steps):
+        for r in seq:
+            r.step()
+    torch.cuda.synchronize()
+    dts = time.perf_counter() - t1
+    n = len(probs)
+    shapes = [p["grid"].shape for p in probs]
+    log(f"bench: batch {n} problems x {steps} iterations in {dt:.3f} s ({dt / steps * 1e3:.2f} ms per batch "
+        f"iteration); one at a time {dts / seq_steps * 1e3:.2f} ms per round")
+    return {"value": round(n * steps / dt, 4), "unit": "problem SIMP iter/s", "ms_per_step": round(1e3 * dt / steps, 3),
+            "sequential_ms_per_step": round(1e3 * dts / seq_steps, 3),
+            "speedup_vs_one_at_a_time": round((dts / seq_steps) / (dt / steps), 4), "pcg_iters_total": int(cg),
+            "shapes": [list(s) for s in shapes], "load_cases": 4, "final_compliance": comp}
+
+
 def apply_roofline(fem, reps=20, nbuf=8):
     """Average duration of one FP32 K·u apply over rotating (u, y) buffers."""
     import torch
@@ -198,7 +287,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
-    ap.add_argument("--workload", choices=["cantilever", "large"], default="cantilever",
+    ap.add_argument("--workload", choices=["cantilever", "large", "batch"], default="cantilever",
                     help="cantilever: 192x96x96, 100 iterations (the headline); large: 384x192x192 with "
                          "keep-outs, keep-solid slabs and 3 load cases, 60 iterations")
     args = ap.parse_args()
@@ -209,6 +298,20 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     from anneal_amd.optimize import SimpRun
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.workload == "batch":
+        steps = 20 if args.steps is None else args.steps
+        res = batch_bench(steps, max(2, steps // 4), log)
+        out = {"metric": "problem-iterations/s, 16 uneven parts x 4 load cases, lock-step batch (plain MG-PCG @1e-6)",
+               "value": res["value"], "unit": res["unit"], "n_gpus": 1, "steps": steps, "warmup": 1,
+               "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 operator/smoother, f64 dots/refinement/density", "data": "synthetic (no dataset)",
+               "config": {"workload": f"batch seed {BATCH_SEED}: 16 parts from {list(BATCH_SHAPES)}, seeded clamp "
+                                      f"faces, 4 point-load cases each, vf U(0.2,0.4), rmin 1.5, Emin 1e-9",
+                          **{k: v for k, v in res.items() if k not in ("value", "unit", "ms_per_step")}},
+               "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
 
     if args.workload == "cantilever":
         shape, iters = SHAPE, 100

@@ -213,6 +213,19 @@ int anneal_fem_learned_stage(void* fem, const float* r_dev, float* z_dev, int nr
 int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev, int nrhs, int level, double e_ref,
                             uint16_t* in_dev, float* tgt_dev);
 
+/* ---- several independent problems solved together (no counterpart in the
+ * reference, whose FEM.solve handles one grid at a time, fem.py:206-232) ----
+ * A batch groups FEM contexts of any grid sizes that share one stream.  Each
+ * PCG iteration of all unconverged problems is one hipGraph launch whose
+ * branches (one per problem) run concurrently; the host waits once per
+ * iteration for the whole batch.  F[p], U[p]: FP64 [nrhs[p]][3][ldn_p];
+ * status[p] as anneal_fem_solve (0, 3 not converged, 4 non-finite);
+ * iters[p * 64 + r] the PCG iterations of rhs r.  Plain multigrid only. */
+int anneal_batch_create(void** batch, void* const* fems, int nprob);
+int anneal_batch_destroy(void* batch);
+int anneal_batch_solve(void* batch, const double* const* F_dev, double* const* U_dev, const int* nrhs, double rtol,
+                       int maxiter, int use_x0, int* status, int* iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,74 @@
+"""Lock-step batch of independent problems (anneal_batch_*) against solving each problem alone."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _clamped(shape):
+    from anneal_amd.fem import Grid
+    g = Grid(shape, 1.0, (0, 0, 0))
+    n = np.arange(g.nnode)
+    clamp = n[n % (shape[0] + 1) == 0]
+    return g, (3 * clamp[:, None] + np.arange(3)).ravel()
+
+
+def test_batch_solve_matches_individual_solves():
+    from anneal_amd.batch import BatchSolve
+    from anneal_amd.fem import FEM
+    rng = np.random.default_rng(0)
+    specs = [((16, 8, 8), 2), ((12, 12, 6), 1), ((20, 10, 10), 3), ((8, 8, 8), 4)]
+    fems, Ks, Fs, Us, Fred = [], [], [], [], []
+    for shape, nr in specs:
+        g, fixed = _clamped(shape)
+        fem = FEM(g, np.ones(g.nelem, bool), fixed, 0.3, nrhs_max=4)
+        K = fem.assemble(rng.uniform(0.01, 1.0, g.nelem))
+        fem._activate(K)
+        fem.rebuild()
+        F = [fem.reduce_load(rng.normal(size=3 * g.nnode)) for _ in range(nr)]
+        fems.append(fem)
+        Ks.append(K)
+        Fred.append(F)
+        Fs.append(fem._to_device(F))
+        Us.append(torch.zeros_like(Fs[-1]))
+    print("batch solve of", [s for s, _ in specs], flush=True)
+    b = BatchSolve(fems)
+    its = b.solve(Fs, Us, [nr for _, nr in specs], 1e-9, 2000)
+    print("batch iterations", its, flush=True)
+    for fem, K, F, U, (shape, nr) in zip(fems, Ks, Fred, Us, specs):
+        ub = fem._from_device(U, nr)
+        ui = fem.solve(K, F, rtol=1e-9)
+        for r in range(nr):
+            assert np.linalg.norm(F[r] - K @ ub[r]) <= 1e-9 * np.linalg.norm(F[r]) * 1.0001
+            assert np.linalg.norm(ub[r] - ui[r]) <= 1e-7 * np.linalg.norm(ui[r])
+    # a second solve of the same batch replays the recorded graph; warm start from the solution
+    its2 = b.solve(Fs, Us, [nr for _, nr in specs], 1e-9, 2000, use_x0=True)
+    assert all(i == 0 for p in its2 for i in p)
+    b.close()
+    for f in fems:
+        f.close()
+
+
+def test_batch_simp_matches_individual_runs():
+    import bench
+    from anneal_amd.batch import BatchSimp
+    from anneal_amd.optimize import SimpRun
+    probs = bench.batch_problems(seed=11, n=3, ncase=2, iters=8, shapes=((16, 8, 8), (12, 12, 12), (20, 10, 8)))
+    for p in probs:
+        p["rtol"] = 1e-10
+    print("batch SIMP of", [p["grid"].shape for p in probs], flush=True)
+    b = BatchSimp(probs)
+    for _ in range(8):
+        b.step()
+    keys = ("grid", "fill", "active", "frozen", "fixed", "loads", "weights", "E0", "nu", "vf")
+    for p, rb in zip(probs, b.runs):
+        ri = SimpRun(*[p[k] for k in keys], **{k: v for k, v in p.items() if k not in keys}, log=lambda m: None)
+        for _ in range(8):
+            ri.step()
+        for hb, hi in zip(rb.hist, ri.hist):
+            assert np.allclose(hb["compliance"], hi["compliance"], rtol=1e-7, atol=0)
+            assert abs(hb["volumeFraction"] - hi["volumeFraction"]) < 1e-9
+        assert np.abs(rb.result()["x"] - ri.result()["x"]).max() < 1e-6
+        ri.fem.close()
+    b.close()
