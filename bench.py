@@ -29,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 SHAPE = (192, 96, 96)
+# FP32 apply tile shapes (TX, TY, ZC) in the library's variant order
+TILE_SHAPES = ["16,16,8", "16,16,16", "32,8,8", "32,8,16", "32,16,8", "16,16,4", "64,4,16", "16,16,2", "8,8,2",
+               "8,8,4", "16,8,2"]
 HBM_PEAK_GBS = 8000.0
 
 
@@ -185,14 +188,54 @@ def batch_bench(steps, seq_steps, log):
             "shapes": [list(s) for s in shapes], "load_cases": 4, "final_compliance": comp}
 
 
+COPY_BYTES = 256 * 1024 * 1024
+
+
+def traffic_probe():
+    """Counter-run workload: the benchmark's apply on rotating buffers, then a plain copy of known size."""
+    import numpy as np
+    import torch
+    from anneal_amd import _lib
+    from anneal_amd.fem import FEM
+    g, fill, active, frozen, fixed, F = cantilever(SHAPE)
+    fem = FEM(g, active, fixed, 0.3, nrhs_max=1)
+    rng = np.random.default_rng(0)
+    fem.assemble_device(torch.from_numpy(1e-9 + rng.random(g.nelem) ** 3).cuda())
+    bufs = [(torch.randn(1, 3, fem.ldn, device="cuda"), torch.empty(1, 3, fem.ldn, device="cuda")) for _ in range(8)]
+    torch.cuda.synchronize()
+    for variant in range(len(TILE_SHAPES)):        # every tile shape, 8 launches each on distinct buffers
+        fem.autotune(variant)
+        for i in range(8):
+            u, y = bufs[i]
+            _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(u), _lib.ptr(y), 1)
+    a = torch.empty(COPY_BYTES // 4, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"probe: {len(TILE_SHAPES)} tile shapes x 8 applies, 4 copies of {COPY_BYTES} bytes", file=sys.stderr,
+          flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters, copy-calibrated) from the committed
+    counter summary profiles/apply_192x96x96_traffic.json, or None."""
+    path = os.path.join(ROOT, "profiles", "apply_192x96x96_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
 def apply_roofline(fem, reps=20, nbuf=8):
     """Average duration of one FP32 K·u apply over rotating (u, y) buffers."""
     import torch
     from anneal_amd import _lib
     g = fem.grid
-    shapes = ["16,16,8", "16,16,16", "32,8,8", "32,8,16", "32,16,8", "16,16,4", "64,4,16", "16,16,2", "8,8,2",
-              "8,8,4", "16,8,2"]
-    tile = shapes[fem.tile()] + ",kApply"
+    tile = TILE_SHAPES[fem.tile()] + ",kApply"
     bufs = [(torch.randn(1, 3, fem.ldn, device="cuda", dtype=torch.float32),
              torch.empty(1, 3, fem.ldn, device="cuda", dtype=torch.float32)) for _ in range(nbuf)]
     for u, y in bufs:           # warm
@@ -211,9 +254,10 @@ def apply_roofline(fem, reps=20, nbuf=8):
     ms = e0.elapsed_time(e1) / n
     nbytes = 24 * g.nnode + 4 * g.nelem          # u read, y write (12 B/node each), Ee (4 B/elem)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    kernel = f"k_apply<float,{tile}>"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_apply<float,{tile}>", "bytes_per_launch": nbytes, "us_per_launch": round(ms * 1e3, 2),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
+            "kernel": kernel, "bytes_per_launch": nbytes, "us_per_launch": round(ms * 1e3, 2),
             "dof_applies_per_s": 3 * g.nnode / (ms * 1e-3)}
 
 
@@ -287,6 +331,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--probe", choices=["apply-traffic"], default=None,
+                    help="apply-traffic: 16 FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
+                         "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); prints nothing else")
     ap.add_argument("--workload", choices=["cantilever", "large", "batch"], default="cantilever",
                     help="cantilever: 192x96x96, 100 iterations (the headline); large: 384x192x192 with "
                          "keep-outs, keep-solid slabs and 3 load cases, 60 iterations")
@@ -299,6 +346,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     from anneal_amd.optimize import SimpRun
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.probe == "apply-traffic":
+        traffic_probe()
+        return
     if args.workload == "batch":
         steps = 20 if args.steps is None else args.steps
         res = batch_bench(steps, max(2, steps // 4), log)
