@@ -20,7 +20,7 @@ SIGNATURES = {
     "anneal_abi_version": [],
     "anneal_ke_host": [c_double, c_double, c_void_p, c_void_p],
     "anneal_fem_create": [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
-                          c_int, c_void_p],
+                          c_int, c_int, c_void_p],
     "anneal_fem_destroy": [c_void_p],
     "anneal_fem_info": [c_void_p, c_i64p, c_intp, c_void_p],
     "anneal_fem_level_info": [c_void_p, c_int, c_intp, c_intp, c_intp, c_intp],

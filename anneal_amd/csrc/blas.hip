@@ -154,17 +154,21 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
 // updates kGJB rows; the pivot rows M_K,: are staged in LDS.
 constexpr int kGJB = 16;
 __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k0,
-                                                  int bk) {
+                                                  int bk, int stage) {
   extern __shared__ double sm[];
   double* P = sm;                  // kGJB x kGJB
   double* T = P + kGJB * kGJB;     // kGJB x kGJB: rows of this workgroup times P
-  double* MK = T + kGJB * kGJB;    // bk x n pivot rows
+  // bk x n pivot rows: staged in LDS when they fit, else read in place (L2)
+  const double* MK = stage ? T + kGJB * kGJB : A + (int64_t)k0 * n;
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x * kGJB;
   const int nr = min(kGJB, n - r0);
-  for (int idx = tid; idx < bk * n; idx += 256) {
-    const int l = idx / n, j = idx - l * n;
-    MK[idx] = A[(int64_t)(k0 + l) * n + j];
+  if (stage) {
+    double* mk = T + kGJB * kGJB;
+    for (int idx = tid; idx < bk * n; idx += 256) {
+      const int l = idx / n, j = idx - l * n;
+      mk[idx] = A[(int64_t)(k0 + l) * n + j];
+    }
   }
   __syncthreads();
   const int pi = tid / kGJB, pj = tid % kGJB;
@@ -220,8 +224,10 @@ void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* k
   hipLaunchKernelGGL(k_dense_assemble, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, Ee, mask, ke, A, nx, ny,
                      nz);
   ANNEAL_HIP(hipGetLastError());
-  const size_t lds = (size_t)(2 * kGJB * kGJB + kGJB * n) * sizeof(double);
-  ANNEAL_CHECK(lds <= 150 * 1024, "coarsest multigrid level too large for the dense inverse");
+  ANNEAL_CHECK(n <= 3 * 4096, "coarsest multigrid level too large for the dense inverse");
+  const size_t lds_staged = (size_t)(2 * kGJB * kGJB + kGJB * n) * sizeof(double);
+  const int stage = lds_staged <= 150 * 1024 ? 1 : 0;
+  const size_t lds = stage ? lds_staged : (size_t)(2 * kGJB * kGJB) * sizeof(double);
   static bool attr = false;
   if (!attr) {
     ANNEAL_HIP(hipFuncSetAttribute((const void*)k_bgj_step, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -233,7 +239,7 @@ void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* k
   const unsigned g = (unsigned)((n + kGJB - 1) / kGJB);
   for (int k0 = 0; k0 < n; k0 += kGJB) {
     const int bk = std::min(kGJB, n - k0);
-    hipLaunchKernelGGL(k_bgj_step, dim3(g), dim3(256), lds, s, src, dst, n, k0, bk);
+    hipLaunchKernelGGL(k_bgj_step, dim3(g), dim3(256), lds, s, src, dst, n, k0, bk, stage);
     ANNEAL_HIP(hipGetLastError());
     std::swap(src, dst);
   }

@@ -74,7 +74,7 @@ int anneal_ke_host(double nu, double h, double* ke_host, double* coef_host) {
 }
 
 int anneal_fem_create(void** ctx, int nx, int ny, int nz, double h, double nu, const uint8_t* active_host,
-                      const uint8_t* fixed_host, int nrhs_max, void* stream) {
+                      const uint8_t* fixed_host, int nrhs_max, int max_coarse_nodes, void* stream) {
   return guarded([&] {
     ANNEAL_CHECK(ctx && active_host && fixed_host, "null argument");
     ANNEAL_CHECK(h > 0.0, "voxel size must be positive");
@@ -82,7 +82,9 @@ int anneal_fem_create(void** ctx, int nx, int ny, int nz, double h, double nu, c
     bool any = false;
     for (int64_t e = 0; e < ne && !any; ++e) any = active_host[e] != 0;
     if (!any) throw std::invalid_argument("No active voxels: the design space is empty.");
-    auto* fem = new FemContext(nx, ny, nz, h, nu, active_host, fixed_host, nrhs_max, (hipStream_t)stream, 200);
+    ANNEAL_CHECK(max_coarse_nodes >= 0 && max_coarse_nodes <= 4096, "max_coarse_nodes must be in [0, 4096]");
+    auto* fem = new FemContext(nx, ny, nz, h, nu, active_host, fixed_host, nrhs_max, (hipStream_t)stream,
+                               max_coarse_nodes > 0 ? max_coarse_nodes : kDefaultCoarseNodes);
     uint8_t* flags = nullptr;
     try {
       ANNEAL_HIP(hipMalloc(&flags, ne));
@@ -212,7 +214,10 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       ANNEAL_CHECK(value > 0.0 && value < 1.0, "inner_tol_floor must be in (0, 1)");
       f->inner_tol_floor = value;
     } else if (n == "fuse_first_step") {
-      f->fuse_first_step = value != 0.0;
+      f->fuse_first_from = value != 0.0 ? 0 : 99;
+    } else if (n == "fuse_first_level") {
+      ANNEAL_CHECK(value >= 0.0, "fuse_first_level must be >= 0");
+      f->fuse_first_from = (int)value;
       f->invalidate_graphs();
     } else if (n == "rebuild_each_solve") {
       f->rebuild_each_solve = value != 0.0;

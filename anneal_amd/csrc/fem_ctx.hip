@@ -561,7 +561,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   float* nxt = x;
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
-  if (nu_pre == 2 && fuse_first_step) {
+  if (nu_pre == 2 && l >= fuse_first_from) {
     // steps 1 and 2 fused: x1 = omega D^-1 b is formed inside the kernel,
     // x2 lands where the unfused sequence would put it
     a.u = nullptr;

@@ -14,6 +14,9 @@ namespace anneal {
 
 class UNet;
 
+// Coarsening stops once a level has at most this many nodes (dense FP64 inverse there).
+constexpr int kDefaultCoarseNodes = 700;
+
 // 24x24 hex8 stiffness for E = 1, cube of side h, 2x2x2 Gauss quadrature.
 // Corner q = a + 2b + 4c for the corner at (a, b, c) in {0,1}^3, dof 3q + comp.
 void hex8_stiffness(double nu, double h, double* ke /*576*/);
@@ -96,7 +99,9 @@ class FemContext {
   // onto the network level in output units.
   void learned_pair(const double* F, const double* U, int nrhs, int level, double e_ref, uint16_t* in, float* tgt);
   void invalidate_graphs() { drop_graphs(); }
-  bool fuse_first_step = false;  // form x1 = omega D^-1 b inside the first Chebyshev kernel
+  // Levels l >= fuse_first_from form x1 = omega D^-1 b inside the first
+  // Chebyshev kernel (one launch fewer per level and cycle)
+  int fuse_first_from = 1;
   int tile() const { return lv_[0].tile; }
 
   const GridDims& grid() const { return lv_[0].g; }

@@ -360,14 +360,18 @@ __device__ __forceinline__ void fw_stencil(int I, int n, int (&idx)[3], float (&
   }
 }
 
-// Full-weighting restriction b_c = P^T r (P = trilinear prolongation), one
-// coarse node per thread, all three components, fixed summation order.
+// Full-weighting restriction b_c = P^T r (P = trilinear prolongation): one
+// thread per (coarse node, component), 27 loads each, fixed summation order.
+// (A thread per node with all three components made the coarse levels
+// latency-bound: too few waves, 81 loads each.)
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ maskc, int nx,
                            int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
   const int64_t nc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
-  const int64_t N = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (N >= nc) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * nc) return;
+  const int c = int(t / nc);
+  const int64_t N = t - (int64_t)c * nc;
   const int r = blockIdx.y;
   const int I = N % (cx + 1), J = (N / (cx + 1)) % (cy + 1), K = N / ((int64_t)(cx + 1) * (cy + 1));
   const uint8_t mk = maskc[N];
@@ -377,21 +381,17 @@ __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const u
   fw_stencil(J, ny, iy, wy);
   fw_stencil(K, nz, iz, wz);
   const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
-  float acc[3] = {0.f, 0.f, 0.f};
+  const T* src = rf + (int64_t)(3 * r + c) * ldf;
+  float acc = 0.f;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T* src = rf + (int64_t)(3 * r + c) * ldf;
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const float w = wz[a] * wy[b];
-        const T* row = src + iz[a] * sxy + iy[b] * sx;
-        acc[c] += w * (wx[0] * (float)row[ix[0]] + wx[1] * (float)row[ix[1]] + wx[2] * (float)row[ix[2]]);
-      }
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) bc[(int64_t)(3 * r + c) * ldc + N] = ((mk >> c) & 1) ? T(acc[c]) : T(0);
+    for (int b = 0; b < 3; ++b) {
+      const float w = wz[a] * wy[b];
+      const T* row = src + iz[a] * sxy + iy[b] * sx;
+      acc += w * (wx[0] * (float)row[ix[0]] + wx[1] * (float)row[ix[1]] + wx[2] * (float)row[ix[2]]);
+    }
+  bc[(int64_t)(3 * r + c) * ldc + N] = ((mk >> c) & 1) ? T(acc) : T(0);
 }
 
 // Prolongation with correction: x_f += M_f P x_c (up to 8 coarse nodes per fine node).
@@ -513,7 +513,7 @@ void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const Grid
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(c.nnode(), 256), nrhs), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(3 * c.nnode(), 256), nrhs), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
                      c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
   ANNEAL_HIP(hipGetLastError());
 }

@@ -148,7 +148,7 @@ class DeviceOperator:
 class FEM:
     """Linear elastic solve on the active voxels of a grid (fem.py:130-263), on the GPU."""
 
-    def __init__(self, grid, active, fixed_dofs, nu, log=None, nrhs_max=4):
+    def __init__(self, grid, active, fixed_dofs, nu, log=None, nrhs_max=4, max_coarse_nodes=None):
         t0 = time.time()
         torch = _torch()
         self.grid = grid
@@ -169,7 +169,8 @@ class FEM:
         ctx = _lib.c_void_p()
         act_u8 = np.ascontiguousarray(active, dtype=np.uint8)
         _lib.call("anneal_fem_create", _lib.ctypes.byref(ctx), grid.nx, grid.ny, grid.nz, grid.h, self.nu,
-                  _lib.ptr(act_u8), _lib.ptr(fixed_bits), self.nrhs_max, _lib.c_void_p(self.stream.cuda_stream))
+                  _lib.ptr(act_u8), _lib.ptr(fixed_bits), self.nrhs_max, int(max_coarse_nodes or 0),
+                  _lib.c_void_p(self.stream.cuda_stream))
         self._ctx = ctx
         ldn, lev = _lib.c_int64(), _lib.c_int()
         mask = np.zeros(grid.nnode, np.uint8)

@@ -46,9 +46,11 @@ int anneal_ke_host(double nu, double h, double* ke_host, double* coef_host);
 /* FEM.__init__ (fem.py:133-196): active voxels and fixed dofs define the free
  * set.  active_host: nelem bytes (non-zero = active).  fixed_host: nnode bytes,
  * bit c set when dof 3n+c is fixed.  nrhs_max: largest right-hand-side block
- * (load cases) solved at once.  Builds the multigrid hierarchy. */
+ * (load cases) solved at once.  Builds the multigrid hierarchy, coarsening
+ * until a level has at most max_coarse_nodes nodes (0 = the library default;
+ * that level is solved with a dense FP64 inverse, so keep it <= 4096). */
 int anneal_fem_create(void** ctx, int nx, int ny, int nz, double h, double nu, const uint8_t* active_host,
-                      const uint8_t* fixed_host, int nrhs_max, void* stream);
+                      const uint8_t* fixed_host, int nrhs_max, int max_coarse_nodes, void* stream);
 int anneal_fem_destroy(void* ctx);
 /* Leading dimension of nodal vectors, number of multigrid levels, and the
  * free-dof mask (nnode bytes, bit c = dof 3n+c free) copied to mask_host if
