@@ -206,6 +206,10 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree must be in [1, 15]");
       f->nu_pre = (int)value;
       f->invalidate_graphs();
+    } else if (n == "smoother_degree_coarse") {
+      ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree_coarse must be in [1, 15]");
+      f->nu_coarse = (int)value;
+      f->invalidate_graphs();
     } else if (n == "cheb_ratio") {
       ANNEAL_CHECK(value > 1.0, "cheb_ratio must exceed 1");
       f->cheb_ratio = value;

@@ -545,25 +545,27 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
     return;
   }
   const int64_t n = L.g.nnode(), ld = L.g.ldn();
+  const int nu = l == 0 ? nu_pre : nu_coarse;  // smoothing degree on this level
   // Chebyshev recurrence (three-term form): x_{k+1} = x_k + c1_k (x_k - x_{k-1}) + c2_k D^-1 (b - K x_k)
   const double lmax = cheb_lmax, lmin = lmax / cheb_ratio;
   const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sg = th / de;
   double c1[16], c2[16];
   double rho = 1.0 / sg;
-  for (int k = 1; k < nu_pre && k < 16; ++k) {
+  for (int k = 1; k < nu && k < 16; ++k) {
     const double rn = 1.0 / (2.0 * sg - rho);
     c1[k] = rn * rho;
     c2[k] = 2.0 * rn / de;
     rho = rn;
   }
-  // 2*nu_pre - 1 buffer swaps in total: start in x2 so the result lands in x.
+  // 2*nu - 1 buffer swaps in total: start in x2 so the result lands in x.
   float* cur = L.x2;
   float* nxt = x;
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
-  if (nu_pre == 2 && l >= fuse_first_from) {
+  if (nu >= 2 && l >= fuse_first_from) {
     // steps 1 and 2 fused: x1 = omega D^-1 b is formed inside the kernel,
-    // x2 lands where the unfused sequence would put it
+    // x2 lands where the unfused sequence would put it; step 3 forms its
+    // previous iterate x1 at the node as well
     a.u = nullptr;
     a.prev = nullptr;
     a.y = nxt;
@@ -572,9 +574,21 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
     a.c2 = float(c2[1]);
     launch_apply(a, kCheb0, nrhs, s_);
     std::swap(cur, nxt);
+    for (int k = 2; k < nu; ++k) {
+      a.u = cur;
+      a.prev = (k == 2) ? nullptr : nxt;
+      a.prev_j0 = (k == 2) ? 1 : 0;
+      a.omega0 = float(1.0 / th);
+      a.y = nxt;
+      a.c1 = float(c1[k]);
+      a.c2 = float(c2[k]);
+      launch_apply(a, kCheb, nrhs, s_);
+      std::swap(cur, nxt);
+    }
+    a.prev_j0 = 0;
   } else {
     jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
-    for (int k = 1; k < nu_pre; ++k) {
+    for (int k = 1; k < nu; ++k) {
       a.u = cur;
       a.prev = (k == 1) ? nullptr : nxt;
       a.y = nxt;
@@ -596,7 +610,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   a.omega = float(1.0 / th);
   launch_apply(a, kJacobi, nrhs, s_);
   std::swap(cur, nxt);
-  for (int k = 1; k < nu_pre; ++k) {
+  for (int k = 1; k < nu; ++k) {
     a.u = cur;
     a.prev = nxt;
     a.y = nxt;

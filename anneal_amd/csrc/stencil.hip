@@ -176,6 +176,10 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     if (MODE == kCheb && pr) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
+    } else if (MODE == kCheb && a.prev_j0) {
+      // previous iterate x1 = omega0 D^-1 b of a smoothing that started from zero
+#pragma unroll
+      for (int c = 0; c < 3; ++c) E.pv[c] = a.omega0 * T(E.dv) * E.bv[c];
     }
   };
 

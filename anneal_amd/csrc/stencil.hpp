@@ -31,6 +31,8 @@ struct ApplyArgs {
   T omega;
   const T* prev;       // kCheb: previous iterate (read at the node only, so it may alias y)
   T c1, c2;            // kCheb recurrence coefficients
+  T omega0;            // kCheb with prev_j0: the previous iterate is omega0 D^-1 b (formed at the node)
+  int prev_j0;
   int nx, ny, nz;
   int64_t ldn;
   int ntx, nty, nzc;   // tiles in x, y and z chunks (set by the launcher)
