@@ -101,7 +101,7 @@ class FemContext {
   void invalidate_graphs() { drop_graphs(); }
   // Levels l >= fuse_first_from form x1 = omega D^-1 b inside the first
   // Chebyshev kernel (one launch fewer per level and cycle)
-  int fuse_first_from = 1;
+  int fuse_first_from = 0;
   int tile() const { return lv_[0].tile; }
 
   const GridDims& grid() const { return lv_[0].g; }
