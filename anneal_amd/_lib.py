@@ -70,6 +70,8 @@ SIGNATURES = {
     "anneal_batch_destroy": [c_void_p],
     "anneal_batch_solve": [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_intp, c_double, c_int, c_int,
                            c_intp, c_intp],
+    "anneal_fem_stress": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                          c_void_p],
     "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p],
 }

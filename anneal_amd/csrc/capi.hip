@@ -253,6 +253,27 @@ int anneal_fem_element_energy(void* ctx, const double* U_dev, int nrhs, double* 
   });
 }
 
+int anneal_fem_stress(void* ctx, const double* U_dev, int nrhs, const double* B0_host, const double* D_host,
+                      const int* local_host, const double* E_dev, double E, double* sig_dev, double* vm_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(U_dev && B0_host && D_host && local_host && sig_dev && nrhs >= 1, "null argument");
+    FemHandle* h = fem_of(ctx);
+    StressMats m;
+    for (int a = 0; a < 6; ++a) {
+      for (int b = 0; b < 24; ++b) m.B0[a][b] = B0_host[a * 24 + b];
+      for (int b = 0; b < 6; ++b) m.D[a][b] = D_host[a * 6 + b];
+    }
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c) {
+        m.local[q][c] = local_host[q * 3 + c];
+        ANNEAL_CHECK(m.local[q][c] == 0 || m.local[q][c] == 1, "corner offsets must be 0 or 1");
+      }
+    element_stress(U_dev, nrhs, h->fem->ldn(), h->fem->grid(), m, h->flags, E_dev, E, sig_dev, vm_dev,
+                   h->fem->stream());
+    return 0;
+  });
+}
+
 int anneal_fem_sensitivity(void* ctx, const double* U_dev, int nrhs, const uint8_t* flags_dev, const double* xphys_dev,
                            const double* dxp_dev, const double* phi_dev, const double* coef_dev, double p, double emin,
                            double E0, double* dcdx_dev, double* dvdx_dev) {

@@ -598,6 +598,64 @@ __global__ __launch_bounds__(384) void k_design_stats_final(const double* __rest
   if (lane == 0) out[q] = v;
 }
 
+// Element-centre stress and von Mises (anneal/fem.py:244-263): eps = B0 u_e
+// with the 24 element dofs in the reference's corner order (corner q at
+// offset (a, b, c) = local[q]), sig = E D eps, Voigt xx yy zz xy yz zx.
+__global__ void k_element_stress(const double* __restrict__ U, int nrhs, int64_t ldn, int nx, int ny, int nz,
+                                 StressMats m, const uint8_t* __restrict__ flags, const double* __restrict__ E,
+                                 double Es, double* __restrict__ sig, double* __restrict__ vm) {
+  const int64_t ne = (int64_t)nx * ny * nz;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int i = e % nx, j = (e / nx) % ny, kk = e / ((int64_t)nx * ny);
+  const bool act = flags[e] & kActive;
+  const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+  const double Ee = E ? E[e] : Es;
+  for (int r = 0; r < nrhs; ++r) {
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    if (act) {
+      double ue[24];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t node = (i + m.local[q][0]) + sx * (j + m.local[q][1]) + sxy * (kk + m.local[q][2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * q + c] = U[(int64_t)(3 * r + c) * ldn + node];
+      }
+      double eps[6];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < 24; ++b) t += m.B0[a][b] * ue[b];
+        eps[a] = t;
+      }
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) t += m.D[a][b] * eps[b];
+        s[a] = t * Ee;
+      }
+    }
+    double* o = sig + ((int64_t)r * ne + e) * 6;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) o[a] = s[a];
+    if (vm) {
+      const double d0 = s[0] - s[1], d1 = s[1] - s[2], d2 = s[2] - s[0];
+      vm[(int64_t)r * ne + e] =
+          sqrt(0.5 * (d0 * d0 + d1 * d1 + d2 * d2) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+    }
+  }
+}
+
+void element_stress(const double* U, int nrhs, int64_t ldn, const GridDims& g, const StressMats& m,
+                    const uint8_t* flags, const double* E, double Es, double* sig, double* vm, hipStream_t s) {
+  const int64_t n = g.nelem();
+  hipLaunchKernelGGL(k_element_stress, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, U, nrhs, ldn, g.nx, g.ny,
+                     g.nz, m, flags, E, Es, sig, vm);
+  ANNEAL_HIP(hipGetLastError());
+}
+
 void element_energy(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
                     const uint8_t* flags, double* out, hipStream_t s) {
   const int64_t n = g.nelem();

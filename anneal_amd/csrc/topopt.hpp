@@ -60,6 +60,18 @@ class FilterContext {
 // ce_r(e) = u_e^T ke u_e for E = 1 (anneal/fem.py:239-242), out [nrhs][nelem].
 void element_energy(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,
                     const uint8_t* flags, double* out, hipStream_t s);
+// Strain-displacement matrix at the element centre, elasticity matrix and the
+// corner order of the 24 element dofs (anneal/fem.py:20-21, 36-84).
+struct StressMats {
+  double B0[6][24];
+  double D[6][6];
+  int local[8][3];
+};
+// sig[r][e][6] = E_e D B0 u_e (Voigt xx yy zz xy yz zx), vm[r][e] its von Mises
+// stress (fem.py:244-263); E per element (full grid) or the scalar Es when E is
+// null; inactive voxels get zeros.  vm may be null.
+void element_stress(const double* U, int nrhs, int64_t ldn, const GridDims& g, const StressMats& m,
+                    const uint8_t* flags, const double* E, double Es, double* sig, double* vm, hipStream_t s);
 // dc*dxp and phi*dxp (anneal/optimize.py:118, 125-142):
 //   dce_r = -p x^(p-1) (1-emin) E0 phi ce_r; dc = min(sum_r coef_r dce_r, 0)
 void sensitivity(const double* U, int nrhs, int64_t ldn, const GridDims& g, const KeCoef<double>& k,

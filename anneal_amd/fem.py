@@ -359,11 +359,37 @@ class FEM:
         en = self.grid.element_nodes(self.act)
         return (3 * en[:, :, None] + np.arange(3)).reshape(-1, 24)
 
+    def stress_device(self, U, nrhs, E_material, von_mises=True):
+        """Element-centre stress on the device for SoA FP64 solutions U [nrhs, 3, ldn].
+
+        Returns (sig [nrhs, nelem, 6], vm [nrhs, nelem] or None) over the full grid
+        (zeros on inactive voxels); E_material is a scalar or one value per active
+        element (fem.py:244-263).
+        """
+        torch = _torch()
+        nel = self.grid.nelem
+        sig = torch.empty(nrhs, nel, 6, dtype=torch.float64, device=self.device)
+        vm = torch.empty(nrhs, nel, dtype=torch.float64, device=self.device) if von_mises else None
+        E_dev, Es = None, 0.0
+        if np.ndim(E_material):
+            Ef = np.zeros(nel)
+            Ef[self.act] = np.asarray(E_material, float)
+            E_dev = torch.from_numpy(Ef).to(self.device)
+        else:
+            Es = float(E_material)
+        B0 = np.ascontiguousarray(self.B0, dtype=np.float64)
+        D = np.ascontiguousarray(self.D, dtype=np.float64)
+        loc = np.ascontiguousarray(LOCAL, dtype=np.int32)
+        _lib.call("anneal_fem_stress", self._ctx, _lib.ptr(U), int(nrhs), _lib.ptr(B0), _lib.ptr(D), _lib.ptr(loc),
+                  _lib.ptr(E_dev) if E_dev is not None else None, Es, _lib.ptr(sig),
+                  _lib.ptr(vm) if vm is not None else None)
+        return sig, vm
+
     def stress(self, u_full, E_material):
-        """Element-centre stress (MPa), Voigt order xx yy zz xy yz zx (fem.py:244-248)."""
-        ue = np.asarray(u_full)[self.edof]
-        sig = (ue @ self.B0.T) @ self.D.T
-        return sig * (np.asarray(E_material)[:, None] if np.ndim(E_material) else E_material)
+        """Element-centre stress (MPa), Voigt order xx yy zz xy yz zx, per active element (fem.py:244-248)."""
+        U = self._to_device([np.asarray(u_full, float)], reduced=False)
+        sig, _ = self.stress_device(U, 1, E_material, von_mises=False)
+        return sig[0].cpu().numpy()[self.act]
 
     @staticmethod
     def normal_stress(sig, d):
@@ -374,7 +400,7 @@ class FEM:
                 + 2 * (sig[:, 3] * x * y + sig[:, 4] * y * z + sig[:, 5] * z * x))
 
     def von_mises(self, u_full, E_material):
-        """Element-centre von Mises stress (fem.py:258-263)."""
-        s = self.stress(u_full, E_material)
-        sx, sy, sz, txy, tyz, tzx = s.T
-        return np.sqrt(0.5 * ((sx - sy) ** 2 + (sy - sz) ** 2 + (sz - sx) ** 2) + 3 * (txy ** 2 + tyz ** 2 + tzx ** 2))
+        """Element-centre von Mises stress per active element (fem.py:258-263), computed on the GPU."""
+        U = self._to_device([np.asarray(u_full, float)], reduced=False)
+        _, vm = self.stress_device(U, 1, E_material)
+        return vm[0].cpu().numpy()[self.act]

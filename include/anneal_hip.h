@@ -151,6 +151,15 @@ int anneal_design_stats(const double* xold_dev, const double* xnew_dev, const do
 int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs, double* out_dev,
                    double* work_dev, void* stream);
 
+/* FEM.stress / FEM.von_mises (fem.py:244-263) for nodal solutions U_dev
+ * (FP64 [nrhs][3][ldn]): sig_dev[r][e][6] = E_e D B0 u_e at the element centre
+ * (Voigt xx yy zz xy yz zx), vm_dev[r][e] (may be null) its von Mises stress.
+ * B0_host (6x24, dof 3q+c in the corner order local_host[8][3]) and D_host
+ * (6x6) are hex8_matrices' outputs (fem.py:71-84); E_dev is a per-element
+ * modulus over the full grid or null for the scalar E.  Inactive voxels get 0. */
+int anneal_fem_stress(void* ctx, const double* U_dev, int nrhs, const double* B0_host, const double* D_host,
+                      const int* local_host, const double* E_dev, double E, double* sig_dev, double* vm_dev);
+
 /* ---- result fields on a display mesh (anneal/fields.py) ------------------ */
 
 /* fields.corner_values (fields.py:17-37) after the npz has been read

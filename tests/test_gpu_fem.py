@@ -220,3 +220,23 @@ def test_vcycle_is_symmetric_positive_definite(torch, shape, inactive):
         for j in range(i + 1, 3):
             a, b = d(rs[i], zs[j]), d(rs[j], zs[i])
             assert abs(a - b) <= 1e-5 * (abs(a) + abs(b)), (a, b)
+
+
+@pytest.mark.parametrize("scalar_E", [True, False])
+def test_stress_and_von_mises_match_oracle(scalar_E):
+    """FEM.stress / von_mises on the device vs the FP64 restatement of fem.py:244-263."""
+    from anneal_amd.fem import FEM
+    g, active, fixed, Ee = random_problem((10, 6, 5), 21, inactive_frac=0.2)
+    fem = FEM(g, active, fixed, 0.3)
+    ref = oracle_fem(g, active, fixed)
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=3 * g.nnode)
+    E = 210e3 if scalar_E else rng.uniform(1.0, 3.0, fem.act.size)
+    sig, sig_ref = fem.stress(u, E), ref.stress(u, E)
+    vm, vm_ref = fem.von_mises(u, E), ref.von_mises(u, E)
+    assert sig.shape == sig_ref.shape and vm.shape == vm_ref.shape
+    assert np.abs(sig - sig_ref).max() <= 1e-12 * np.abs(sig_ref).max()
+    assert np.abs(vm - vm_ref).max() <= 1e-12 * np.abs(vm_ref).max()
+    d = rng.normal(size=3)
+    assert np.allclose(FEM.normal_stress(sig, d), ref.normal_stress(sig_ref, d), rtol=1e-12, atol=0)
+    fem.close()
