@@ -217,9 +217,11 @@ def traffic_probe():
           flush=True)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters, copy-calibrated) from the committed
     counter summary profiles/apply_192x96x96_traffic.json, or None."""
+    if tuple(shape) != SHAPE:
+        return None
     path = os.path.join(ROOT, "profiles", "apply_192x96x96_traffic.json")
     try:
         with open(path) as f:
@@ -256,7 +258,7 @@ def apply_roofline(fem, reps=20, nbuf=8):
     achieved = nbytes / (ms * 1e-3) / 1e9
     kernel = f"k_apply<float,{tile}>"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, (g.nx, g.ny, g.nz)),
             "kernel": kernel, "bytes_per_launch": nbytes, "us_per_launch": round(ms * 1e3, 2),
             "dof_applies_per_s": 3 * g.nnode / (ms * 1e-3)}
 
@@ -334,9 +336,10 @@ def main():
     ap.add_argument("--probe", choices=["apply-traffic"], default=None,
                     help="apply-traffic: 16 FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
                          "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); prints nothing else")
-    ap.add_argument("--workload", choices=["cantilever", "large", "batch"], default="cantilever",
-                    help="cantilever: 192x96x96, 100 iterations (the headline); large: 384x192x192 with "
-                         "keep-outs, keep-solid slabs and 3 load cases, 60 iterations")
+    ap.add_argument("--workload", choices=["cantilever", "large", "batch", "cantilever128"], default="cantilever",
+                    help="cantilever: 192x96x96, 100 iterations (the headline); cantilever128: the same at "
+                         "128x64x64 (the learned-preconditioner size); large: 384x192x192 with keep-outs, "
+                         "keep-solid slabs and 3 load cases, 60 iterations; batch: 16 parts x 4 load cases")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this path runs on a single MI355X (no multi-GPU variant is built)",
@@ -363,11 +366,12 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    if args.workload == "cantilever":
-        shape, iters = SHAPE, 100
+    if args.workload in ("cantilever", "cantilever128"):
+        shape, iters = (SHAPE if args.workload == "cantilever" else (128, 64, 64)), 100
         g, fill, active, frozen, fixed, F = cantilever(shape)
         loads, weights = {"c": F}, {"c": 1.0}
-        workload = "cantilever 192x96x96 hex8, rmin 3, Emin 1e-9, MG-PCG rtol 1e-6, 100-it schedule"
+        workload = (f"cantilever {shape[0]}x{shape[1]}x{shape[2]} hex8, rmin 3, Emin 1e-9, MG-PCG rtol 1e-6, "
+                    f"100-it schedule")
     else:
         shape, iters = LARGE_SHAPE, 60
         g, fill, active, frozen, fixed, loads, weights = large_part()
@@ -403,7 +407,7 @@ def main():
     roof = apply_roofline(run.fem, reps=args.apply_reps)
     print(f"bench: apply {roof['us_per_launch']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
     lrn = None
-    if not args.no_learned and args.workload == "cantilever":
+    if not args.no_learned and args.workload in ("cantilever", "cantilever128"):
         lrn = learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, ms, mode=args.learned_mode,
                           log=log)
         print(f"bench: learned {lrn['ms_per_step']} ms/iter ({lrn['speedup_vs_plain']}x plain), PCG iterations "
