@@ -150,30 +150,25 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
 // pivot block K of up to kGJB rows, out of place between two buffers:
 //   new_KK = P = M_KK^-1, new_KJ = P M_KJ, new_IK = -M_IK P,
 //   new_IJ = M_IJ - (M_IK P) M_KJ.
-// Each workgroup inverts the small pivot block in LDS (redundantly) and
-// updates kGJB rows; the pivot rows M_K,: are staged in LDS.
-constexpr int kGJB = 16;
+// A workgroup owns kGJB rows x kGJC columns: it inverts the small pivot block
+// in LDS (redundantly), forms T = M_IK P for its rows, stages the pivot rows of
+// its columns in LDS, and writes its tile with coalesced row segments.
+constexpr int kGJB = 16, kGJC = 256;
 __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k0,
-                                                  int bk, int stage) {
-  extern __shared__ double sm[];
-  double* P = sm;                  // kGJB x kGJB
-  double* T = P + kGJB * kGJB;     // kGJB x kGJB: rows of this workgroup times P
-  // bk x n pivot rows: staged in LDS when they fit, else read in place (L2)
-  const double* MK = stage ? T + kGJB * kGJB : A + (int64_t)k0 * n;
+                                                  int bk) {
+  __shared__ double P[kGJB * kGJB];
+  __shared__ double T[kGJB * kGJB];
+  __shared__ double MK[kGJB * kGJC];
   const int tid = threadIdx.x;
-  const int r0 = blockIdx.x * kGJB;
-  const int nr = min(kGJB, n - r0);
-  if (stage) {
-    double* mk = T + kGJB * kGJB;
-    for (int idx = tid; idx < bk * n; idx += 256) {
-      const int l = idx / n, j = idx - l * n;
-      mk[idx] = A[(int64_t)(k0 + l) * n + j];
-    }
-  }
-  __syncthreads();
+  const int r0 = blockIdx.y * kGJB, c0 = blockIdx.x * kGJC;
+  const int nr = min(kGJB, n - r0), nc = min(kGJC, n - c0);
   const int pi = tid / kGJB, pj = tid % kGJB;
   const bool inP = pi < bk && pj < bk;
-  if (inP) P[pi * kGJB + pj] = MK[pi * n + k0 + pj];
+  if (inP) P[pi * kGJB + pj] = A[(int64_t)(k0 + pi) * n + k0 + pj];
+  for (int idx = tid; idx < bk * kGJC; idx += 256) {
+    const int l = idx / kGJC, cj = idx - l * kGJC;
+    MK[idx] = cj < nc ? A[(int64_t)(k0 + l) * n + c0 + cj] : 0.0;
+  }
   __syncthreads();
   for (int k = 0; k < bk; ++k) {
     const double piv = P[k * kGJB + k];
@@ -197,21 +192,24 @@ __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, 
     T[pi * kGJB + pj] = t;
   }
   __syncthreads();
-  for (int idx = tid; idx < nr * n; idx += 256) {
-    const int ri = idx / n, j = idx - ri * n;
+  const int cj = tid;  // one column of the tile per thread, rows in turn
+  if (cj >= nc) return;
+  const int j = c0 + cj;
+  const bool ck = j >= k0 && j < k0 + bk;
+  for (int ri = 0; ri < nr; ++ri) {
     const int row = r0 + ri;
-    const bool rk = row >= k0 && row < k0 + bk, ck = j >= k0 && j < k0 + bk;
+    const bool rk = row >= k0 && row < k0 + bk;
     double v;
     if (rk && ck) {
       v = P[(row - k0) * kGJB + (j - k0)];
     } else if (rk) {
       v = 0.0;
-      for (int l = 0; l < bk; ++l) v += P[(row - k0) * kGJB + l] * MK[l * n + j];
+      for (int l = 0; l < bk; ++l) v += P[(row - k0) * kGJB + l] * MK[l * kGJC + cj];
     } else if (ck) {
       v = -T[ri * kGJB + (j - k0)];
     } else {
       v = A[(int64_t)row * n + j];
-      for (int l = 0; l < bk; ++l) v -= T[ri * kGJB + l] * MK[l * n + j];
+      for (int l = 0; l < bk; ++l) v -= T[ri * kGJB + l] * MK[l * kGJC + cj];
     }
     B[(int64_t)row * n + j] = v;
   }
@@ -225,21 +223,12 @@ void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* k
                      nz);
   ANNEAL_HIP(hipGetLastError());
   ANNEAL_CHECK(n <= 3 * 4096, "coarsest multigrid level too large for the dense inverse");
-  const size_t lds_staged = (size_t)(2 * kGJB * kGJB + kGJB * n) * sizeof(double);
-  const int stage = lds_staged <= 150 * 1024 ? 1 : 0;
-  const size_t lds = stage ? lds_staged : (size_t)(2 * kGJB * kGJB) * sizeof(double);
-  static bool attr = false;
-  if (!attr) {
-    ANNEAL_HIP(hipFuncSetAttribute((const void*)k_bgj_step, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   150 * 1024));
-    attr = true;
-  }
   double* src = A;
   double* dst = B;
-  const unsigned g = (unsigned)((n + kGJB - 1) / kGJB);
+  const dim3 grid((unsigned)((n + kGJC - 1) / kGJC), (unsigned)((n + kGJB - 1) / kGJB));
   for (int k0 = 0; k0 < n; k0 += kGJB) {
     const int bk = std::min(kGJB, n - k0);
-    hipLaunchKernelGGL(k_bgj_step, dim3(g), dim3(256), lds, s, src, dst, n, k0, bk, stage);
+    hipLaunchKernelGGL(k_bgj_step, grid, dim3(256), 0, s, src, dst, n, k0, bk);
     ANNEAL_HIP(hipGetLastError());
     std::swap(src, dst);
   }

@@ -68,7 +68,12 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
     rlam = 1.0 / sqrt(a.oc[kOcLam]);
     mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
   }
-  for (int idx = threadIdx.x; idx < LX * LY * LZ; idx += FTX * FTY) {
+  // fixed trip count, unrolled: every thread's tile loads are in flight together
+  constexpr int kStage = (LX * LY * LZ + FTX * FTY - 1) / (FTX * FTY);
+#pragma unroll
+  for (int it = 0; it < kStage; ++it) {
+    const int idx = threadIdx.x + it * FTX * FTY;
+    if (idx >= LX * LY * LZ) break;
     const int lx = idx % LX, ly = (idx / LX) % LY, lz = idx / (LX * LY);
     const int gx = X0 - R + lx, gy = Y0 - R + ly, gz = Z0 - R + lz;
     double v = 0.0;
