@@ -22,7 +22,7 @@ int apply_variants();
 
 void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
                      hipStream_t s);
-void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, hipStream_t s);
+void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, float q, hipStream_t s);
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
                      hipStream_t s);

@@ -210,6 +210,9 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree_coarse must be in [1, 15]");
       f->nu_coarse = (int)value;
       f->invalidate_graphs();
+    } else if (n == "coarsen_power") {
+      ANNEAL_CHECK(value > 0.0 && value <= 1.0, "coarsen_power must be in (0, 1]");
+      f->coarsen_power = value;
     } else if (n == "cheb_ratio") {
       ANNEAL_CHECK(value > 1.0, "cheb_ratio must exceed 1");
       f->cheb_ratio = value;

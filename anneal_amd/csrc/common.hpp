@@ -44,6 +44,16 @@ struct HipError : std::runtime_error {
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Consecutive workgroups are dealt round-robin to the 8 XCDs of the MI355X;
+// this remaps a hardware block index to a logical tile index so that each XCD
+// receives a contiguous run of logical tiles, keeping the halos that
+// neighbouring tiles share inside one L2.
+__device__ __forceinline__ int xcd_logical(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = bid & 7;
+  return x * q + min(x, r) + (bid >> 3);
+}
+
+
 struct GridDims {
   int nx, ny, nz;
   int64_t nelem() const { return (int64_t)nx * ny * nz; }

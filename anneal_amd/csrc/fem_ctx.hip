@@ -462,7 +462,8 @@ void FemContext::set_Ee(const double* Ee_dev) {
   const GridDims& g0 = lv_[0].g;
   ANNEAL_HIP(hipMemcpyAsync(Ee64_, Ee_dev, g0.nelem() * sizeof(double), hipMemcpyDeviceToDevice, s_));
   cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
-  for (size_t l = 1; l < lv_.size(); ++l) launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, s_);
+  for (size_t l = 1; l < lv_.size(); ++l)
+    launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, (float)coarsen_power, s_);
   for (size_t l = 0; l < lv_.size(); ++l) launch_inv_diag(lv_[l].Ee, lv_[l].mask, lv_[l].dinv, lv_[l].g, lv_[l].kdiag, s_);
   if (!tuned_ && g0.nelem() >= 4096) {
     tuned_ = true;

@@ -120,6 +120,8 @@ class FemContext {
   // <= bound * sum_e Ee x_e^T diag(ke) x_e = bound * x^T D x), so the V-cycle
   // stays symmetric positive definite however heterogeneous Ee is.
   int nu_pre = 2;
+  // coarse Ee = power mean of the children, (mean E^q)^(1/q); takes effect at the next set_Ee
+  double coarsen_power = 0.5;
   int nu_coarse = 3;  // degree on the coarse levels (l >= 1): cheaper per cell, fewer PCG iterations
   double cheb_ratio = 15.0;
   double cheb_lmax = 0.0;  // set from ke at construction

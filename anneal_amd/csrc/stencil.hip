@@ -53,14 +53,6 @@ __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t v
   }
 }
 
-// Block index -> (tile x, tile y, z chunk).  Consecutive workgroups are
-// dealt round-robin to the 8 XCDs; remap so that each XCD receives a
-// contiguous run of logical tiles (z chunks fastest), keeping the plane and
-// column halos that neighbouring tiles share inside one L2.
-__device__ __forceinline__ int xcd_logical(int bid, int nb) {
-  const int q = nb >> 3, r = nb & 7, x = bid & 7;
-  return x * q + min(x, r) + (bid >> 3);
-}
 
 template <typename T, int TX, int TY, int ZC, int MODE>
 __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
@@ -335,8 +327,12 @@ __global__ void k_inv_diag(const float* __restrict__ Ee, const uint8_t* __restri
 }
 
 // Rediscretised coarse stiffness scale: mean of the (up to 8) children.
+// Coarse-element stiffness scale from its (up to) 8 children: the power mean
+// (mean of E^q)^(1/q); q = 1 is the arithmetic mean (an upper bound of the
+// block's stiffness), smaller q lean toward the soft children, which matters
+// where solid and void voxels mix.  Missing children (odd sizes) count as 0.
 __global__ void k_coarsen_Ee(const float* __restrict__ Ef, float* __restrict__ Ec, int nx, int ny, int nz, int cx,
-                             int cy, int cz) {
+                             int cy, int cz, float q) {
   const int64_t nc = (int64_t)cx * cy * cz;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= nc) return;
@@ -346,9 +342,13 @@ __global__ void k_coarsen_Ee(const float* __restrict__ Ef, float* __restrict__ E
     for (int dj = 0; dj < 2; ++dj)
       for (int di = 0; di < 2; ++di) {
         const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
-        if (i < nx && j < ny && k < nz) s += Ef[i + (int64_t)nx * (j + (int64_t)ny * k)];
+        if (i < nx && j < ny && k < nz) {
+          const double v = Ef[i + (int64_t)nx * (j + (int64_t)ny * k)];
+          s += q == 1.0f ? v : pow(fmax(v, 0.0), (double)q);
+        }
       }
-  Ec[e] = float(s * 0.125);
+  const double m = s * 0.125;
+  Ec[e] = float(q == 1.0f ? m : pow(m, 1.0 / (double)q));
 }
 
 // 1-D stencil of the trilinear prolongation seen from coarse index I: fine
@@ -509,9 +509,9 @@ void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const Gr
   hipLaunchKernelGGL(k_inv_diag, dim3(nblk(g.nnode(), 256)), dim3(256), 0, s, Ee, mask, dinv, g.nx, g.ny, g.nz, kdiag);
   ANNEAL_HIP(hipGetLastError());
 }
-void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, hipStream_t s) {
+void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, float q, hipStream_t s) {
   hipLaunchKernelGGL(k_coarsen_Ee, dim3(nblk(c.nelem(), 256)), dim3(256), 0, s, Ef, Ec, f.nx, f.ny, f.nz, c.nx, c.ny,
-                     c.nz);
+                     c.nz, q);
   ANNEAL_HIP(hipGetLastError());
 }
 template <typename T>

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
-  int bid = blockIdx.x;
+  int bid = xcd_logical(blockIdx.x, gridDim.x);  // neighbouring tiles share one XCD's L2
   const int bx = bid % ntx;
   bid /= ntx;
   const int by = bid % nty;
