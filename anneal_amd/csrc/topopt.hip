@@ -32,7 +32,7 @@ struct FilterArgs {
   int nx, ny, nz;
 };
 
-constexpr int FTX = 32, FTY = 8, FTZ = 8;
+constexpr int FTX = 32, FTY = 8, FTZ = 16;
 
 __device__ __forceinline__ void project(double xt, double beta, double& xp, double& dxp) {
   if (beta <= 0.0) {
