@@ -388,7 +388,7 @@ void FemContext::learned_pair(const double* F, const double* U, int nrhs, int le
 
 void FemContext::vcycle_pcg(int nrhs) {
   if (!use_graphs) {
-    vcycle(0, pr_, pz_, nrhs);
+    vcycle(0, pr_, pz_, nrhs, part_);
     return;
   }
   if (!vgraph_[nrhs]) {
@@ -397,7 +397,7 @@ void FemContext::vcycle_pcg(int nrhs) {
     hipGraph_t g = nullptr;
     try {
       ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
-      vcycle(0, pr_, pz_, nrhs);
+      vcycle(0, pr_, pz_, nrhs, part_);
       ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
     } catch (...) {
       s_ = saved;
@@ -539,7 +539,7 @@ int FemContext::autotune(int variant, float* us_out) {
   return lv_[0].tile;
 }
 
-void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
+void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_part) {
   Level& L = lv_[l];
   if (l == (int)lv_.size() - 1) {
     dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_);
@@ -609,6 +609,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
   a.u = cur;
   a.y = nxt;
   a.omega = float(1.0 / th);
+  a.dot_part = (l == 0 && nu == 1) ? dot_part : nullptr;  // the last kernel of the cycle forms <b, x>
   launch_apply(a, kJacobi, nrhs, s_);
   std::swap(cur, nxt);
   for (int k = 1; k < nu; ++k) {
@@ -617,6 +618,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs) {
     a.y = nxt;
     a.c1 = float(c1[k]);
     a.c2 = float(c2[k]);
+    a.dot_part = (l == 0 && k == nu - 1) ? dot_part : nullptr;
     launch_apply(a, kCheb, nrhs, s_);
     std::swap(cur, nxt);
   }
@@ -680,9 +682,12 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
     if (net_ && net_mode_ == 1) learned_warm_start(nrhs);
-    precondition(nrhs);
-    dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
-    finalize_st(part_, kDotBlocks, nrhs, st_, kRZ, 0, s_);
+    if (!precondition(nrhs)) {
+      dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+      finalize_st(part_, kDotBlocks, nrhs, st_, kRZ, 0, s_);
+    } else {
+      finalize_st(part_, nblk_apply, nrhs, st_, kRZ, 0, s_);
+    }
     pcg_dir(pp_, pz_, st_, n, ld, nrhs, 1, s_);
     // rr of the last two iterations (per rhs) to predict convergence: when the
     // next residual is not expected to meet the tolerance, the next V-cycle,
@@ -707,14 +712,15 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       }
       const bool flexible = net_ && net_mode_ == 2;
       auto next_direction = [&]() {
-        precondition(nrhs);
-        dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+        const bool fused = precondition(nrhs);
+        if (!fused) dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+        const int nblk_rz = fused ? nblk_apply : kDotBlocks;
         if (flexible) {
-          finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 4, s_);  // rz for the next alpha
+          finalize_st(part_, nblk_rz, nrhs, st_, kRZnew, 4, s_);  // rz for the next alpha
           dot_partial(pz_, pq_, n, ld, nrhs, part_, s_);
           finalize_st(part_, kDotBlocks, nrhs, st_, kZQ, 3, s_);  // beta = -<z, K p> / <p, K p>
         } else {
-          finalize_st(part_, kDotBlocks, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
+          finalize_st(part_, nblk_rz, nrhs, st_, kRZnew, 2, s_);  // rz_new, beta
         }
         pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
       };
@@ -773,16 +779,19 @@ void FemContext::scaled_network_step(const float* r, float* z1, float* t, int nr
   net_scaled_resid(r, pt_, z1, t, net_nrm_ + 192, ld, nrhs, s_);  // t = r - gamma K z1, z1 *= gamma
 }
 
-void FemContext::precondition(int nrhs) {
+bool FemContext::precondition(int nrhs) {
   if (!net_ || net_mode_ != 2) {
     vcycle_pcg(nrhs);
-    return;
+    // the cycle's last smoothing kernel leaves the <r, z> partials in part_;
+    // a one-level hierarchy is a dense solve only and leaves none
+    return lv_.size() > 1;
   }
   // multiplicative two-stage preconditioner: z = z1 + V(r - K z1)
   const int64_t nv = (int64_t)nrhs * 3 * lv_[0].g.ldn();
   scaled_network_step(pr_, pz1_, pt_, nrhs);
   vcycle_graph(pt_, pz_, nrhs);
   add_f32(pz_, pz1_, nv, s_);
+  return false;
 }
 
 void FemContext::learned_warm_start(int nrhs) {

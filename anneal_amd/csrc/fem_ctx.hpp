@@ -64,7 +64,9 @@ class FemContext {
   // F, U: device, [nrhs][3][ldn] FP64; U holds x0 when use_x0.  Returns 0 on
   // convergence, 1 when maxiter was exhausted, 2 on a non-finite result.
   int solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out);
-  void vcycle(int l, const float* b, float* x, int nrhs);
+  // x = M b; with dot_part the last fine-level smoothing kernel also writes the
+  // per-block partials of <b, x> (apply grid, [nrhs][blocks]).
+  void vcycle(int l, const float* b, float* x, int nrhs, double* dot_part = nullptr);
   // Tile shape of the FP32 apply: variant >= 0 sets it on every level,
   // variant < 0 times every shape on every multigrid level and keeps the
   // fastest per level (coarse levels are latency-bound and prefer short z
@@ -157,7 +159,8 @@ class FemContext {
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
   void drop_graphs();
-  void precondition(int nrhs);  // pz_ = M pr_
+  // pz_ = M pr_; returns true when part_ already holds the apply-grid partials of <pr_, pz_>
+  bool precondition(int nrhs);
   void scaled_network_step(const float* r, float* z1, float* t, int nrhs);
   void learned_warm_start(int nrhs);
   UNet* net_ = nullptr;

@@ -279,6 +279,9 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
             bstore<T>(out, rsrc(y + c * ldn + lb, nb_out), ovo);  // dropped for lanes without a node
             if (MODE == kApply) {
               if (a.dot_part && ncol) dot += (double)uc * (double)out;
+            } else if (MODE == kCheb || MODE == kJacobi) {
+              // <b, y>: the PCG's <r, z> when this step ends the fine-level V-cycle
+              if (a.dot_part && ncol) dot += (double)E.bv[c] * (double)out;
             }
           }
         }
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) Fprev[c][m] = Fcur[c][m];
   }
-  if (MODE == kApply && a.dot_part) {
+  if ((MODE == kApply || MODE == kCheb || MODE == kJacobi) && a.dot_part) {
     // fixed-order block reduction
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((tid & 63) == 0) sred[tid >> 6] = dot;
