@@ -192,7 +192,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       const int64_t n = g0.nnode(), ld = g0.ldn();
       for (int r = 0; r < nr; ++r) {
         st.sc[r] = st.done[r] ? 0.0 : 1.0 / st.rnorm[r];
-        const double want = st.done[r] ? 1.0 : 0.5 * rtol * st.fnorm[r] / st.rnorm[r];
+        const double want = st.done[r] ? 1.0 : f.inner_tol_factor * rtol * st.fnorm[r] / st.rnorm[r];
         const double t = std::max(want, f.inner_tol_floor);
         st.tol2[r] = t * t;
         for (int k = 0; k < kSt; ++k) st.hst[r * kSt + k] = 0.0;

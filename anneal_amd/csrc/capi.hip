@@ -213,6 +213,9 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
     } else if (n == "coarsen_power") {
       ANNEAL_CHECK(value > 0.0 && value <= 1.0, "coarsen_power must be in (0, 1]");
       f->coarsen_power = value;
+    } else if (n == "inner_tol_factor") {
+      ANNEAL_CHECK(value > 0.0 && value < 1.0, "inner_tol_factor must be in (0, 1)");
+      f->inner_tol_factor = value;
     } else if (n == "cheb_ratio") {
       ANNEAL_CHECK(value > 1.0, "cheb_ratio must exceed 1");
       f->cheb_ratio = value;

@@ -670,7 +670,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     // inner FP32 PCG on the residual scaled to unit norm
     for (int r = 0; r < nrhs; ++r) {
       sc[r] = done[r] ? 0.0 : 1.0 / rnorm[r];
-      const double want = done[r] ? 1.0 : 0.5 * rtol * fnorm[r] / rnorm[r];
+      const double want = done[r] ? 1.0 : inner_tol_factor * rtol * fnorm[r] / rnorm[r];
       const double t = std::max(want, inner_tol_floor);
       tol2[r] = t * t;
       for (int k = 0; k < kSt; ++k) hst[r * kSt + k] = 0.0;

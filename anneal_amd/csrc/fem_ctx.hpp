@@ -129,6 +129,9 @@ class FemContext {
   double cheb_lmax = 0.0;  // set from ke at construction
   float omega = 1.0f;      // unused by the V-cycle; kept for kJacobi callers
   double inner_tol_floor = 1e-5;
+  // the inner FP32 PCG stops at inner_tol_factor * rtol * |F| / |r| (relative to its start), so one
+  // refinement pass usually meets rtol on the FP64 residual
+  double inner_tol_factor = 0.7;
   int inner_max = 400;
 
  private:
