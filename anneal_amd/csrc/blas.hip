@@ -278,22 +278,6 @@ void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs
   ANNEAL_HIP(hipGetLastError());
 }
 
-__global__ void k_pcg_alpha(double* st, int nrhs) {
-  const int r = threadIdx.x;
-  if (r >= nrhs) return;
-  double* q = st + r * kSt;
-  const double pq = q[kPQ];
-  q[kAlpha] = (q[kDone] == 0.0 && pq > 0.0 && isfinite(pq)) ? q[kRZ] / pq : 0.0;
-}
-__global__ void k_pcg_beta(double* st, int nrhs) {
-  const int r = threadIdx.x;
-  if (r >= nrhs) return;
-  double* q = st + r * kSt;
-  const double rzn = q[kRZnew];
-  q[kBeta] = (q[kDone] == 0.0 && q[kRZ] > 0.0) ? rzn / q[kRZ] : 0.0;
-  if (q[kDone] == 0.0) q[kRZ] = rzn;
-}
-
 // x += alpha p ; r -= alpha q ; partial of r.r
 __global__ __launch_bounds__(kRedThreads) void k_pcg_update(float* __restrict__ x, float* __restrict__ rv,
                                                              const float* __restrict__ p, const float* __restrict__ q,
@@ -342,14 +326,6 @@ __global__ void k_pcg_dir(float* __restrict__ p, const float* __restrict__ z, co
     const int64_t o = (int64_t)(3 * r + c) * ld + i;
     p[o] = z[o] + be * p[o];
   }
-}
-void pcg_alpha(double* st, int nrhs, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, s, st, nrhs);
-  ANNEAL_HIP(hipGetLastError());
-}
-void pcg_beta(double* st, int nrhs, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, st, nrhs);
-  ANNEAL_HIP(hipGetLastError());
 }
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_dir, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, p, z, st, n, ld, first);

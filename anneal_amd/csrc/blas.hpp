@@ -38,12 +38,10 @@ void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, f
 void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s);
 void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
                            int ny, int nz, hipStream_t s);
-void pcg_alpha(double* st, int nrhs, hipStream_t s);
 // st[field] = sum of the partials, then op: 0 none; 1 alpha = rz / t; 2 beta = t / rz, rz = t;
 // 3 (flexible) beta = -t / pq with t = <z_new, K p>; 4 rz = t.
 void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, int op, hipStream_t s);
 void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s);
-void pcg_beta(double* st, int nrhs, hipStream_t s);
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
                 double* part, double* out_rr, hipStream_t s);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);

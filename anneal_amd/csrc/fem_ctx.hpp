@@ -127,7 +127,7 @@ class FemContext {
   int nu_coarse = 3;  // degree on the coarse levels (l >= 1): cheaper per cell, fewer PCG iterations
   double cheb_ratio = 15.0;
   double cheb_lmax = 0.0;  // set from ke at construction
-  float omega = 1.0f;      // unused by the V-cycle; kept for kJacobi callers
+  float omega = 1.0f;      // default Jacobi weight in the apply arguments (the V-cycle sets its own)
   double inner_tol_floor = 1e-5;
   // the inner FP32 PCG stops at inner_tol_factor * rtol * |F| / |r| (relative to its start), so one
   // refinement pass usually meets rtol on the FP64 residual
