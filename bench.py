@@ -217,6 +217,60 @@ def traffic_probe():
           flush=True)
 
 
+def unet_flops(dims, B, cin=8, cout=3, widths=(16, 32, 64, 64)):
+    """Multiply-add FLOPs (2 per MAC) of one U-Net forward on B samples of `dims` nodes."""
+    L = len(widths)
+    d = [tuple(dims)]
+    for _ in range(1, L):
+        d.append(tuple(n // 2 + 1 for n in d[-1]))
+    vox = [B * a * b * c for a, b, c in d]
+    macs = 0
+    for l in range(L):
+        ci = cin if l == 0 else widths[l - 1]
+        macs += vox[l] * 27 * (ci * widths[l] + widths[l] * widths[l])
+    for l in range(L - 1):
+        macs += vox[l] * 27 * ((widths[l + 1] + widths[l]) * widths[l] + widths[l] * widths[l])
+    macs += vox[0] * widths[0] * cout
+    return 2 * macs
+
+
+def unet_probe(reps=20):
+    """HIP-event timings of the BF16 MFMA U-Net at the learned stage's sizes (level 1 of 192x96x96 and
+    128x64x64, and a training batch of 32 at 25x13x13) and the achieved MFMA rate."""
+    import torch
+    from anneal_amd.learned import UNet
+    net = UNet(seed=0)
+    out = {"peak_tflops_bf16": 2500.0, "cases": []}
+    for dims, B in [((97, 49, 49), 1), ((65, 33, 33), 1), ((25, 13, 13), 32)]:
+        V = dims[0] * dims[1] * dims[2]
+        x = torch.randn(B, V, 8, device="cuda").to(torch.bfloat16)
+        y = net.forward(x, dims)
+        dout = torch.randn_like(y)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            net.forward(x, dims, y)
+        e1.record()
+        torch.cuda.synchronize()
+        f_ms = e0.elapsed_time(e1) / reps
+        e0.record()
+        for _ in range(reps):
+            net.forward(x, dims, y)
+            net.zero_grad()
+            net.backward(dout)
+        e1.record()
+        torch.cuda.synchronize()
+        fb_ms = e0.elapsed_time(e1) / reps
+        fl = unet_flops(dims, B)
+        out["cases"].append({"dims": list(dims), "batch": B, "forward_ms": round(f_ms, 4),
+                             "forward_backward_ms": round(fb_ms, 4), "forward_gflop": round(fl / 1e9, 3),
+                             "forward_tflops": round(fl / (f_ms * 1e-3) / 1e12, 2),
+                             "forward_backward_tflops": round(3 * fl / (fb_ms * 1e-3) / 1e12, 2)})
+        print(f"probe: unet {dims} x{B}: forward {f_ms:.3f} ms, fwd+bwd {fb_ms:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters, copy-calibrated) from the committed
     counter summary profiles/apply_192x96x96_traffic.json, or None."""
@@ -333,9 +387,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
-    ap.add_argument("--probe", choices=["apply-traffic"], default=None,
-                    help="apply-traffic: 16 FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
-                         "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); prints nothing else")
+    ap.add_argument("--probe", choices=["apply-traffic", "unet"], default=None,
+                    help="apply-traffic: FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
+                         "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); unet: U-Net forward and "
+                         "forward+backward timings at the learned stage's grid sizes (prints one JSON line)")
     ap.add_argument("--workload", choices=["cantilever", "large", "batch", "cantilever128"], default="cantilever",
                     help="cantilever: 192x96x96, 100 iterations (the headline); cantilever128: the same at "
                          "128x64x64 (the learned-preconditioner size); large: 384x192x192 with keep-outs, "
@@ -351,6 +406,9 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if args.probe == "apply-traffic":
         traffic_probe()
+        return
+    if args.probe == "unet":
+        print(json.dumps(unet_probe()), flush=True)
         return
     if args.workload == "batch":
         steps = 20 if args.steps is None else args.steps
