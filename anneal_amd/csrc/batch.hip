@@ -1,15 +1,15 @@
 // Lock-step mixed-precision PCG over several independent FEM contexts (grids of
 // different sizes, each with its own block of right-hand sides).
 //
-// One inner PCG iteration of every active problem is recorded as ONE hipGraph:
-// the problems' kernels sit on parallel branches (one capture stream each,
-// forked from and joined back to the launch stream), and the host waits once
-// per iteration for the whole batch instead of once per problem.  The graph is
-// recorded once per refinement pass for the problems that need inner
-// iterations; a problem (or right-hand side) that meets its tolerance earlier
-// keeps its device "done" flags, so its branch leaves x, r and p untouched
-// until the pass ends.  The FP64 true-residual refinement and its decisions are
-// the same as in FemContext::solve.
+// One inner PCG iteration of a problem is a hipGraph recorded once (per rhs
+// count) and replayed on the problem's own stream; an iteration of the batch
+// launches the graphs of the problems still iterating, forked from and joined
+// back to the launch stream, so independent problems can run concurrently and
+// finished ones cost nothing, and the host waits once per iteration for the
+// whole batch instead of once per problem.  Within a problem, a right-hand
+// side that met its tolerance keeps its device "done" flag, so its x, r and p
+// stay untouched.  The FP64 true-residual refinement and its decisions are the
+// same as in FemContext::solve.
 #include "batch.hpp"
 
 #include <algorithm>
@@ -24,8 +24,10 @@ BatchSolver::BatchSolver(const std::vector<FemContext*>& fems) : f_(fems) {
     ANNEAL_CHECK(f != nullptr, "null FEM context in the batch");
     ANNEAL_CHECK(f->s_ == s_, "the contexts of a batch must share one stream");
   }
-  ANNEAL_HIP(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
   br_.resize(f_.size());
+  graph_.assign(f_.size(), nullptr);
+  graph_nrhs_.assign(f_.size(), 0);
+  graph_gen_.assign(f_.size(), 0);
   join_.resize(f_.size());
   for (size_t i = 0; i < f_.size(); ++i) {
     ANNEAL_HIP(hipStreamCreateWithFlags(&br_[i], hipStreamNonBlocking));
@@ -37,18 +39,20 @@ BatchSolver::BatchSolver(const std::vector<FemContext*>& fems) : f_(fems) {
 
 BatchSolver::~BatchSolver() {
   (void)hipStreamSynchronize(s_);
-  drop_graph();
+  for (auto s : br_) (void)hipStreamSynchronize(s);
+  drop_graphs();
   for (auto s : br_) (void)hipStreamDestroy(s);
   for (auto e : join_) (void)hipEventDestroy(e);
   if (fork_) (void)hipEventDestroy(fork_);
   if (done_ev_) (void)hipEventDestroy(done_ev_);
-  if (cap_) (void)hipStreamDestroy(cap_);
 }
 
-void BatchSolver::drop_graph() {
-  if (graph_) (void)hipGraphExecDestroy(graph_);
-  graph_ = nullptr;
-  graph_key_.clear();
+void BatchSolver::drop_graphs() {
+  for (auto& g : graph_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 // One inner iteration of problem p (the body of FemContext::solve's inner loop
@@ -71,39 +75,33 @@ void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
   pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nrhs, 0, f.s_);
 }
 
-void BatchSolver::record(const std::vector<int>& active, const std::vector<int>& nrhs) {
-  std::vector<int> key;
-  for (int p : active) {
-    key.push_back(p);
-    key.push_back(nrhs[p]);
+void BatchSolver::record(int p, int nrhs) {
+  FemContext& f = *f_[p];
+  if (graph_[p] && graph_nrhs_[p] == nrhs && graph_gen_[p] == f.graph_generation()) return;
+  if (graph_[p]) {
+    ANNEAL_HIP(hipStreamSynchronize(br_[p]));
+    (void)hipGraphExecDestroy(graph_[p]);
+    graph_[p] = nullptr;
   }
-  if (graph_ && key == graph_key_) return;
-  drop_graph();
-  std::vector<hipStream_t> saved(f_.size());
-  for (size_t i = 0; i < f_.size(); ++i) saved[i] = f_[i]->s_;
+  hipStream_t saved = f.s_;
   hipGraph_t g = nullptr;
   try {
-    ANNEAL_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
-    ANNEAL_HIP(hipEventRecord(fork_, cap_));
-    for (int p : active) {
-      ANNEAL_HIP(hipStreamWaitEvent(br_[p], fork_, 0));
-      f_[p]->s_ = br_[p];
-      issue_iteration(*f_[p], nrhs[p]);
-      ANNEAL_HIP(hipEventRecord(join_[p], br_[p]));
-      ANNEAL_HIP(hipStreamWaitEvent(cap_, join_[p], 0));
-    }
-    ANNEAL_HIP(hipStreamEndCapture(cap_, &g));
+    ANNEAL_HIP(hipStreamBeginCapture(br_[p], hipStreamCaptureModeThreadLocal));
+    f.s_ = br_[p];
+    issue_iteration(f, nrhs);
+    f.s_ = saved;
+    ANNEAL_HIP(hipStreamEndCapture(br_[p], &g));
   } catch (...) {
-    for (size_t i = 0; i < f_.size(); ++i) f_[i]->s_ = saved[i];
+    f.s_ = saved;
     hipGraph_t dummy = nullptr;
-    (void)hipStreamEndCapture(cap_, &dummy);
+    (void)hipStreamEndCapture(br_[p], &dummy);
     if (dummy) (void)hipGraphDestroy(dummy);
     throw;
   }
-  for (size_t i = 0; i < f_.size(); ++i) f_[i]->s_ = saved[i];
-  ANNEAL_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+  ANNEAL_HIP(hipGraphInstantiate(&graph_[p], g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
-  graph_key_ = key;
+  graph_nrhs_[p] = nrhs;
+  graph_gen_[p] = f.graph_generation();
 }
 
 void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<double*>& U,
@@ -211,14 +209,21 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       active.push_back(p);
     }
     if (active.empty()) break;
-    record(active, nrhs);
+    for (int p : active) record(p, nrhs[p]);
     int inner_max = 0;
     for (int p : active) inner_max = std::max(inner_max, f_[p]->inner_max);
     for (int it = 0; it < inner_max; ++it) {
       bool any = false;
-      for (int p : active) any = any || (S[p].inner && S[p].total < maxiter);
+      ANNEAL_HIP(hipEventRecord(fork_, s_));
+      for (int p : active) {
+        if (!S[p].inner || S[p].total >= maxiter) continue;
+        any = true;
+        ANNEAL_HIP(hipStreamWaitEvent(br_[p], fork_, 0));
+        ANNEAL_HIP(hipGraphLaunch(graph_[p], br_[p]));
+        ANNEAL_HIP(hipEventRecord(join_[p], br_[p]));
+        ANNEAL_HIP(hipStreamWaitEvent(s_, join_[p], 0));
+      }
       if (!any) break;
-      ANNEAL_HIP(hipGraphLaunch(graph_, s_));
       ANNEAL_HIP(hipEventRecord(done_ev_, s_));
       ANNEAL_HIP(hipEventSynchronize(done_ev_));
       for (int p : active) {
@@ -238,8 +243,8 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
           }
           iall = iall && st.idone[r];
         }
-        if (st.bad || iall || st.total >= maxiter) {
-          // freeze this problem's branch: every rhs done on the device
+        if (st.bad || iall || st.total >= maxiter || it + 1 >= f.inner_max) {
+          // this problem's inner solve ends: its graph is no longer launched
           for (int r = 0; r < nrhs[p]; ++r) st.hst[r * kSt + kDone] = 1.0;
           changed = true;
           st.inner = false;

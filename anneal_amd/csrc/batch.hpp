@@ -23,16 +23,17 @@ class BatchSolver {
 
  private:
   void issue_iteration(FemContext& f, int nrhs);
-  void record(const std::vector<int>& active, const std::vector<int>& nrhs);
-  void drop_graph();
+  void record(int p, int nrhs);  // problem p's iteration graph (recorded once per shape)
+  void drop_graphs();
 
   std::vector<FemContext*> f_;
-  hipStream_t s_ = nullptr, cap_ = nullptr;
-  std::vector<hipStream_t> br_;
+  hipStream_t s_ = nullptr;
+  std::vector<hipStream_t> br_;  // one stream per problem
   std::vector<hipEvent_t> join_;
   hipEvent_t fork_ = nullptr, done_ev_ = nullptr;
-  hipGraphExec_t graph_ = nullptr;
-  std::vector<int> graph_key_;
+  std::vector<hipGraphExec_t> graph_;  // per problem
+  std::vector<int> graph_nrhs_;
+  std::vector<uint64_t> graph_gen_;
 };
 
 }  // namespace anneal

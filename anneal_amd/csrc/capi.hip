@@ -225,6 +225,7 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       f->inner_tol_floor = value;
     } else if (n == "fuse_first_step") {
       f->fuse_first_from = value != 0.0 ? 0 : 99;
+      f->invalidate_graphs();
     } else if (n == "fuse_first_level") {
       ANNEAL_CHECK(value >= 0.0, "fuse_first_level must be >= 0");
       f->fuse_first_from = (int)value;

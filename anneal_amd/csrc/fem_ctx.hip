@@ -274,6 +274,7 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
 }
 
 void FemContext::drop_graphs() {
+  ++graph_gen_;  // graphs recorded elsewhere from this context's kernels are stale too
   for (auto& g : vgraph_)
     if (g) {
       (void)hipGraphExecDestroy(g);

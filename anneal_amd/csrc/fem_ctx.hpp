@@ -101,6 +101,9 @@ class FemContext {
   // onto the network level in output units.
   void learned_pair(const double* F, const double* U, int nrhs, int level, double e_ref, uint16_t* in, float* tgt);
   void invalidate_graphs() { drop_graphs(); }
+  // Changes whenever recorded graphs of this context's kernels become stale
+  // (tile shapes, smoother settings).
+  uint64_t graph_generation() const { return graph_gen_; }
   // Levels l >= fuse_first_from form x1 = omega D^-1 b inside the first
   // Chebyshev kernel (one launch fewer per level and cycle)
   int fuse_first_from = 0;
@@ -162,6 +165,7 @@ class FemContext {
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
   void drop_graphs();
+  uint64_t graph_gen_ = 0;
   // pz_ = M pr_; returns true when part_ already holds the apply-grid partials of <pr_, pz_>
   bool precondition(int nrhs);
   void scaled_network_step(const float* r, float* z1, float* t, int nrhs);
