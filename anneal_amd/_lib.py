@@ -74,8 +74,28 @@ SIGNATURES = {
                           c_void_p],
     "anneal_fields_corner_values": [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p],
+    "anneal_region_select": [c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                             c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "anneal_region_sdf": [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
+    "anneal_boundary_fields": [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                               c_void_p],
+    "anneal_gather_node_bits": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "anneal_voxel_components": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_i64p, c_void_p],
+    "anneal_voxel_link": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.c_uint32, c_void_p, c_void_p,
+                          c_void_p, c_i64p, ctypes.POINTER(ctypes.c_uint32), c_void_p],
+    "anneal_voxel_threshold": [c_int64, c_void_p, c_double, c_void_p, c_void_p],
+    "anneal_voxel_count_above": [c_int64, c_void_p, c_double, c_void_p, c_i64p, c_void_p],
+    "anneal_output_field": [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "anneal_choose_level_work_bytes": [],
+    "anneal_choose_level": [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double),
+                            ctypes.POINTER(c_double), c_void_p],
+    "anneal_paths_sssp": [c_int, c_int, c_int, c_void_p, c_void_p, c_double, c_void_p, ctypes.c_uint32, c_void_p,
+                          c_void_p, c_void_p, c_void_p, c_intp, c_void_p],
+    "anneal_paths_trace": [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_i64p, c_i64p, ctypes.POINTER(c_double), c_void_p],
+    "anneal_paths_raise": [c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p, c_double, c_void_p, c_void_p],
 }
-_RESTYPE = {"anneal_last_error": ctypes.c_char_p}
+_RESTYPE = {"anneal_last_error": ctypes.c_char_p, "anneal_choose_level_work_bytes": c_int64}
 
 
 class AnnealLibError(RuntimeError):

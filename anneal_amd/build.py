@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libanneal_hip.so")
-SOURCES = ["stencil.hip", "blas.hip", "fem_ctx.hip", "topopt.hip", "fields.hip", "unet.hip", "learned.hip", "batch.hip", "capi.hip"]
+SOURCES = ["stencil.hip", "blas.hip", "fem_ctx.hip", "topopt.hip", "fields.hip", "unet.hip", "learned.hip", "batch.hip", "voxel.hip", "capi.hip"]
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) is switched off: some of its forms have returned
 # wrong results on the target machines, so every HIP file is built without it.

@@ -12,6 +12,7 @@
 #include "fields.hpp"
 #include "topopt.hpp"
 #include "unet.hpp"
+#include "voxel.hpp"
 
 namespace anneal {
 static thread_local std::string g_err;
@@ -571,6 +572,170 @@ int anneal_batch_solve(void* batch, const double* const* F_dev, double* const* U
     }
     if (worst) set_error("some problems of the batch did not converge (see status)");
     return worst;
+  });
+}
+
+
+// ---- voxel passes (problem.py, export.py) -----------------------------------
+namespace {
+GridDims voxel_grid(int nx, int ny, int nz) {
+  ANNEAL_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "need a non-empty grid");
+  return GridDims{nx, ny, nz};
+}
+}  // namespace
+
+int anneal_region_select(int nx, int ny, int nz, double h, const double* origin_host, const int* kind_dev,
+                         const double* prm_dev, const int* first_dev, const double* thr_dev, int nanchor,
+                         int at_nodes, const uint8_t* emask_dev, uint32_t* bits_dev, long long* counts_dev,
+                         void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(h > 0.0 && origin_host, "need h > 0 and an origin");
+    ANNEAL_CHECK(nanchor >= 0 && nanchor <= kMaxAnchors, "nanchor must be in [0, 32]");
+    ANNEAL_CHECK(emask_dev && bits_dev && counts_dev && (nanchor == 0 || (kind_dev && prm_dev && first_dev && thr_dev)),
+                 "null argument");
+    region_select(g, h, origin_host, RegionTable{kind_dev, prm_dev, first_dev, thr_dev, nanchor}, at_nodes != 0,
+                  emask_dev, bits_dev, counts_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_region_sdf(int64_t npts, const double* pts_dev, const int* kind_dev, const double* prm_dev,
+                      const int* first_dev, int nanchor, double* out_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(npts >= 0 && nanchor >= 0 && nanchor <= kMaxAnchors, "bad sizes");
+    ANNEAL_CHECK(npts == 0 || nanchor == 0 || (pts_dev && kind_dev && prm_dev && first_dev && out_dev),
+                 "null argument");
+    region_sdf(npts, pts_dev, RegionTable{kind_dev, prm_dev, first_dev, nullptr, nanchor}, out_dev,
+               (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_boundary_fields(int nx, int ny, int nz, const uint32_t* node_bits_dev, int nanchor,
+                           const uint8_t* dofmask_dev, const double* load_dev, int ncase, uint8_t* fixed_bits_dev,
+                           double* F_dev, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(nanchor >= 0 && nanchor <= kMaxAnchors && ncase >= 0, "bad sizes");
+    ANNEAL_CHECK(node_bits_dev && fixed_bits_dev && (nanchor == 0 || dofmask_dev) &&
+                     (ncase == 0 || (F_dev && (nanchor == 0 || load_dev))),
+                 "null argument");
+    boundary_fields(g, node_bits_dev, nanchor, dofmask_dev, load_dev, ncase, fixed_bits_dev, F_dev,
+                    (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_gather_node_bits(int nx, int ny, int nz, const uint32_t* node_bits_dev, const uint8_t* emask_dev,
+                            uint32_t* ebits_dev, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(node_bits_dev && emask_dev && ebits_dev, "null argument");
+    gather_node_bits(g, node_bits_dev, emask_dev, ebits_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_voxel_components(int nx, int ny, int nz, const uint8_t* mask_dev, int32_t* label_dev,
+                            long long* work_dev, int64_t* ncomp_out, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(mask_dev && label_dev && work_dev && ncomp_out, "null argument");
+    *ncomp_out = components(g, mask_dev, label_dev, work_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_voxel_link(int nx, int ny, int nz, const uint8_t* mask_dev, const int32_t* label_dev,
+                      const uint32_t* ebits_dev, uint32_t seed, uint8_t* keep_dev, uint8_t* out_mask_dev,
+                      long long* work_dev, int64_t* nkept_out, uint32_t* linked_out, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(mask_dev && label_dev && ebits_dev && keep_dev && work_dev && nkept_out, "null argument");
+    *nkept_out = link_components(g, mask_dev, label_dev, ebits_dev, seed, keep_dev, out_mask_dev, linked_out,
+                                 work_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_voxel_threshold(int64_t n, const double* f_dev, double t, uint8_t* mask_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(n >= 0 && (n == 0 || (f_dev && mask_dev)), "bad arguments");
+    threshold_mask(n, f_dev, t, mask_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_voxel_count_above(int64_t n, const double* f_dev, double t, long long* work_dev, int64_t* count_out,
+                             void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(n >= 0 && work_dev && count_out && (n == 0 || f_dev), "bad arguments");
+    *count_out = count_above(n, f_dev, t, work_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_output_field(int64_t n, const double* x_dev, const double* fill_dev, const uint8_t* active_dev,
+                        const double* solid_fill_dev, const double* void_fill_dev, const uint8_t* frozen_dev,
+                        double* f_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(n >= 0, "n must be >= 0");
+    ANNEAL_CHECK(n == 0 || (x_dev && fill_dev && active_dev && solid_fill_dev && void_fill_dev && frozen_dev && f_dev),
+                 "null argument");
+    output_field(n, x_dev, fill_dev, active_dev, solid_fill_dev, void_fill_dev, frozen_dev, f_dev,
+                 (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int64_t anneal_choose_level_work_bytes(void) { return (int64_t)choose_level_work_bytes(); }
+
+int anneal_choose_level(int64_t n, const double* f_dev, const double* x_dev, const double* fill_dev, void* work_dev,
+                        double* level_out, double* target_out, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(n >= 0 && work_dev && level_out && (n == 0 || (f_dev && x_dev && fill_dev)), "bad arguments");
+    *level_out = choose_level(n, f_dev, x_dev, fill_dev, work_dev, target_out, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_paths_sssp(int nx, int ny, int nz, const uint8_t* active_dev, const double* f_dev, double need,
+                      const uint32_t* ebits_dev, uint32_t source_groups, double* dist_dev, int32_t* pred_dev,
+                      int32_t* stamp_dev, int* flag_dev, int* sweeps_out, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(active_dev && f_dev && ebits_dev && dist_dev && pred_dev && stamp_dev && flag_dev, "null argument");
+    ANNEAL_CHECK(std::isfinite(need), "need must be finite");
+    const int bound = (int)std::min<int64_t>(g.nelem() + 2, (int64_t)1 << 30);
+    const int sw = sssp(g, active_dev, f_dev, need, ebits_dev, source_groups, dist_dev, pred_dev, stamp_dev, flag_dev,
+                        bound, (hipStream_t)stream);
+    if (sweeps_out) *sweeps_out = sw;
+    return 0;
+  });
+}
+
+int anneal_paths_trace(int nx, int ny, int nz, const uint8_t* active_dev, const uint32_t* ebits_dev, int group,
+                       const double* dist_dev, const int32_t* pred_dev, int32_t* path_dev, long long* work_dev,
+                       int64_t* start_out, int64_t* len_out, double* dstart_out, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(active_dev && ebits_dev && dist_dev && pred_dev && path_dev && work_dev && start_out && len_out,
+                 "null argument");
+    *len_out = trace_path(g, active_dev, ebits_dev, group, dist_dev, pred_dev, path_dev, work_dev, start_out,
+                          dstart_out, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_paths_raise(int nx, int ny, int nz, const int32_t* path_dev, int64_t len, const uint8_t* active_dev,
+                       const double* fill_dev, double need, double* f_dev, void* stream) {
+  return guarded([&] {
+    const GridDims g = voxel_grid(nx, ny, nz);
+    ANNEAL_CHECK(len >= 0 && len <= g.nelem(), "path length out of range");
+    ANNEAL_CHECK(len == 0 || (path_dev && active_dev && fill_dev && f_dev), "null argument");
+    raise_path(g, path_dev, len, active_dev, fill_dev, need, f_dev, (hipStream_t)stream);
+    return 0;
   });
 }
 

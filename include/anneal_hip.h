@@ -226,16 +226,92 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
 
 /* ---- several independent problems solved together (no counterpart in the
  * reference, whose FEM.solve handles one grid at a time, fem.py:206-232) ----
- * A batch groups FEM contexts of any grid sizes that share one stream.  Each
- * PCG iteration of all unconverged problems is one hipGraph launch whose
- * branches (one per problem) run concurrently; the host waits once per
- * iteration for the whole batch.  F[p], U[p]: FP64 [nrhs[p]][3][ldn_p];
+ * A batch groups FEM contexts of any grid sizes that share one stream.  One
+ * PCG iteration of a problem is a hipGraph recorded once and replayed on the
+ * problem's own stream; each batch iteration launches the graphs of the
+ * unconverged problems between a fork and a join on the shared stream, and the
+ * host waits once per iteration for the whole batch.  F[p], U[p]: FP64 [nrhs[p]][3][ldn_p];
  * status[p] as anneal_fem_solve (0, 3 not converged, 4 non-finite);
  * iters[p * 64 + r] the PCG iterations of rhs r.  Plain multigrid only. */
 int anneal_batch_create(void** batch, void* const* fems, int nprob);
 int anneal_batch_destroy(void* batch);
 int anneal_batch_solve(void* batch, const double* const* F_dev, double* const* U_dev, const int* nrhs, double rtol,
                        int maxiter, int use_x0, int* status, int* iters);
+
+/* ---- voxel passes before and after the SIMP loop (anneal/problem.py,
+ * anneal/export.py).  Element fields cover the full nx*ny*nz grid (voxel
+ * i + nx (j + ny k)); node fields (nx+1)(ny+1)(nz+1).  Masks are one byte per
+ * voxel (0/1).  work_dev arguments are 4 device int64 of scratch.  Entry
+ * points that return counts synchronise the stream. ----------------------- */
+
+/* Anchor regions as signed-distance primitives (anneal/kernel.py:40-125):
+ * kind_dev[nprim] (0 axis box: c, half; 1 framed box: o, x, y, z axes,
+ * z0 + size_z / 2, half; 2 sphere: c, r; 3 capped cylinder or pipe: a, unit
+ * axis, L, r, r_in), prm_dev[nprim][16], anchor a = primitives
+ * [first_dev[a], first_dev[a+1]) combined by minimum (Union, kernel.py:118-121),
+ * selected where that minimum is <= thr_dev[a].  at_nodes = 1: the points are
+ * the nodes of voxels with emask set at origin + h (i, j, k) (Anchor.nodes on
+ * the candidate nodes of Problem.boundary / _connected, problem.py:103-113,
+ * 276-300); at_nodes = 0: voxel centres origin + h (ijk + 0.5) with emask set
+ * (the anchor groups of export.keep_load_paths, export.py:41-45).
+ * bits_dev[point] bit a = selected; counts_dev[32] int64 per-anchor totals. */
+int anneal_region_select(int nx, int ny, int nz, double h, const double* origin_host, const int* kind_dev,
+                         const double* prm_dev, const int* first_dev, const double* thr_dev, int nanchor,
+                         int at_nodes, const uint8_t* emask_dev, uint32_t* bits_dev, long long* counts_dev,
+                         void* stream);
+/* out_dev[a][p] = the anchor's signed distance at pts_dev[p][3] (Shape.sdf). */
+int anneal_region_sdf(int64_t npts, const double* pts_dev, const int* kind_dev, const double* prm_dev,
+                      const int* first_dev, int nanchor, double* out_dev, void* stream);
+/* Problem.boundary after the anchor selection (problem.py:302-326):
+ * fixed_bits_dev[nnode] = OR of dofmask_dev[a] (bit d = dof d) over the anchors
+ * selecting the node; F_dev[ncase][3 nnode] (the reference's interleaved dof
+ * layout) = sum over anchors in order of load_dev[c][a][d] = v_d / count_a. */
+int anneal_boundary_fields(int nx, int ny, int nz, const uint32_t* node_bits_dev, int nanchor,
+                           const uint8_t* dofmask_dev, const double* load_dev, int ncase, uint8_t* fixed_bits_dev,
+                           double* F_dev, void* stream);
+/* ebits_dev[e] = OR of the bits of voxel e's 8 nodes where emask_dev[e]. */
+int anneal_gather_node_bits(int nx, int ny, int nz, const uint32_t* node_bits_dev, const uint8_t* emask_dev,
+                            uint32_t* ebits_dev, void* stream);
+/* Face-connected components (scipy.ndimage.label with its default structure,
+ * problem.py:278, export.py:49): label_dev[e] = smallest voxel index of the
+ * component (-1 outside the mask); *ncomp_out = number of components. */
+int anneal_voxel_components(int nx, int ny, int nz, const uint8_t* mask_dev, int32_t* label_dev,
+                            long long* work_dev, int64_t* ncomp_out, void* stream);
+/* Components holding a voxel with ebits & seed ("touch a fixed anchor",
+ * problem.py:283-297; fixed_labels, export.py:51): *nkept_out of them;
+ * out_mask_dev (may be null) = mask on those components; *linked_out = OR of
+ * ebits over their voxels.  keep_dev: nelem bytes of scratch. */
+int anneal_voxel_link(int nx, int ny, int nz, const uint8_t* mask_dev, const int32_t* label_dev,
+                      const uint32_t* ebits_dev, uint32_t seed, uint8_t* keep_dev, uint8_t* out_mask_dev,
+                      long long* work_dev, int64_t* nkept_out, uint32_t* linked_out, void* stream);
+/* mask_dev[e] = f_dev[e] > t;  *count_out = number of e with f_dev[e] > t. */
+int anneal_voxel_threshold(int64_t n, const double* f_dev, double t, uint8_t* mask_dev, void* stream);
+int anneal_voxel_count_above(int64_t n, const double* f_dev, double t, long long* work_dev, int64_t* count_out,
+                             void* stream);
+/* export.output_field (export.py:8-14). */
+int anneal_output_field(int64_t n, const double* x_dev, const double* fill_dev, const uint8_t* active_dev,
+                        const double* solid_fill_dev, const double* void_fill_dev, const uint8_t* frozen_dev,
+                        double* f_dev, void* stream);
+/* export.choose_level (export.py:17-28): *level_out, *target_out = sum x fill.
+ * work_dev: anneal_choose_level_work_bytes() bytes. */
+int64_t anneal_choose_level_work_bytes(void);
+int anneal_choose_level(int64_t n, const double* f_dev, const double* x_dev, const double* fill_dev, void* work_dev,
+                        double* level_out, double* target_out, void* stream);
+/* Shortest paths of export.keep_load_paths' repair stage (export.py:70-83,
+ * scipy dijkstra min_only from the fixed groups): dist_dev[nelem] FP64 (inf
+ * where unreachable or inactive), pred_dev[nelem] (-1 for sources / none).
+ * stamp_dev: nelem int32, flag_dev: one int of scratch. */
+int anneal_paths_sssp(int nx, int ny, int nz, const uint8_t* active_dev, const double* f_dev, double need,
+                      const uint32_t* ebits_dev, uint32_t source_groups, double* dist_dev, int32_t* pred_dev,
+                      int32_t* stamp_dev, int* flag_dev, int* sweeps_out, void* stream);
+/* Start voxel of group `group` (export.py:86-87) and its path back to a source
+ * (export.py:88-95) into path_dev (nelem int32); *len_out = 0 if unreachable. */
+int anneal_paths_trace(int nx, int ny, int nz, const uint8_t* active_dev, const uint32_t* ebits_dev, int group,
+                       const double* dist_dev, const int32_t* pred_dev, int32_t* path_dev, long long* work_dev,
+                       int64_t* start_out, int64_t* len_out, double* dstart_out, void* stream);
+/* Material added along a path (export.py:96-101). */
+int anneal_paths_raise(int nx, int ny, int nz, const int32_t* path_dev, int64_t len, const uint8_t* active_dev,
+                       const double* fill_dev, double need, double* f_dev, void* stream);
 
 #ifdef __cplusplus
 }
