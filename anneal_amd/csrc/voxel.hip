@@ -25,11 +25,13 @@
 // slot), and the count above any trial threshold is a suffix sum over the
 // slots.  4 passes instead of 40.
 //
-// Shortest paths: label-correcting Bellman-Ford sweeps with a 64-bit atomic
-// minimum on the distance bits (non-negative doubles order like their bit
-// patterns).  The fixed point is the same floating-point value Dijkstra
-// computes (left-to-right path sums, non-negative costs); where several
-// predecessors give the same distance the smallest voxel index is taken.
+// Shortest paths: Bellman-Ford pull sweeps (each voxel recomputes its own
+// distance from its neighbours when one of them changed in the last two
+// sweeps, then each 64-voxel row relaxes along x in registers; no atomics on
+// the distances).  The fixed point is the same
+// floating-point value Dijkstra computes (left-to-right path sums,
+// non-negative costs); where several predecessors give the same distance the
+// smallest voxel index is taken.
 #include "voxel.hpp"
 
 #include <algorithm>
@@ -43,6 +45,23 @@ namespace {
 
 constexpr int kThreads = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+
+// Grid-shaped launches (64 x blockDim.y threads per block, one z plane per
+// block layer): the kernel gets (i, j, k) without integer division.
+const dim3 kBlock3(64, 4, 1);
+inline dim3 grid3(int ex, int ey, int ez) { return dim3((ex + 63) / 64, (ey + 3) / 4, ez); }
+struct Idx3 {
+  int i, j, k;
+  bool in;
+};
+__device__ __forceinline__ Idx3 idx3(int ex, int ey, int ez) {
+  Idx3 r;
+  r.i = int(blockIdx.x) * 64 + int(threadIdx.x);
+  r.j = int(blockIdx.y * blockDim.y + threadIdx.y);
+  r.k = int(blockIdx.z);
+  r.in = r.i < ex && r.j < ey && r.k < ez;
+  return r;
+}
 
 __device__ __forceinline__ double sdf_box(double qx, double qy, double qz, double hx, double hy, double hz) {
 #pragma clang fp contract(off)
@@ -87,22 +106,32 @@ __device__ __forceinline__ double anchor_sdf(const RegionTable& t, int a, double
   return s;
 }
 
-__device__ __forceinline__ void wave_count(bool pred, long long* dst) {
+// Block-wide count of pred added to *dst with one global atomic per block
+// (every thread of the block must call it; same-address global atomics are
+// serialised in L2, so per-wave atomics would dominate these kernels).
+__device__ __forceinline__ void block_count(bool pred, long long* dst) {
+  __shared__ unsigned long long s_cnt;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
   const unsigned long long m = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd((unsigned long long*)dst, (unsigned long long)__popcll(m));
+  if ((tid & 63) == 0 && m) atomicAdd(&s_cnt, (unsigned long long)__popcll(m));
+  __syncthreads();
+  if (tid == 0 && s_cnt) atomicAdd((unsigned long long*)dst, s_cnt);
 }
 
 template <bool NODES>
-__global__ void __launch_bounds__(kThreads) k_region_select(int nx, int ny, int nz, double h, double ox, double oy,
+__global__ void __launch_bounds__(1024) k_region_select(int nx, int ny, int nz, double h, double ox, double oy,
                                                             double oz, RegionTable t,
                                                             const uint8_t* __restrict__ emask,
                                                             uint32_t* __restrict__ bits, long long* counts) {
 #pragma clang fp contract(off)
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t ex = NODES ? nx + 1 : nx, ey = NODES ? ny + 1 : ny, ez = NODES ? nz + 1 : nz;
+  const int ex = NODES ? nx + 1 : nx, ey = NODES ? ny + 1 : ny, ez = NODES ? nz + 1 : nz;
+  const Idx3 q = idx3(ex, ey, ez);
   uint32_t b = 0;
-  if (p < ex * ey * ez) {
-    const int i = int(p % ex), j = int((p / ex) % ey), k = int(p / (ex * ey));
+  if (q.in) {
+    const int i = q.i, j = q.j, k = q.k;
+    const int64_t p = i + int64_t(ex) * (j + int64_t(ey) * k);
     bool on = false;
     double x, y, z;
     if (NODES) {
@@ -127,7 +156,16 @@ __global__ void __launch_bounds__(kThreads) k_region_select(int nx, int ny, int 
         if (anchor_sdf(t, a, x, y, z) <= t.thr[a]) b |= 1u << a;
     bits[p] = b;
   }
-  for (int a = 0; a < t.nanchor; ++a) wave_count((b >> a) & 1u, counts + a);
+  __shared__ unsigned long long s_cnt[kMaxAnchors];
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid < kMaxAnchors) s_cnt[tid] = 0;
+  __syncthreads();
+  for (int a = 0; a < t.nanchor; ++a) {
+    const unsigned long long m = __ballot((b >> a) & 1u);
+    if ((tid & 63) == 0 && m) atomicAdd(&s_cnt[a], (unsigned long long)__popcll(m));
+  }
+  __syncthreads();
+  if (tid < t.nanchor && s_cnt[tid]) atomicAdd((unsigned long long*)(counts + tid), s_cnt[tid]);
 }
 
 __global__ void __launch_bounds__(kThreads) k_region_sdf(int64_t n, const double* __restrict__ pts, RegionTable t,
@@ -164,12 +202,12 @@ __global__ void __launch_bounds__(kThreads) k_gather_node_bits(int nx, int ny, i
                                                                const uint32_t* __restrict__ nb,
                                                                const uint8_t* __restrict__ emask,
                                                                uint32_t* __restrict__ eb) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t ne = int64_t(nx) * ny * nz;
-  if (e >= ne) return;
+  const Idx3 q = idx3(nx, ny, nz);
+  if (!q.in) return;
+  const int64_t i = q.i, j = q.j, k = q.k;
+  const int64_t e = i + int64_t(nx) * (j + int64_t(ny) * k);
   uint32_t b = 0;
   if (emask[e]) {
-    const int64_t i = e % nx, j = (e / nx) % ny, k = e / (int64_t(nx) * ny);
     const int64_t sx = nx + 1, sxy = int64_t(nx + 1) * (ny + 1);
     const int64_t n0 = i + sx * j + sxy * k;
     b = nb[n0] | nb[n0 + 1] | nb[n0 + sx] | nb[n0 + sx + 1] | nb[n0 + sxy] | nb[n0 + sxy + 1] | nb[n0 + sxy + sx] |
@@ -200,8 +238,7 @@ __device__ int find_root(int32_t* par, int x) {
   return cur;
 }
 
-// Root of x without writes (the labelling pass: there a splitting write could
-// land after the owner stored the final root and replace it by an ancestor).
+// Root of x without writes (the labelling pass writes only final roots).
 __device__ int find_root_ro(const int32_t* par, int x) {
   int p = ld_par(par, x);
   while (p != x) {
@@ -236,24 +273,35 @@ __global__ void __launch_bounds__(kThreads) k_cc_init(int64_t ne, const uint8_t*
 
 __global__ void __launch_bounds__(kThreads) k_cc_merge(int nx, int ny, int nz, const uint8_t* __restrict__ mask,
                                                        int32_t* par, int* err) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t sxy = int64_t(nx) * ny, ne = sxy * nz;
-  if (e >= ne || !mask[e]) return;
-  const int64_t i = e % nx, j = (e / nx) % ny, k = e / sxy;
+  const Idx3 q = idx3(nx, ny, nz);
+  if (!q.in) return;
+  const int64_t i = q.i, j = q.j, k = q.k, sxy = int64_t(nx) * ny;
+  const int64_t e = i + nx * j + sxy * k;
+  if (!mask[e]) return;
   if (i + 1 < nx && mask[e + 1]) unite(par, int(e), int(e + 1), err);
   if (j + 1 < ny && mask[e + nx]) unite(par, int(e), int(e + nx), err);
   if (k + 1 < nz && mask[e + sxy]) unite(par, int(e), int(e + sxy), err);
 }
 
-__global__ void __launch_bounds__(kThreads) k_cc_compress(int64_t ne, int32_t* par, long long* nroot) {
+// Labelling pass (after the merge pass, so every root is final): each voxel
+// finds its root without writes, then points every node of its path at that
+// root.  Only final roots are ever written, so concurrent walks stay valid and
+// no label is replaced by a non-root.
+__global__ void __launch_bounds__(1024) k_cc_compress(int64_t ne, int32_t* par, long long* nroot) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool root = false;
   if (e < ne && ld_par(par, int(e)) >= 0) {
     const int r = find_root_ro(par, int(e));
+    int cur = int(e);
+    while (cur != r) {
+      const int nxt = ld_par(par, cur);
+      if (nxt != r) st_par(par, cur, r);
+      cur = nxt;
+    }
     st_par(par, int(e), r);
     root = r == int(e);
   }
-  wave_count(root, nroot);
+  block_count(root, nroot);
 }
 
 __global__ void k_zero_ll(long long* p, int n) {
@@ -278,7 +326,7 @@ __global__ void __launch_bounds__(kThreads) k_link_mark(int64_t ne, const uint8_
 }
 
 // work[0]: marked components, work[1]: OR of ebits over their voxels.
-__global__ void __launch_bounds__(kThreads) k_link_apply(int64_t ne, const uint8_t* __restrict__ mask,
+__global__ void __launch_bounds__(1024) k_link_apply(int64_t ne, const uint8_t* __restrict__ mask,
                                                          const int32_t* __restrict__ lab,
                                                          const uint32_t* __restrict__ eb,
                                                          const uint8_t* __restrict__ keep, uint8_t* out_mask,
@@ -291,9 +339,14 @@ __global__ void __launch_bounds__(kThreads) k_link_apply(int64_t ne, const uint8
     if (out_mask) out_mask[e] = m ? 1 : 0;
     if (m) b = eb[e];
   }
-  wave_count(m && lab[e] == int32_t(e), work);
+  block_count(m && lab[e] == int32_t(e), work);
   for (int off = 32; off > 0; off >>= 1) b |= __shfl_xor(b, off);
-  if ((threadIdx.x & 63) == 0 && b) atomicOr((unsigned long long*)(work + 1), (unsigned long long)b);
+  __shared__ unsigned s_or;
+  if (threadIdx.x == 0) s_or = 0;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0 && b) atomicOr(&s_or, b);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_or) atomicOr((unsigned long long*)(work + 1), (unsigned long long)s_or);
 }
 
 __global__ void __launch_bounds__(kThreads) k_threshold(int64_t n, const double* __restrict__ f, double t,
@@ -302,10 +355,10 @@ __global__ void __launch_bounds__(kThreads) k_threshold(int64_t n, const double*
   if (e < n) mask[e] = f[e] > t ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kThreads) k_count_above(int64_t n, const double* __restrict__ f, double t,
+__global__ void __launch_bounds__(1024) k_count_above(int64_t n, const double* __restrict__ f, double t,
                                                           long long* cnt) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  wave_count(e < n && f[e] > t, cnt);
+  block_count(e < n && f[e] > t, cnt);
 }
 
 __global__ void __launch_bounds__(kThreads) k_output_field(int64_t n, const double* __restrict__ x,
@@ -335,12 +388,16 @@ struct LevelState {
   double lo, hi, target, pad;
 };
 
-// hist[kLevelLeaves] is the "over" slot: values above the current interval,
-// which exceed every trial threshold of the pass.
+// hist[r][kLevelLeaves] is the "over" slot: values above the current
+// interval, which exceed every trial threshold of the pass.  kHistCopies
+// copies of the histogram (block b adds into copy b % kHistCopies) spread the
+// global atomics over more cache lines.
+constexpr int kHistCopies = 64;
+constexpr int kHistStride = kLevelLeaves + 64;
 struct LevelWork {
   LevelState st;
   double part[kSumBlocks];
-  unsigned hist[kLevelLeaves + 1];
+  unsigned hist[kHistCopies][kHistStride];
 };
 
 // Deterministic FP64 sum of x*fill: fixed block partials, then one block.
@@ -374,7 +431,7 @@ __global__ void __launch_bounds__(kThreads) k_level_init(LevelWork* w) {
     w->st.lo = 0.05;
     w->st.hi = 0.95;
   }
-  for (int b = threadIdx.x; b <= kLevelLeaves; b += kThreads) w->hist[b] = 0;
+  for (int b = threadIdx.x; b < kHistCopies * kHistStride; b += kThreads) (&w->hist[0][0])[b] = 0;
 }
 
 // Trial threshold of heap node k (root 1) at depth d below the interval (lo, hi):
@@ -390,7 +447,8 @@ __device__ __forceinline__ double tree_threshold(double lo, double hi, int k, in
 }
 
 // Values <= lo exceed no trial threshold of the pass and are skipped; values
-// > hi exceed all of them (over slot); the rest walk the tree to their slot.
+// > hi exceed all of them (counted per thread, then per block); the rest walk
+// the tree to their slot.
 __global__ void __launch_bounds__(kThreads) k_level_hist(int64_t n, const double* __restrict__ f, LevelWork* w) {
   __shared__ double thr[kLevelLeaves];
   __shared__ unsigned hist[kLevelLeaves + 1];
@@ -398,21 +456,25 @@ __global__ void __launch_bounds__(kThreads) k_level_hist(int64_t n, const double
   for (int k = 1 + threadIdx.x; k < kLevelLeaves; k += kThreads) thr[k] = tree_threshold(lo, hi, k, 31 - __clz(k));
   for (int b = threadIdx.x; b <= kLevelLeaves; b += kThreads) hist[b] = 0;
   __syncthreads();
+  unsigned over = 0;
   for (int64_t e = int64_t(blockIdx.x) * kThreads + threadIdx.x; e < n; e += int64_t(gridDim.x) * kThreads) {
     const double v = f[e];
     if (!(v > lo)) continue;
-    int slot = kLevelLeaves;
-    if (v <= hi) {
-      int k = 1;
-#pragma unroll
-      for (int l = 0; l < kLevelDepth; ++l) k = 2 * k + (v > thr[k] ? 1 : 0);
-      slot = k - kLevelLeaves;
+    if (v > hi) {
+      ++over;
+      continue;
     }
-    atomicAdd(&hist[slot], 1u);
+    int k = 1;
+#pragma unroll
+    for (int l = 0; l < kLevelDepth; ++l) k = 2 * k + (v > thr[k] ? 1 : 0);
+    atomicAdd(&hist[k - kLevelLeaves], 1u);
   }
+  for (int off = 32; off > 0; off >>= 1) over += __shfl_xor(over, off);
+  if ((threadIdx.x & 63) == 0 && over) atomicAdd(&hist[kLevelLeaves], over);
   __syncthreads();
+  unsigned* dst = w->hist[blockIdx.x % kHistCopies];
   for (int b = threadIdx.x; b <= kLevelLeaves; b += kThreads)
-    if (hist[b]) atomicAdd(&w->hist[b], hist[b]);
+    if (hist[b]) atomicAdd(&dst[b], hist[b]);
 }
 
 // Replays kLevelDepth decisions from the histogram (count above the trial of
@@ -420,11 +482,26 @@ __global__ void __launch_bounds__(kThreads) k_level_hist(int64_t n, const double
 // and clears the histogram for the next pass.
 __global__ void __launch_bounds__(kLevelLeaves) k_level_walk(LevelWork* w) {
 #pragma clang fp contract(off)
-  __shared__ unsigned long long suf[kLevelLeaves + 2];  // suf[s] = count in slots >= s
+  // suffix sums suf[s] = count in slots >= s (s <= kLevelLeaves), by doubling steps in LDS
+  __shared__ unsigned long long buf[2][kLevelLeaves + 1];
   const int t = threadIdx.x;
+  unsigned long long h = 0, ho = 0;
+  for (int r = 0; r < kHistCopies; ++r) {
+    h += w->hist[r][t];
+    if (t == 0) ho += w->hist[r][kLevelLeaves];
+  }
+  buf[0][t] = h;
+  if (t == 0) buf[0][kLevelLeaves] = ho;
+  __syncthreads();
+  int cur = 0;
+  for (int off = 1; off <= kLevelLeaves; off <<= 1) {
+    buf[cur ^ 1][t] = buf[cur][t] + (t + off <= kLevelLeaves ? buf[cur][t + off] : 0ull);
+    if (t == 0) buf[cur ^ 1][kLevelLeaves] = buf[cur][kLevelLeaves];
+    __syncthreads();
+    cur ^= 1;
+  }
+  const unsigned long long* suf = buf[cur];
   if (t == 0) {
-    suf[kLevelLeaves + 1] = 0;
-    for (int s = kLevelLeaves; s >= 0; --s) suf[s] = suf[s + 1] + w->hist[s];
     double lo = w->st.lo, hi = w->st.hi;
     const double target = w->st.target;
     int k = 1;
@@ -443,9 +520,10 @@ __global__ void __launch_bounds__(kLevelLeaves) k_level_walk(LevelWork* w) {
     w->st.lo = lo;
     w->st.hi = hi;
   }
-  __syncthreads();
-  w->hist[t] = 0;
-  if (t == 0) w->hist[kLevelLeaves] = 0;
+  for (int r = 0; r < kHistCopies; ++r) {
+    w->hist[r][t] = 0;
+    if (t == 0) w->hist[r][kLevelLeaves] = 0;
+  }
 }
 
 // ---- shortest paths -------------------------------------------------------
@@ -468,40 +546,74 @@ __global__ void __launch_bounds__(kThreads) k_sssp_init(int64_t ne, const uint8_
   const bool s = act[e] && (eb[e] & src);
   dist[e] = s ? 0.0 : INFINITY;
   pred[e] = -1;
-  stamp[e] = s ? 0 : -1;
+  stamp[e] = s ? 0 : -2;
 }
 
+// One pull sweep: an active voxel whose neighbours changed in this sweep or
+// the previous one takes min(dist, dist[v] + cost(u)) over its active
+// neighbours v; then each wave (64 consecutive voxels along x) relaxes its
+// row in register, forward and backward, one edge at a time, so a distance
+// crosses the whole row within the sweep.  Every update is one exact
+// relaxation d[u] = min(d[u], d[v] + cost(u)), so the fixed point is the
+// same.  Only the owner writes dist[u] (values only decrease; a neighbour
+// read mid-update is an older, larger distance and the voxel is revisited
+// next sweep through its stamp).
 __global__ void __launch_bounds__(kThreads) k_sssp_sweep(int nx, int ny, int nz, const uint8_t* __restrict__ act,
                                                          const double* __restrict__ f, double need, double* dist,
                                                          int32_t* stamp, int* flag, int sweep) {
 #pragma clang fp contract(off)
-  const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t sxy = int64_t(nx) * ny, ne = sxy * nz;
-  if (v >= ne) return;
-  if (__hip_atomic_load(stamp + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != sweep) return;
-  const double dv = ld_dist(dist, v);
-  const int64_t i = v % nx, j = (v / nx) % ny, k = v / sxy;
+  const Idx3 q3 = idx3(nx, ny, nz);
+  const int i = q3.i, j = q3.j, k = q3.k;
+  const int lane = threadIdx.x;  // blockDim.x == 64: one wave per row segment
+  const int64_t sxy = int64_t(nx) * ny;
+  const int64_t u = q3.in ? i + int64_t(nx) * j + sxy * k : 0;
+  const bool a = q3.in && act[u];
   int64_t nb[6];
   int cnt = 0;
-  if (i > 0) nb[cnt++] = v - 1;
-  if (i + 1 < nx) nb[cnt++] = v + 1;
-  if (j > 0) nb[cnt++] = v - nx;
-  if (j + 1 < ny) nb[cnt++] = v + nx;
-  if (k > 0) nb[cnt++] = v - sxy;
-  if (k + 1 < nz) nb[cnt++] = v + sxy;
-  bool any = false;
-  for (int q = 0; q < cnt; ++q) {
-    const int64_t u = nb[q];
-    if (!act[u]) continue;
-    const double nd = dv + path_cost(f, u, need);
-    const unsigned long long nbits = (unsigned long long)__double_as_longlong(nd);
-    const unsigned long long old = atomicMin((unsigned long long*)(dist + u), nbits);
-    if (nbits < old) {
-      __hip_atomic_store(stamp + u, sweep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      any = true;
-    }
+  bool fresh = false;
+  if (a) {
+    if (i > 0) nb[cnt++] = u - 1;
+    if (i + 1 < nx) nb[cnt++] = u + 1;
+    if (j > 0) nb[cnt++] = u - nx;
+    if (j + 1 < ny) nb[cnt++] = u + nx;
+    if (k > 0) nb[cnt++] = u - sxy;
+    if (k + 1 < nz) nb[cnt++] = u + sxy;
+    for (int q = 0; q < cnt; ++q)
+      fresh = fresh || __hip_atomic_load(stamp + nb[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sweep - 1;
   }
-  if (any) atomicMax(flag, sweep + 1);
+  if (__ballot(fresh) == 0) return;  // wave-uniform
+  const double cu = a ? path_cost(f, u, need) : 0.0;
+  const double du = a ? ld_dist(dist, u) : INFINITY;
+  double best = du;
+  if (fresh)
+    for (int q = 0; q < cnt; ++q) {
+      const int64_t v = nb[q];
+      if (!act[v]) continue;
+      const double c = ld_dist(dist, v) + cu;
+      if (c < best) best = c;
+    }
+  // in-row relaxation; inactive lanes hold +inf and never change
+  for (int it = 0; it < 63; ++it) {
+    const double from = __shfl_up(best, 1);
+    const double c = from + cu;
+    const bool imp = a && lane > 0 && c < best;
+    if (imp) best = c;
+    if (__ballot(imp) == 0) break;
+  }
+  for (int it = 0; it < 63; ++it) {
+    const double from = __shfl_down(best, 1);
+    const double c = from + cu;
+    const bool imp = a && lane < 63 && c < best;
+    if (imp) best = c;
+    if (__ballot(imp) == 0) break;
+  }
+  if (a && best < du) {
+    __hip_atomic_store((unsigned long long*)(dist + u), (unsigned long long)__double_as_longlong(best),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(stamp + u, sweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // one atomic per sweep and L2 slice in practice: later changers see the flag already set
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sweep + 1) atomicMax(flag, sweep + 1);
+  }
 }
 
 __global__ void __launch_bounds__(kThreads) k_sssp_pred(int nx, int ny, int nz, const uint8_t* __restrict__ act,
@@ -510,14 +622,14 @@ __global__ void __launch_bounds__(kThreads) k_sssp_pred(int nx, int ny, int nz, 
                                                         const double* __restrict__ dist,
                                                         int32_t* __restrict__ pred) {
 #pragma clang fp contract(off)
-  const int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t sxy = int64_t(nx) * ny, ne = sxy * nz;
-  if (u >= ne) return;
+  const Idx3 q3 = idx3(nx, ny, nz);
+  if (!q3.in) return;
+  const int64_t i = q3.i, j = q3.j, k = q3.k, sxy = int64_t(nx) * ny;
+  const int64_t u = i + nx * j + sxy * k;
   int32_t p = -1;
   const double du = dist[u];
   if (act[u] && !(eb[u] & src) && du < INFINITY) {
     const double cu = path_cost(f, u, need);
-    const int64_t i = u % nx, j = (u / nx) % ny, k = u / sxy;
     // neighbours in increasing index order: the first match is the smallest
     const int64_t cand[6] = {k > 0 ? u - sxy : -1,     j > 0 ? u - nx : -1,      i > 0 ? u - 1 : -1,
                              i + 1 < nx ? u + 1 : -1, j + 1 < ny ? u + nx : -1, k + 1 < nz ? u + sxy : -1};
@@ -538,8 +650,13 @@ __global__ void __launch_bounds__(kThreads) k_group_min(int64_t ne, const uint8_
                                                         const uint32_t* __restrict__ eb, uint32_t gbit,
                                                         const double* __restrict__ dist, long long* work) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < ne && act[e] && (eb[e] & gbit))
-    atomicMin((unsigned long long*)work, (unsigned long long)__double_as_longlong(dist[e]));
+  unsigned long long d = ~0ull;
+  if (e < ne && act[e] && (eb[e] & gbit)) d = (unsigned long long)__double_as_longlong(dist[e]);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(d, off);
+    d = o < d ? o : d;
+  }
+  if ((threadIdx.x & 63) == 0 && d != ~0ull) atomicMin((unsigned long long*)work, d);
 }
 
 __global__ void __launch_bounds__(kThreads) k_group_argmin(int64_t ne, const uint8_t* __restrict__ act,
@@ -594,13 +711,14 @@ __global__ void __launch_bounds__(kThreads) k_raise_path(const int32_t* __restri
 void region_select(const GridDims& g, double h, const double* origin, const RegionTable& t, bool at_nodes,
                    const uint8_t* emask, uint32_t* bits, long long* counts, hipStream_t s) {
   ANNEAL_CHECK(t.nanchor >= 0 && t.nanchor <= kMaxAnchors, "at most 32 anchors");
-  const int64_t np = at_nodes ? g.nnode() : g.nelem();
   k_zero_ll<<<1, 64, 0, s>>>(counts, kMaxAnchors);
+  // 64 x 16 threads per block: one global atomic per block and anchor
+  const dim3 blk(64, 16, 1);
   if (at_nodes)
-    k_region_select<true><<<nblk(np), kThreads, 0, s>>>(g.nx, g.ny, g.nz, h, origin[0], origin[1], origin[2], t, emask,
+    k_region_select<true><<<dim3((g.nx + 64) / 64, (g.ny + 16) / 16, g.nz + 1), blk, 0, s>>>(g.nx, g.ny, g.nz, h, origin[0], origin[1], origin[2], t, emask,
                                                        bits, counts);
   else
-    k_region_select<false><<<nblk(np), kThreads, 0, s>>>(g.nx, g.ny, g.nz, h, origin[0], origin[1], origin[2], t,
+    k_region_select<false><<<dim3((g.nx + 63) / 64, (g.ny + 15) / 16, g.nz), blk, 0, s>>>(g.nx, g.ny, g.nz, h, origin[0], origin[1], origin[2], t,
                                                         emask, bits, counts);
   ANNEAL_HIP(hipGetLastError());
 }
@@ -620,7 +738,7 @@ void boundary_fields(const GridDims& g, const uint32_t* node_bits, int nanchor, 
 
 void gather_node_bits(const GridDims& g, const uint32_t* node_bits, const uint8_t* emask, uint32_t* ebits,
                       hipStream_t s) {
-  k_gather_node_bits<<<nblk(g.nelem()), kThreads, 0, s>>>(g.nx, g.ny, g.nz, node_bits, emask, ebits);
+  k_gather_node_bits<<<grid3(g.nx, g.ny, g.nz), kBlock3, 0, s>>>(g.nx, g.ny, g.nz, node_bits, emask, ebits);
   ANNEAL_HIP(hipGetLastError());
 }
 
@@ -629,8 +747,8 @@ int64_t components(const GridDims& g, const uint8_t* mask, int32_t* label, long 
   const int64_t ne = g.nelem();
   k_zero_ll<<<1, 64, 0, s>>>(counter, 2);
   k_cc_init<<<nblk(ne), kThreads, 0, s>>>(ne, mask, label);
-  k_cc_merge<<<nblk(ne), kThreads, 0, s>>>(g.nx, g.ny, g.nz, mask, label, (int*)(counter + 1));
-  k_cc_compress<<<nblk(ne), kThreads, 0, s>>>(ne, label, counter);
+  k_cc_merge<<<grid3(g.nx, g.ny, g.nz), kBlock3, 0, s>>>(g.nx, g.ny, g.nz, mask, label, (int*)(counter + 1));
+  k_cc_compress<<<unsigned((ne + 1023) / 1024), 1024, 0, s>>>(ne, label, counter);
   ANNEAL_HIP(hipGetLastError());
   long long host[2] = {0, 0};
   ANNEAL_HIP(hipMemcpyAsync(host, counter, 2 * sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -646,7 +764,7 @@ int64_t link_components(const GridDims& g, const uint8_t* mask, const int32_t* l
   k_zero_ll<<<1, 64, 0, s>>>(counter, 2);
   k_fill_u8<<<nblk(ne), kThreads, 0, s>>>(ne, keep, 0);
   k_link_mark<<<nblk(ne), kThreads, 0, s>>>(ne, mask, label, ebits, seed, keep);
-  k_link_apply<<<nblk(ne), kThreads, 0, s>>>(ne, mask, label, ebits, keep, out_mask, counter);
+  k_link_apply<<<unsigned((ne + 1023) / 1024), 1024, 0, s>>>(ne, mask, label, ebits, keep, out_mask, counter);
   ANNEAL_HIP(hipGetLastError());
   long long host[2] = {0, 0};
   ANNEAL_HIP(hipMemcpyAsync(host, counter, 2 * sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -663,7 +781,7 @@ void threshold_mask(int64_t n, const double* f, double t, uint8_t* mask, hipStre
 
 int64_t count_above(int64_t n, const double* f, double t, long long* counter, hipStream_t s) {
   k_zero_ll<<<1, 64, 0, s>>>(counter, 1);
-  if (n > 0) k_count_above<<<nblk(n), kThreads, 0, s>>>(n, f, t, counter);
+  if (n > 0) k_count_above<<<unsigned((n + 1023) / 1024), 1024, 0, s>>>(n, f, t, counter);
   ANNEAL_HIP(hipGetLastError());
   long long host = 0;
   ANNEAL_HIP(hipMemcpyAsync(&host, counter, sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -685,7 +803,7 @@ double choose_level(int64_t n, const double* f, const double* x, const double* f
   LevelWork* w = (LevelWork*)work;
   // every one of the kSumBlocks partials is written (blocks past the data write 0)
   k_level_sum<<<kSumBlocks, kThreads, 0, s>>>(n, x, fill, w);
-  const unsigned hb = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
+  const unsigned hb = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
   k_level_init<<<1, kThreads, 0, s>>>(w);
   for (int p = 0; p < kLevelPasses; ++p) {
     k_level_hist<<<hb, kThreads, 0, s>>>(n, f, w);
@@ -714,7 +832,7 @@ int sssp(const GridDims& g, const uint8_t* active, const double* f, double need,
   while (!done && sweep < max_sweeps) {
     const int upto = std::min(sweep + kBatch, max_sweeps);
     for (; sweep < upto; ++sweep)
-      k_sssp_sweep<<<nblk(ne), kThreads, 0, s>>>(g.nx, g.ny, g.nz, active, f, need, dist, stamp, flag, sweep);
+      k_sssp_sweep<<<grid3(g.nx, g.ny, g.nz), kBlock3, 0, s>>>(g.nx, g.ny, g.nz, active, f, need, dist, stamp, flag, sweep);
     ANNEAL_HIP(hipGetLastError());
     int last = 0;
     ANNEAL_HIP(hipMemcpyAsync(&last, flag, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -722,7 +840,7 @@ int sssp(const GridDims& g, const uint8_t* active, const double* f, double need,
     done = last < sweep;  // the last sweep improved nothing
   }
   if (!done) throw std::runtime_error("shortest paths did not settle within the sweep bound");
-  k_sssp_pred<<<nblk(ne), kThreads, 0, s>>>(g.nx, g.ny, g.nz, active, f, need, ebits, source_groups, dist, pred);
+  k_sssp_pred<<<grid3(g.nx, g.ny, g.nz), kBlock3, 0, s>>>(g.nx, g.ny, g.nz, active, f, need, ebits, source_groups, dist, pred);
   ANNEAL_HIP(hipGetLastError());
   return sweep;
 }

@@ -271,6 +271,124 @@ def unet_probe(reps=20):
     return out
 
 
+class _AxisBox:
+    """Axis-aligned anchor region with the attributes the device passes read (kernel.py:56-65)."""
+
+    def __init__(self, lo, hi):
+        import numpy as np
+        self.lo, self.hi, self.frame = np.asarray(lo, float), np.asarray(hi, float), None
+
+    def bounds(self):
+        return self.lo, self.hi
+
+
+def voxel_problem(shape=SHAPE, seed=0):
+    """A 192x96x96 part for the voxel passes: wall (fixed), tip and top (load) anchors, 3 % voids."""
+    from types import SimpleNamespace
+    import numpy as np
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    grid = SimpleNamespace(shape=shape, nx=nx, ny=ny, nz=nz, h=1.0, origin=np.zeros(3), nelem=nx * ny * nz,
+                           nnode=(nx + 1) * (ny + 1) * (nz + 1))
+    anchors = [SimpleNamespace(name="wall", role="fixed", dofs=[0, 1, 2], loads={},
+                               region=_AxisBox([-1, -1, -1], [0.5, ny + 1, nz + 1])),
+               SimpleNamespace(name="tip", role="load", dofs=[0, 1, 2], loads={"down": np.array([0, 0, -1.0])},
+                               region=_AxisBox([nx - 0.5, -1, -1], [nx + 1, ny + 1, nz + 1])),
+               SimpleNamespace(name="top", role="load", dofs=[0, 1, 2], loads={"push": np.array([0, -1.0, 0])},
+                               region=_AxisBox([nx / 2 - 2, ny - 0.5, nz / 2 - 2], [nx / 2 + 2, ny + 1, nz / 2 + 2]))]
+    n = grid.nelem
+    active = rng.random(n) >= 0.03
+    prob = SimpleNamespace(grid=grid, anchors=anchors, cases=["down", "push"], fill=np.where(active, 1.0, 0.0),
+                           active=active, frozen=np.zeros(n, bool), solid_fill=np.zeros(n), void_fill=np.zeros(n),
+                           log=lambda *a: None)
+    fields = {"x": rng.random(n), "f_low": 0.3 * rng.random(n), "mask": rng.random(n) < 0.3116}
+    return prob, fields
+
+
+def _timed(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps * 1e3, r
+
+
+def voxel_cpu_baseline(prob, fields):
+    """The oracle (numpy / scipy restatement of problem.py and export.py) on the same inputs, one core."""
+    from scipy import ndimage
+    from oracle import voxel_ref as V
+    from oracle.fem_ref import Grid as GridRef
+    g = GridRef(prob.grid.shape, prob.grid.h, prob.grid.origin)
+    anchors = [V.Anchor(a.name, a.role, V.Box(a.region.lo, a.region.hi), loads=a.loads) for a in prob.anchors]
+    vp = V.Prob(g, anchors, prob.fill, active=prob.active)
+    out = {}
+    t = time.perf_counter()
+    V.boundary(vp, vp.active)
+    out["boundary"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    ndimage.label(g.to3d(fields["mask"]))
+    out["components"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    V.connected(vp, fields["mask"])
+    out["connected"] = (time.perf_counter() - t) * 1e3
+    f = V.output_field(vp, fields["x"])
+    t = time.perf_counter()
+    V.choose_level(vp, f, fields["x"])
+    out["choose_level"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    V.keep_load_paths(vp, fields["f_low"], 0.5, log=lambda *a: None)
+    out["keep_load_paths"] = (time.perf_counter() - t) * 1e3
+    return out
+
+
+def voxel_probe(reps=5, cpu=True):
+    """Device timings of the voxel passes around the SIMP loop at 192x96x96 (wall clock: every pass ends in a
+    host read of its counts), with the oracle's CPU time on the same inputs."""
+    import numpy as np
+    import torch
+    from anneal_amd import problem as P, export as X
+    prob, fields = voxel_problem()
+    n = prob.grid.nelem
+    dev = torch.device("cuda", 0)
+    from types import SimpleNamespace
+    host_prob = prob
+    # device-resident copies of the part's fields: the timings exclude host-to-device copies
+    prob = SimpleNamespace(**vars(host_prob))
+    for k in ("fill", "solid_fill", "void_fill"):
+        setattr(prob, k, torch.from_numpy(getattr(host_prob, k)).to(dev))
+    for k in ("active", "frozen"):
+        setattr(prob, k, torch.from_numpy(getattr(host_prob, k).astype(np.uint8)).to(dev))
+    mask_d = torch.from_numpy(fields["mask"].astype(np.uint8)).to(dev)
+    x_d = torch.from_numpy(fields["x"]).to(dev)
+    f_d = X.output_field(prob, x_d)
+    res = {}
+    res["boundary"], _ = _timed(lambda: P.boundary_device(prob, prob.active), reps)
+    res["components"], (lab, ncomp) = _timed(lambda: P.components(prob.grid, mask_d), reps)
+    res["connected"], _ = _timed(lambda: P.connected(prob, mask_d), reps)
+    res["output_field"], _ = _timed(lambda: X.output_field(prob, x_d), reps)
+    res["choose_level"], level = _timed(lambda: X.choose_level(prob, f_d, x_d), reps)
+    f_low = torch.from_numpy(fields["f_low"]).to(dev)
+    res["keep_load_paths"], _ = _timed(lambda: X.keep_load_paths(prob, f_low, 0.5, log=lambda *a: None), 1)
+    _, _, sweeps, _ = X.path_arrays(prob, f_low, 0.44)
+    # algorithmic bytes per voxel: choose_level reads x, fill (sum) then f in 4 passes; output_field 6 inputs + 1 out
+    gbs = {"choose_level": 48.0 * n / (res["choose_level"] * 1e-3) / 1e9,
+           "output_field": 42.0 * n / (res["output_field"] * 1e-3) / 1e9,
+           "components": 5.0 * n / (res["components"] * 1e-3) / 1e9}
+    out = {"probe": "voxel passes", "grid": list(prob.grid.shape), "components": ncomp, "level": level,
+           "sssp_sweeps": sweeps, "device_ms": {k: round(v, 3) for k, v in res.items()},
+           "device_GBps": {k: round(v, 1) for k, v in gbs.items()}}
+    if cpu:
+        c = voxel_cpu_baseline(host_prob, fields)
+        out["cpu_ms"] = {k: round(v, 1) for k, v in c.items()}
+        out["cpu_cores"] = 1
+    for k, v in res.items():
+        print(f"probe: {k} {v:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE counters, copy-calibrated) from the committed
     counter summary profiles/apply_192x96x96_traffic.json, or None."""
@@ -387,10 +505,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
-    ap.add_argument("--probe", choices=["apply-traffic", "unet"], default=None,
+    ap.add_argument("--probe", choices=["apply-traffic", "unet", "voxel"], default=None,
                     help="apply-traffic: FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
                          "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); unet: U-Net forward and "
-                         "forward+backward timings at the learned stage's grid sizes (prints one JSON line)")
+                         "forward+backward timings at the learned stage's grid sizes; voxel: the boundary, "
+                         "connectivity, iso-level and load-path passes at 192x96x96 next to the oracle's CPU time "
+                         "(unet and voxel print one JSON line)")
     ap.add_argument("--workload", choices=["cantilever", "large", "batch", "cantilever128"], default="cantilever",
                     help="cantilever: 192x96x96, 100 iterations (the headline); cantilever128: the same at "
                          "128x64x64 (the learned-preconditioner size); large: 384x192x192 with keep-outs, "
@@ -409,6 +529,9 @@ def main():
         return
     if args.probe == "unet":
         print(json.dumps(unet_probe()), flush=True)
+        return
+    if args.probe == "voxel":
+        print(json.dumps(voxel_probe(cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "batch":
         steps = 20 if args.steps is None else args.steps
