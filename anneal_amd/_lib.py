@@ -67,6 +67,7 @@ SIGNATURES = {
     "anneal_fem_learned_stage": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_learned_pair": [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
     "anneal_batch_create": [ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int],
+    "anneal_fem_group_create": [ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int, c_int, c_void_p],
     "anneal_batch_destroy": [c_void_p],
     "anneal_batch_solve": [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_intp, c_double, c_int, c_int,
                            c_intp, c_intp],

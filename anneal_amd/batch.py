@@ -2,11 +2,13 @@
 
 No counterpart in the reference, which runs one part at a time
 (anneal/optimize.py:50-180).  The solves of all problems go through one
-``BatchSolve`` (anneal_batch_solve): every PCG iteration of the whole batch
-is a single hipGraph launch whose per-problem branches run concurrently, with
-one host wait per iteration for all problems.  The rest of each SIMP iteration
-(filter, projection, sensitivities, OC) is the per-problem device code of
-``SimpRun``.
+``BatchSolve``: problems of the same grid shape form a ``FemGroup`` (one block
+of right-hand sides, every kernel of the solve covering all of them in one
+launch), and the groups and remaining single problems advance in lock-step
+(anneal_batch_solve: one iteration graph per context, launched concurrently,
+one host wait per iteration for the whole batch).  The rest of each SIMP
+iteration (filter, projection, sensitivities, OC) is the per-problem device
+code of ``SimpRun``.
 """
 import ctypes
 
@@ -17,19 +19,102 @@ from .fem import NOT_CONVERGED, SolverError
 from .optimize import SimpRun
 
 
-class BatchSolve:
-    """anneal_batch_* over FEM objects that share the current stream."""
+class FemGroup:
+    """Problems of one grid shape solved as one block of right-hand sides (anneal_fem_group_create).
 
-    def __init__(self, fems):
+    Problem p owns rhs [p * k, (p + 1) * k) of a solve with nrhs = k * len(fems).
+    The member FEMs keep their stiffness and coarse rebuilds and must outlive the group."""
+
+    def __init__(self, fems, nrhs_max):
+        import torch
         self.fems = list(fems)
+        self.ldn = self.fems[0].ldn
+        self.nrhs_max = int(nrhs_max)
         self._ctx = ctypes.c_void_p()
         arr = (ctypes.c_void_p * len(self.fems))(*[f._ctx.value for f in self.fems])
-        _lib.call("anneal_batch_create", ctypes.byref(self._ctx), arr, len(self.fems))
+        _lib.call("anneal_fem_group_create", ctypes.byref(self._ctx), arr, len(self.fems), self.nrhs_max,
+                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        self._F = self._U = None
+
+    def buffers(self, k):
+        """Device blocks [k * len(fems), 3, ldn] FP64 for k right-hand sides per problem."""
+        import torch
+        n = k * len(self.fems)
+        if self._F is None or self._F.shape[0] != n:
+            dev = torch.device("cuda", torch.cuda.current_device())
+            self._F = torch.zeros((n, 3, self.ldn), dtype=torch.float64, device=dev)
+            self._U = torch.zeros_like(self._F)
+        return self._F, self._U
+
+    def solve_device(self, F, U, nrhs, rtol, maxiter, use_x0):
+        its = (ctypes.c_int * nrhs)()
+        try:
+            _lib.call("anneal_fem_solve", self._ctx, _lib.ptr(F), _lib.ptr(U), nrhs, float(rtol), int(maxiter),
+                      int(bool(use_x0)), its)
+        except _lib.AnnealLibError as e:
+            if e.status in (3, 4):
+                raise SolverError(NOT_CONVERGED.format(maxiter=maxiter)) from None
+            raise
+        return [int(v) for v in its]
+
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            _lib.call("anneal_fem_destroy", self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _shape_key(f):
+    g = f.grid
+    return (tuple(g.shape), float(g.h), float(f.nu), int(f.levels))
+
+
+class BatchSolve:
+    """anneal_batch_* over FEM objects that share the current stream.
+
+    With ``group=True`` (default) problems of the same grid shape, voxel size,
+    Poisson ratio and hierarchy depth are solved as a FemGroup (at most 64
+    right-hand sides per group, so ``group_rhs`` cases per problem fit); the
+    batch then runs over the groups and the remaining single problems.  A
+    grouped problem's F and U are copied into and out of the group's blocks
+    around each solve (device to device)."""
+
+    def __init__(self, fems, group=True, group_rhs=4):
+        self.fems = list(fems)
+        self.groups = []          # (FemGroup, [problem indices])
+        self.units = []           # per batch member: ("group", gi) or ("single", p)
+        if group:
+            by = {}
+            for p, f in enumerate(self.fems):
+                by.setdefault(_shape_key(f), []).append(p)
+            cap = max(1, 64 // max(1, int(group_rhs)))
+            for key, idx in by.items():
+                for s0 in range(0, len(idx), cap):
+                    part = idx[s0:s0 + cap]
+                    if len(part) >= 2:
+                        self.groups.append((FemGroup([self.fems[p] for p in part], len(part) * int(group_rhs)), part))
+                        self.units.append(("group", len(self.groups) - 1))
+                    else:
+                        self.units.append(("single", part[0]))
+        else:
+            self.units = [("single", p) for p in range(len(self.fems))]
+        ctxs = [self.groups[i][0]._ctx.value if kind == "group" else self.fems[i]._ctx.value for kind, i in self.units]
+        self._ctx = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(ctxs))(*ctxs)
+        _lib.call("anneal_batch_create", ctypes.byref(self._ctx), arr, len(ctxs))
 
     def close(self):
         if getattr(self, "_ctx", None) is not None and self._ctx.value:
             _lib.call("anneal_batch_destroy", self._ctx)
             self._ctx = ctypes.c_void_p()
+        for g, _ in self.groups:
+            g.close()
+        self.groups = []
 
     def __del__(self):
         try:
@@ -39,20 +124,51 @@ class BatchSolve:
 
     def solve(self, F, U, nrhs, rtol, maxiter=2000, use_x0=False):
         """F[p], U[p]: FP64 SoA device blocks; returns per-problem lists of PCG iterations per rhs."""
-        n = len(self.fems)
-        Fp = (ctypes.c_void_p * n)(*[f.data_ptr() for f in F])
-        Up = (ctypes.c_void_p * n)(*[u.data_ptr() for u in U])
-        nr = (ctypes.c_int * n)(*[int(v) for v in nrhs])
-        status = (ctypes.c_int * n)()
-        iters = (ctypes.c_int * (64 * n))()
+        nrhs = [int(v) for v in nrhs]
+        m = len(self.units)
+        Fs, Us, nr = [], [], []
+        for kind, i in self.units:
+            if kind == "single":
+                Fs.append(F[i])
+                Us.append(U[i])
+                nr.append(nrhs[i])
+                continue
+            g, idx = self.groups[i]
+            k = nrhs[idx[0]]
+            if any(nrhs[p] != k for p in idx):
+                raise ValueError("problems of one group must be solved with the same number of load cases")
+            GF, GU = g.buffers(k)
+            for j, p in enumerate(idx):
+                GF[j * k:(j + 1) * k].copy_(F[p][:k])
+                if use_x0:
+                    GU[j * k:(j + 1) * k].copy_(U[p][:k])
+            Fs.append(GF)
+            Us.append(GU)
+            nr.append(k * len(idx))
+        Fp = (ctypes.c_void_p * m)(*[f.data_ptr() for f in Fs])
+        Up = (ctypes.c_void_p * m)(*[u.data_ptr() for u in Us])
+        nra = (ctypes.c_int * m)(*nr)
+        status = (ctypes.c_int * m)()
+        iters = (ctypes.c_int * (64 * m))()
         lib = _lib.load()
-        st = lib.anneal_batch_solve(self._ctx, Fp, Up, nr, float(rtol), int(maxiter), int(bool(use_x0)), status,
+        st = lib.anneal_batch_solve(self._ctx, Fp, Up, nra, float(rtol), int(maxiter), int(bool(use_x0)), status,
                                     iters)
         if st not in (0, 3, 4):
             raise _lib.AnnealLibError("anneal_batch_solve", st, lib.anneal_last_error().decode())
         if st != 0:
             raise SolverError(NOT_CONVERGED.format(maxiter=maxiter))
-        return [[iters[64 * p + r] for r in range(int(nrhs[p]))] for p in range(n)]
+        out = [None] * len(self.fems)
+        for u, (kind, i) in enumerate(self.units):
+            if kind == "single":
+                out[i] = [iters[64 * u + r] for r in range(nrhs[i])]
+                continue
+            g, idx = self.groups[i]
+            k = nrhs[idx[0]]
+            GF, GU = g.buffers(k)
+            for j, p in enumerate(idx):
+                U[p][:k].copy_(GU[j * k:(j + 1) * k])
+                out[p] = [iters[64 * u + j * k + r] for r in range(k)]
+        return out
 
 
 class BatchSimp:
@@ -76,7 +192,7 @@ class BatchSimp:
             raise ValueError("all problems of a batch must use the same rtol")
         self.rtol = rt.pop()
         self.maxiter = max(r.maxiter for r in self.runs)
-        self.solver = BatchSolve([r.fem for r in self.runs])
+        self.solver = BatchSolve([r.fem for r in self.runs], group_rhs=max(r.ncase for r in self.runs))
         self.it = 0
 
     def step(self):

@@ -121,6 +121,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
     FemContext& f = *f_[p];
     const int nr = nrhs[p];
     ANNEAL_CHECK(nr >= 1 && nr <= f.nrhs_max_ && nr <= 64, "nrhs out of range for a batch member");
+    f.set_rhs_count(nr);
     ANNEAL_CHECK(f.learned_mode() == 0, "batched solves use the plain multigrid preconditioner");
     if (!f.coarse_valid_ || (f.coarse_stale_ && f.rebuild_each_solve)) f.rebuild_coarse();
     const GridDims& g0 = f.lv_[0].g;

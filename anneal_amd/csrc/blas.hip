@@ -86,9 +86,10 @@ void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, f
 
 // Dense coarse solve x = Ainv b with Ainv in FP64 (n x n, row major).
 __global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __restrict__ b, float* __restrict__ x,
-                              int nnode, int64_t ld) {
+                              int nnode, int64_t ld, const OpEntry* __restrict__ ops, int rpo) {
   const int n = 3 * nnode;
   const int r = blockIdx.y;
+  if (ops) Ainv = ops[r / rpo].Ainv;
   const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
@@ -105,9 +106,10 @@ __global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __re
     x[(int64_t)(3 * r + c) * ld + node] = (float)s;
   }
 }
-void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s) {
+void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s,
+                 const OpEntry* ops, int rpo) {
   const int n = 3 * nnode;
-  hipLaunchKernelGGL(k_dense_solve, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, Ainv, b, x, nnode, ld);
+  hipLaunchKernelGGL(k_dense_solve, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, Ainv, b, x, nnode, ld, ops, rpo);
   ANNEAL_HIP(hipGetLastError());
 }
 

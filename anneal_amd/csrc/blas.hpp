@@ -23,19 +23,23 @@ int apply_variants();
 void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
                      hipStream_t s);
 void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, float q, hipStream_t s);
+// With ops set (a device table of the level the mask belongs to), rhs r uses
+// ops[r / rpo].mask instead of the mask argument.
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
-                     hipStream_t s);
+                     hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1);
 template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
-                        hipStream_t s);
+                        hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1);
 
 template <typename TA, typename TB>
 void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s);
 void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s);
 void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
              int nrhs, hipStream_t s);
-void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s);
+// With ops set, rhs r uses ops[r / rpo].Ainv.
+void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s,
+                 const OpEntry* ops = nullptr, int rpo = 1);
 void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
                            int ny, int nz, hipStream_t s);
 // st[field] = sum of the partials, then op: 0 none; 1 alpha = rz / t; 2 beta = t / rz, rz = t;

@@ -246,6 +246,7 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs) {
   return guarded([&] {
     ANNEAL_CHECK(r_dev && z_dev && r_dev != (const float*)z_dev, "vcycle needs distinct r and z");
+    fem_of(ctx)->fem->set_rhs_count(nrhs);
     fem_of(ctx)->fem->vcycle(0, r_dev, z_dev, nrhs);
     return 0;
   });
@@ -255,6 +256,7 @@ int anneal_fem_element_energy(void* ctx, const double* U_dev, int nrhs, double* 
   return guarded([&] {
     ANNEAL_CHECK(U_dev && ce_dev, "null argument");
     FemHandle* h = fem_of(ctx);
+    ANNEAL_CHECK(h->flags, "element energies are computed on the problems of a group");
     element_energy(U_dev, nrhs, h->fem->ldn(), h->fem->grid(), h->fem->coef64(), h->flags, ce_dev,
                    h->fem->stream());
     return 0;
@@ -266,6 +268,7 @@ int anneal_fem_stress(void* ctx, const double* U_dev, int nrhs, const double* B0
   return guarded([&] {
     ANNEAL_CHECK(U_dev && B0_host && D_host && local_host && sig_dev && nrhs >= 1, "null argument");
     FemHandle* h = fem_of(ctx);
+    ANNEAL_CHECK(h->flags, "stresses are computed on the problems of a group");
     StressMats m;
     for (int a = 0; a < 6; ++a) {
       for (int b = 0; b < 24; ++b) m.B0[a][b] = B0_host[a * 24 + b];
@@ -530,6 +533,18 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
   return guarded([&] {
     ANNEAL_CHECK(F_dev && U_dev && in_dev && tgt_dev, "null argument");
     fem_of(fem)->fem->learned_pair(F_dev, U_dev, nrhs, level, e_ref, in_dev, tgt_dev);
+    return 0;
+  });
+}
+
+int anneal_fem_group_create(void** ctx, void* const* members, int nmember, int nrhs_max, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(ctx && members && nmember >= 1 && nmember <= 64, "need 1..64 member FEM contexts");
+    *ctx = nullptr;
+    std::vector<FemContext*> v;
+    for (int i = 0; i < nmember; ++i) v.push_back(fem_of(members[i])->fem);
+    auto* fem = new FemContext(v, nrhs_max, (hipStream_t)stream);
+    *ctx = new FemHandle{fem, nullptr};
     return 0;
   });
 }

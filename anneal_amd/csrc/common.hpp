@@ -54,6 +54,17 @@ __device__ __forceinline__ int xcd_logical(int bid, int nb) {
 }
 
 
+// Operator data of one problem on one multigrid level, for kernels that serve
+// several problems of the same grid shape in one launch (right-hand side r
+// belongs to problem r / rpo).  A null table means one operator for all rhs.
+struct OpEntry {
+  const float* Ee;     // FP32 stiffness scales of the level
+  const double* Ee64;  // FP64 stiffness scales (finest level)
+  const uint8_t* mask; // free-dof bits per node
+  const float* dinv;   // Jacobi diagonal inverse
+  const double* Ainv;  // dense inverse (coarsest level)
+};
+
 struct GridDims {
   int nx, ny, nz;
   int64_t nelem() const { return (int64_t)nx * ny * nz; }

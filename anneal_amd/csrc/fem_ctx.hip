@@ -1,5 +1,7 @@
 // Multigrid hierarchy and mixed-precision PCG driver (host side of the solve).
 #include "fem_ctx.hpp"
+
+#include <cstring>
 #include "learned.hpp"
 #include "unet.hpp"
 
@@ -239,38 +241,125 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
       for (int i = 0; i < 576; ++i) kc[i] = ke_[i] * sc;
       ANNEAL_HIP(hipMemcpy(ke_coarse_dev_, kc, sizeof kc, hipMemcpyHostToDevice));
     }
-    const GridDims& g0 = lv_[0].g;
-    Ee64_ = (double*)alloc(g0.nelem() * sizeof(double));
-    const size_t v32 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(float);
-    const size_t v64 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(double);
-    px_ = (float*)alloc(v32);
-    pr_ = (float*)alloc(v32);
-    pz_ = (float*)alloc(v32);
-    pp_ = (float*)alloc(v32);
-    pq_ = (float*)alloc(v32);
-    x64_ = (double*)alloc(v64);
-    r64_ = (double*)alloc(v64);
-    ANNEAL_HIP(hipMemset(px_, 0, v32));
-    ANNEAL_HIP(hipMemset(pr_, 0, v32));
-    ANNEAL_HIP(hipMemset(pz_, 0, v32));
-    ANNEAL_HIP(hipMemset(pp_, 0, v32));
-    ANNEAL_HIP(hipMemset(pq_, 0, v32));
-    ANNEAL_HIP(hipMemset(r64_, 0, v64));
-    int nblk = kDotBlocks;
-    for (int v = 0; v < apply_variants(); ++v) nblk = std::max(nblk, apply_blocks(g0.nx, g0.ny, g0.nz, v));
-    part_ = (double*)alloc((size_t)nblk * nrhs_max * sizeof(double));
-    st_ = (double*)alloc(64 * kSt * sizeof(double));
-    red_ = (double*)alloc(4 * 64 * sizeof(double));
-    scale_ = (double*)alloc(64 * sizeof(double));
-    ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
-    ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
-    ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
-    vgraph_.assign(nrhs_max + 1, nullptr);
+    Ee64_ = (double*)alloc(lv_[0].g.nelem() * sizeof(double));
+    alloc_solver(nrhs_max);
   } catch (...) {
     for (void* p : owned_) (void)hipFree(p);
     owned_.clear();
     throw;
   }
+}
+
+void FemContext::alloc_solver(int nrhs_max) {
+  const GridDims& g0 = lv_[0].g;
+  const size_t v32 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(float);
+  const size_t v64 = (size_t)nrhs_max * 3 * g0.ldn() * sizeof(double);
+  px_ = (float*)alloc(v32);
+  pr_ = (float*)alloc(v32);
+  pz_ = (float*)alloc(v32);
+  pp_ = (float*)alloc(v32);
+  pq_ = (float*)alloc(v32);
+  x64_ = (double*)alloc(v64);
+  r64_ = (double*)alloc(v64);
+  ANNEAL_HIP(hipMemset(px_, 0, v32));
+  ANNEAL_HIP(hipMemset(pr_, 0, v32));
+  ANNEAL_HIP(hipMemset(pz_, 0, v32));
+  ANNEAL_HIP(hipMemset(pp_, 0, v32));
+  ANNEAL_HIP(hipMemset(pq_, 0, v32));
+  ANNEAL_HIP(hipMemset(r64_, 0, v64));
+  int nblk = kDotBlocks;
+  for (int v = 0; v < apply_variants(); ++v) nblk = std::max(nblk, apply_blocks(g0.nx, g0.ny, g0.nz, v));
+  part_ = (double*)alloc((size_t)nblk * nrhs_max * sizeof(double));
+  st_ = (double*)alloc(64 * kSt * sizeof(double));
+  red_ = (double*)alloc(4 * 64 * sizeof(double));
+  scale_ = (double*)alloc(64 * sizeof(double));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
+  ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+  vgraph_.assign(nrhs_max + 1, nullptr);
+}
+
+FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hipStream_t stream)
+    : s_(stream), nrhs_max_(nrhs_max) {
+  const int nop = (int)members.size();
+  ANNEAL_CHECK(nop >= 1 && nop <= 64, "a group holds 1..64 problems");
+  ANNEAL_CHECK(nrhs_max >= nop && nrhs_max <= 64, "group nrhs_max must be in [group size, 64]");
+  for (FemContext* m : members) {
+    ANNEAL_CHECK(m != nullptr, "null member context");
+    ANNEAL_CHECK(!m->is_group(), "a group member must be a single problem");
+    ANNEAL_CHECK(m->s_ == stream, "group members must use the group's stream");
+  }
+  const FemContext& m0 = *members[0];
+  for (FemContext* m : members) {
+    ANNEAL_CHECK(m->lv_.size() == m0.lv_.size(), "group members need the same multigrid levels");
+    for (size_t l = 0; l < m0.lv_.size(); ++l) {
+      const GridDims &a = m->lv_[l].g, &b = m0.lv_[l].g;
+      ANNEAL_CHECK(a.nx == b.nx && a.ny == b.ny && a.nz == b.nz, "group members need the same grid shape");
+    }
+    ANNEAL_CHECK(std::memcmp(m->ke_, m0.ke_, sizeof(ke_)) == 0, "group members need the same voxel size and Poisson ratio");
+  }
+  try {
+    members_ = members;
+    std::memcpy(ke_, m0.ke_, sizeof(ke_));
+    k64_ = m0.k64_;
+    cheb_lmax = m0.cheb_lmax;
+    nu_pre = std::max(2, m0.nu_pre);
+    nu_coarse = std::max(2, m0.nu_coarse);
+    cheb_ratio = m0.cheb_ratio;
+    coarsen_power = m0.coarsen_power;
+    omega = m0.omega;
+    inner_tol_floor = m0.inner_tol_floor;
+    inner_tol_factor = m0.inner_tol_factor;
+    inner_max = m0.inner_max;
+    use_graphs = m0.use_graphs;
+    fuse_first_from = 0;  // the unfused first step has no per-problem operator path
+    tuned_ = true;
+    lv_.resize(m0.lv_.size());
+    for (size_t l = 0; l < lv_.size(); ++l) {
+      Level& L = lv_[l];
+      const Level& M = m0.lv_[l];
+      L.g = M.g;
+      L.h = M.h;
+      L.k = M.k;
+      L.kdiag = M.kdiag;
+      L.tile = M.tile;
+      L.mask = M.mask;  // member 0's: kernels read the tables below
+      L.Ee = M.Ee;
+      L.dinv = M.dinv;
+      const size_t vb = (size_t)nrhs_max * 3 * L.g.ldn() * sizeof(float);
+      L.x = (float*)alloc(vb);
+      L.x2 = (float*)alloc(vb);
+      L.b = (float*)alloc(vb);
+      L.r = (float*)alloc(vb);
+      ANNEAL_HIP(hipMemset(L.x, 0, vb));
+      ANNEAL_HIP(hipMemset(L.x2, 0, vb));
+      ANNEAL_HIP(hipMemset(L.b, 0, vb));
+      ANNEAL_HIP(hipMemset(L.r, 0, vb));
+      std::vector<OpEntry> tab(nop);
+      for (int p = 0; p < nop; ++p) {
+        const FemContext& m = *members[p];
+        tab[p] = OpEntry{m.lv_[l].Ee, m.Ee64_, m.lv_[l].mask, m.lv_[l].dinv, m.Ainv_};
+      }
+      OpEntry* d = (OpEntry*)alloc(nop * sizeof(OpEntry));
+      ANNEAL_HIP(hipMemcpy(d, tab.data(), nop * sizeof(OpEntry), hipMemcpyHostToDevice));
+      optab_.push_back(d);
+    }
+    Ainv_ = m0.Ainv_;
+    Ee64_ = m0.Ee64_;
+    alloc_solver(nrhs_max);
+  } catch (...) {
+    for (void* p : owned_) (void)hipFree(p);
+    owned_.clear();
+    throw;
+  }
+}
+
+void FemContext::set_rhs_count(int nrhs) {
+  if (!is_group()) return;
+  const int nop = (int)members_.size();
+  ANNEAL_CHECK(nrhs >= nop && nrhs % nop == 0,
+               "a group solves the same number of right-hand sides for every problem (nrhs = k * group size)");
+  rpo_ = nrhs / nop;
 }
 
 void FemContext::drop_graphs() {
@@ -318,6 +407,7 @@ void FemContext::vcycle_graph(const float* b, float* x, int nrhs) {
 
 void FemContext::set_learned(UNet* net, int level, double e_ref, int mode) {
   ANNEAL_CHECK(mode >= 0 && mode <= 2, "learned mode must be 0, 1 or 2");
+  ANNEAL_CHECK(!is_group() || !net || mode == 0, "a group solves with the plain multigrid preconditioner");
   if (!net || mode == 0) {
     net_ = nullptr;
     net_mode_ = 0;
@@ -365,6 +455,7 @@ void FemContext::learned_stage(const float* r, float* z, int nrhs) {
 
 void FemContext::learned_pair(const double* F, const double* U, int nrhs, int level, double e_ref, uint16_t* in,
                               float* tgt) {
+  ANNEAL_CHECK(!is_group(), "training pairs come from single problems");
   ANNEAL_CHECK(nrhs >= 1 && nrhs <= nrhs_max_, "nrhs out of range for this context");
   ANNEAL_CHECK(level >= 1 && level < (int)lv_.size(), "the network level must be a coarse multigrid level");
   ANNEAL_CHECK(e_ref > 0.0, "e_ref must be positive");
@@ -439,6 +530,8 @@ ApplyArgs<float> FemContext::args<float>(int l) const {
   a.ny = L.g.ny;
   a.nz = L.g.nz;
   a.ldn = L.g.ldn();
+  a.ops = optab(l);
+  a.rpo = rpo_;
   return a;
 }
 template <>
@@ -455,11 +548,14 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
   a.ny = L.g.ny;
   a.nz = L.g.nz;
   a.ldn = L.g.ldn();
+  a.ops = optab(0);
+  a.rpo = rpo_;
   (void)l;
   return a;
 }
 
 void FemContext::set_Ee(const double* Ee_dev) {
+  ANNEAL_CHECK(!is_group(), "set the stiffness on the problems of a group, not on the group");
   const GridDims& g0 = lv_[0].g;
   ANNEAL_HIP(hipMemcpyAsync(Ee64_, Ee_dev, g0.nelem() * sizeof(double), hipMemcpyDeviceToDevice, s_));
   cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
@@ -474,6 +570,12 @@ void FemContext::set_Ee(const double* Ee_dev) {
 }
 
 void FemContext::rebuild_coarse() {
+  if (is_group()) {
+    // each member keeps its own rebuild policy; coarse_valid_ stays false so every solve looks again
+    for (FemContext* m : members_)
+      if (!m->coarse_valid_ || (m->coarse_stale_ && m->rebuild_each_solve)) m->rebuild_coarse();
+    return;
+  }
   const Level& C = lv_.back();
   dense_assemble_invert(C.Ee, C.mask, ke_coarse_dev_, Ainv_, Awork_, C.g.nx, C.g.ny, C.g.nz, s_);
   coarse_stale_ = false;
@@ -481,12 +583,14 @@ void FemContext::rebuild_coarse() {
 }
 
 void FemContext::apply_f32(const float* u, float* y, int nrhs) {
+  set_rhs_count(nrhs);
   ApplyArgs<float> a = args<float>(0);
   a.u = u;
   a.y = y;
   launch_apply(a, kApply, nrhs, s_);
 }
 void FemContext::apply_f64(const double* u, double* y, int nrhs) {
+  set_rhs_count(nrhs);
   ApplyArgs<double> a = args<double>(0);
   a.u = u;
   a.y = y;
@@ -494,6 +598,7 @@ void FemContext::apply_f64(const double* u, double* y, int nrhs) {
 }
 
 int FemContext::autotune(int variant, float* us_out) {
+  ANNEAL_CHECK(!is_group(), "a group uses the tile shapes of its first problem");
   const int nv = apply_variants();
   drop_graphs();  // tile shapes are baked into captured V-cycles
   if (variant >= 0) {
@@ -543,7 +648,7 @@ int FemContext::autotune(int variant, float* us_out) {
 void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_part) {
   Level& L = lv_[l];
   if (l == (int)lv_.size() - 1) {
-    dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_);
+    dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_, optab(l), rpo_);
     return;
   }
   const int64_t n = L.g.nnode(), ld = L.g.ldn();
@@ -589,6 +694,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
     }
     a.prev_j0 = 0;
   } else {
+    ANNEAL_CHECK(!is_group(), "a group smooths with the fused first step (degree >= 2)");
     jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
     for (int k = 1; k < nu; ++k) {
       a.u = cur;
@@ -604,9 +710,9 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
   a.y = L.r;
   launch_apply(a, kResid, nrhs, s_);
   Level& C = lv_[l + 1];
-  launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_);
+  launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_, optab(l + 1), rpo_);
   vcycle(l + 1, C.b, C.x, nrhs);
-  launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_);
+  launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_, optab(l), rpo_);
   a.u = cur;
   a.y = nxt;
   a.omega = float(1.0 / th);
@@ -628,6 +734,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
 
 int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out) {
   ANNEAL_CHECK(nrhs >= 1 && nrhs <= nrhs_max_, "nrhs out of range for this context");
+  set_rhs_count(nrhs);
   if (!coarse_valid_ || (coarse_stale_ && rebuild_each_solve)) rebuild_coarse();
   const GridDims& g0 = lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();

@@ -46,6 +46,19 @@ class FemContext {
  public:
   FemContext(int nx, int ny, int nz, double h, double nu, const uint8_t* active_host, const uint8_t* fixed_host,
              int nrhs_max, hipStream_t stream, int max_coarse_nodes);
+  // A group: several problems of one grid shape (same h, Poisson ratio and
+  // multigrid levels) solved as one block of right-hand sides, problem p owning
+  // rhs [p * rpo, (p + 1) * rpo) with rpo = nrhs / members.  Every kernel of the
+  // solve covers all problems in one launch, reading each problem's operator
+  // (stiffness scales, masks, Jacobi diagonals, coarse inverse) from the member
+  // contexts through per-level device tables.  The members keep their own
+  // stiffness (set_Ee) and coarse rebuilds, share the group's stream and must
+  // outlive it.
+  FemContext(const std::vector<FemContext*>& members, int nrhs_max, hipStream_t stream);
+  bool is_group() const { return !members_.empty(); }
+  int group_size() const { return (int)members_.size(); }
+  // group: rhs per problem for the next kernels (checks nrhs is a multiple of the group size)
+  void set_rhs_count(int nrhs);
   ~FemContext();
   FemContext(const FemContext&) = delete;
   FemContext& operator=(const FemContext&) = delete;
@@ -142,6 +155,11 @@ class FemContext {
   template <typename T>
   ApplyArgs<T> args(int l) const;
   void* alloc(size_t bytes);
+  void alloc_solver(int nrhs_max);
+  const OpEntry* optab(int l) const { return optab_.empty() ? nullptr : optab_[l]; }
+  std::vector<FemContext*> members_;
+  std::vector<OpEntry*> optab_;  // per level, one entry per member (device)
+  int rpo_ = 1;
 
   hipStream_t s_;
   int nrhs_max_;

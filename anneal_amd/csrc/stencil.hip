@@ -79,6 +79,17 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   T* y = a.y + (int64_t)3 * r * ldn;  // not restrict: kCheb may write over prev
   const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
   const T* pr = (MODE == kCheb && a.prev) ? a.prev + (int64_t)3 * r * ldn : nullptr;
+  // operator of this right-hand side (one problem of a group, or the only one)
+  const T* opEe = a.Ee;
+  const uint8_t* opmask = a.mask;
+  const float* opdinv = a.dinv;
+  if (a.ops) {
+    const OpEntry& o = a.ops[r / a.rpo];
+    if constexpr (sizeof(T) == 8) opEe = (const T*)o.Ee64;
+    else opEe = (const T*)o.Ee;
+    opmask = o.mask;
+    opdinv = o.dinv;
+  }
 
   // Addressing: every access goes through a buffer resource for one plane
   // (uniform, scalar registers) plus a per-thread byte offset that is fixed
@@ -110,8 +121,8 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     const uint32_t nb = pin ? pbytes : 0u;
     if (MODE == kCheb0) {
       // first smoothing step from a zero guess: u = M omega D^-1 b
-      const __amdgpu_buffer_rsrc_t rm = rsrc(a.mask + pb, pin ? uint32_t(sxy) : 0u);
-      const __amdgpu_buffer_rsrc_t rd = rsrc(a.dinv + pb, pin ? uint32_t(sxy) * 4u : 0u);
+      const __amdgpu_buffer_rsrc_t rm = rsrc(opmask + pb, pin ? uint32_t(sxy) : 0u);
+      const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv + pb, pin ? uint32_t(sxy) * 4u : 0u);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
         const uint32_t node = lvo[k] == kOob ? kOob : lvo[k] / uint32_t(sizeof(T));
@@ -136,7 +147,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   };
   auto fetch_e = [&](int L) -> T {
     const bool lin = L >= 0 && L < nz;
-    const __amdgpu_buffer_rsrc_t re = rsrc(a.Ee + exy * (lin ? L : 0), lin ? uint32_t(exy) * uint32_t(sizeof(T)) : 0u);
+    const __amdgpu_buffer_rsrc_t re = rsrc(opEe + exy * (lin ? L : 0), lin ? uint32_t(exy) * uint32_t(sizeof(T)) : 0u);
     return bload<T>(re, evo);
   };
 
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     const bool ok = L >= Z0 && L <= nz && L < Z0 + ZC;
     const int64_t lb = sxy * (ok ? L : 0);
     const uint32_t nb_out = ok ? pbytes : 0u;
-    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(a.mask + lb, ok ? uint32_t(sxy) : 0u), nvo, 0, 0);
+    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(opmask + lb, ok ? uint32_t(sxy) : 0u), nvo, 0, 0);
     E.dv = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -164,7 +175,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
       for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rsrc(bb + c * ldn + lb, nb_out), ovo);
     }
     if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
-      E.dv = bload<float>(rsrc(a.dinv + lb, ok ? uint32_t(sxy) * 4u : 0u), nvo == kOob ? kOob : nvo * 4u);
+      E.dv = bload<float>(rsrc(opdinv + lb, ok ? uint32_t(sxy) * 4u : 0u), nvo == kOob ? kOob : nvo * 4u);
     if (MODE == kCheb && pr) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
@@ -373,13 +384,15 @@ __device__ __forceinline__ void fw_stencil(int I, int n, int (&idx)[3], float (&
 // latency-bound: too few waves, 81 loads each.)
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ maskc, int nx,
-                           int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
+                           int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs,
+                           const OpEntry* __restrict__ ops, int rpo) {
   const int64_t nc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= 3 * nc) return;
   const int c = int(t / nc);
   const int64_t N = t - (int64_t)c * nc;
   const int r = blockIdx.y;
+  if (ops) maskc = ops[r / rpo].mask;
   const int I = N % (cx + 1), J = (N / (cx + 1)) % (cy + 1), K = N / ((int64_t)(cx + 1) * (cy + 1));
   const uint8_t mk = maskc[N];
   int ix[3], iy[3], iz[3];
@@ -404,11 +417,13 @@ __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const u
 // Prolongation with correction: x_f += M_f P x_c (up to 8 coarse nodes per fine node).
 template <typename T>
 __global__ void k_prolong_add(const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ maskf, int nx,
-                              int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs) {
+                              int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs,
+                              const OpEntry* __restrict__ ops, int rpo) {
   const int64_t nf = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (n >= nf) return;
   const int r = blockIdx.y;
+  if (ops) maskf = ops[r / rpo].mask;
   const uint8_t mk = maskf[n];
   if (!mk) return;
   const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
@@ -519,21 +534,21 @@ void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const Grid
 }
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
-                     hipStream_t s) {
+                     hipStream_t s, const OpEntry* ops, int rpo) {
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(3 * c.nnode(), 256), nrhs), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
-                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo);
   ANNEAL_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
-                        hipStream_t s) {
+                        hipStream_t s, const OpEntry* ops, int rpo) {
   hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(f.nnode(), 256), nrhs), dim3(256), 0, s, xc, xf, maskf, f.nx, f.ny, f.nz,
-                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs);
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo);
   ANNEAL_HIP(hipGetLastError());
 }
 template void launch_restrict<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
-                                     hipStream_t);
+                                     hipStream_t, const OpEntry*, int);
 template void launch_prolong_add<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
-                                        hipStream_t);
+                                        hipStream_t, const OpEntry*, int);
 
 }  // namespace anneal

@@ -37,6 +37,8 @@ struct ApplyArgs {
   int64_t ldn;
   int ntx, nty, nzc;   // tiles in x, y and z chunks (set by the launcher)
   int variant;         // tile shape id (host side)
+  const OpEntry* ops;  // per-problem operators (device table) or null: Ee, mask, dinv above serve every rhs
+  int rpo;             // right-hand sides per problem when ops is set
 };
 
 // Face (2-D) Walsh transform of 4 corner values a=(i,j) b=(i+1,j) c=(i,j+1) d=(i+1,j+1)

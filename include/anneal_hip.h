@@ -234,6 +234,16 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
  * status[p] as anneal_fem_solve (0, 3 not converged, 4 non-finite);
  * iters[p * 64 + r] the PCG iterations of rhs r.  Plain multigrid only. */
 int anneal_batch_create(void** batch, void* const* fems, int nprob);
+/* A problem group: nmember FEM contexts of one grid shape (same h, Poisson
+ * ratio and multigrid levels) solved as ONE block of right-hand sides, problem
+ * p owning rhs [p k, (p + 1) k) of an anneal_fem_solve with nrhs = k * nmember.
+ * Every kernel of the solve (apply, smoothers, transfers, coarse solve, PCG
+ * vectors) covers all problems in one launch, reading each problem's
+ * operator from its member context.  The members keep their own stiffness
+ * (anneal_fem_set_Ee) and coarse rebuilds, use the group's stream and must
+ * outlive it; destroy the group with anneal_fem_destroy.  A group handle can
+ * itself be a member of anneal_batch_create. */
+int anneal_fem_group_create(void** ctx, void* const* members, int nmember, int nrhs_max, void* stream);
 int anneal_batch_destroy(void* batch);
 int anneal_batch_solve(void* batch, const double* const* F_dev, double* const* U_dev, const int* nrhs, double rtol,
                        int maxiter, int use_x0, int* status, int* iters);

@@ -72,3 +72,101 @@ def test_batch_simp_matches_individual_runs():
         assert np.abs(rb.result()["x"] - ri.result()["x"]).max() < 1e-6
         ri.fem.close()
     b.close()
+
+
+def _group_problem(shape, seed, nr):
+    """A problem of a group: its own clamp face, random voids and stiffness contrast, nr load cases."""
+    from anneal_amd.fem import FEM, Grid
+    rng = np.random.default_rng(seed)
+    g = Grid(shape, 1.0, (0, 0, 0))
+    active = rng.random(g.nelem) >= 0.1
+    n = np.arange(g.nnode)
+    axis = seed % 3
+    coord = (n % (shape[0] + 1), (n // (shape[0] + 1)) % (shape[1] + 1), n // ((shape[0] + 1) * (shape[1] + 1)))[axis]
+    fixed = (3 * n[coord == 0][:, None] + np.arange(3)).ravel()
+    fem = FEM(g, active, fixed, 0.3, nrhs_max=4)
+    Ee = np.where(rng.random(g.nelem) < 0.2, 1e-9, rng.uniform(0.05, 1.0, g.nelem))
+    K = fem.assemble(Ee[fem.act])
+    fem._activate(K)
+    fem.rebuild()
+    F = [fem.reduce_load(rng.normal(size=3 * g.nnode)) for _ in range(nr)]
+    return fem, K, F
+
+
+def test_group_solve_matches_individual_solves():
+    """Five problems of one shape (different clamps, voids, stiffness) as one block of 15 right-hand sides."""
+    from anneal_amd.batch import FemGroup
+    shape, nr = (16, 8, 12), 3
+    probs = [_group_problem(shape, s, nr) for s in range(5)]
+    fems = [p[0] for p in probs]
+    print("group solve:", len(fems), "x", shape, "x", nr, "cases", flush=True)
+    grp = FemGroup(fems, len(fems) * nr)
+    GF, GU = grp.buffers(nr)
+    for j, (fem, K, F) in enumerate(probs):
+        GF[j * nr:(j + 1) * nr].copy_(fem._to_device(F))
+    its = grp.solve_device(GF, GU, len(fems) * nr, 1e-9, 2000, False)
+    print("group iterations", its, flush=True)
+    for j, (fem, K, F) in enumerate(probs):
+        ub = fem._from_device(GU[j * nr:(j + 1) * nr].contiguous(), nr)
+        ui = fem.solve(K, F, rtol=1e-9)
+        for r in range(nr):
+            assert np.linalg.norm(F[r] - K @ ub[r]) <= 1e-9 * np.linalg.norm(F[r]) * 1.0001
+            assert np.linalg.norm(ub[r] - ui[r]) <= 1e-6 * np.linalg.norm(ui[r])
+    # warm restart from the solution: nothing left to do
+    its2 = grp.solve_device(GF, GU, len(fems) * nr, 1e-9, 2000, True)
+    assert all(i == 0 for i in its2)
+    grp.close()
+    for f in fems:
+        f.close()
+
+
+def test_batch_with_groups_matches_ungrouped():
+    """Groups and single problems in one lock-step batch give the solutions of the ungrouped batch."""
+    from anneal_amd.batch import BatchSolve
+    nr = 2
+    specs = [((12, 8, 8), 10), ((12, 8, 8), 11), ((12, 8, 8), 12), ((10, 10, 6), 13), ((10, 10, 6), 14),
+             ((14, 6, 6), 15)]
+    probs = [_group_problem(shape, seed, nr) for shape, seed in specs]
+    fems = [p[0] for p in probs]
+    Fs = [p[0]._to_device(p[2]) for p in probs]
+    out = {}
+    for grouped in (True, False):
+        Us = [torch.zeros_like(F) for F in Fs]
+        b = BatchSolve(fems, group=grouped, group_rhs=nr)
+        print("batch grouped" if grouped else "batch ungrouped", [k for k, _ in b.units], flush=True)
+        if grouped:
+            assert sorted(len(idx) for _, idx in b.groups) == [2, 3]
+        b.solve(Fs, Us, [nr] * len(fems), 1e-9, 2000)
+        out[grouped] = [fem._from_device(U, nr) for fem, U in zip(fems, Us)]
+        b.close()
+    for (fem, K, F), ug, uu in zip(probs, out[True], out[False]):
+        for r in range(nr):
+            assert np.linalg.norm(F[r] - K @ ug[r]) <= 1e-9 * np.linalg.norm(F[r]) * 1.0001
+            assert np.linalg.norm(ug[r] - uu[r]) <= 1e-6 * np.linalg.norm(uu[r])
+    for f in fems:
+        f.close()
+
+
+def test_batch_simp_of_one_shape_matches_individual_runs():
+    """BatchSimp over three parts of one grid shape (solved as one group) against separate SimpRuns."""
+    import bench
+    from anneal_amd.batch import BatchSimp
+    from anneal_amd.optimize import SimpRun
+    probs = bench.batch_problems(seed=21, n=3, ncase=2, iters=6, shapes=((16, 8, 8),))
+    for p in probs:
+        p["rtol"] = 1e-10
+    b = BatchSimp(probs)
+    assert len(b.solver.groups) == 1
+    for _ in range(6):
+        b.step()
+    keys = ("grid", "fill", "active", "frozen", "fixed", "loads", "weights", "E0", "nu", "vf")
+    for p, rb in zip(probs, b.runs):
+        ri = SimpRun(*[p[k] for k in keys], **{k: v for k, v in p.items() if k not in keys}, log=lambda m: None)
+        for _ in range(6):
+            ri.step()
+        for hb, hi in zip(rb.hist, ri.hist):
+            assert np.allclose(hb["compliance"], hi["compliance"], rtol=1e-7, atol=0)
+            assert abs(hb["volumeFraction"] - hi["volumeFraction"]) < 1e-9
+        assert np.abs(rb.result()["x"] - ri.result()["x"]).max() < 1e-6
+        ri.fem.close()
+    b.close()
