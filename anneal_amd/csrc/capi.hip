@@ -211,6 +211,10 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       ANNEAL_CHECK(value >= 1 && value <= 15, "smoother_degree_coarse must be in [1, 15]");
       f->nu_coarse = (int)value;
       f->invalidate_graphs();
+    } else if (n == "smoother_degree_deep") {
+      ANNEAL_CHECK(value >= 0 && value <= 15, "smoother_degree_deep must be in [0, 15] (0: as the coarse levels)");
+      f->nu_deep = (int)value;
+      f->invalidate_graphs();
     } else if (n == "coarsen_power") {
       ANNEAL_CHECK(value > 0.0 && value <= 1.0, "coarsen_power must be in (0, 1]");
       f->coarsen_power = value;

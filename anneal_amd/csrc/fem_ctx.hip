@@ -305,6 +305,7 @@ FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hi
     cheb_lmax = m0.cheb_lmax;
     nu_pre = std::max(2, m0.nu_pre);
     nu_coarse = std::max(2, m0.nu_coarse);
+    nu_deep = m0.nu_deep > 0 ? std::max(2, m0.nu_deep) : 0;
     cheb_ratio = m0.cheb_ratio;
     coarsen_power = m0.coarsen_power;
     omega = m0.omega;
@@ -652,7 +653,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
     return;
   }
   const int64_t n = L.g.nnode(), ld = L.g.ldn();
-  const int nu = l == 0 ? nu_pre : nu_coarse;  // smoothing degree on this level
+  const int nu = l == 0 ? nu_pre : (l >= 2 && nu_deep > 0 ? nu_deep : nu_coarse);  // smoothing degree here
   // Chebyshev recurrence (three-term form): x_{k+1} = x_k + c1_k (x_k - x_{k-1}) + c2_k D^-1 (b - K x_k)
   const double lmax = cheb_lmax, lmin = lmax / cheb_ratio;
   const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sg = th / de;

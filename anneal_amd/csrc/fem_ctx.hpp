@@ -141,6 +141,7 @@ class FemContext {
   // coarse Ee = power mean of the children, (mean E^q)^(1/q); takes effect at the next set_Ee
   double coarsen_power = 0.5;
   int nu_coarse = 3;  // degree on the coarse levels (l >= 1): cheaper per cell, fewer PCG iterations
+  int nu_deep = 0;    // degree on levels l >= 2 when > 0 (there every dispatch is latency-bound)
   double cheb_ratio = 15.0;
   double cheb_lmax = 0.0;  // set from ke at construction
   float omega = 1.0f;      // default Jacobi weight in the apply arguments (the V-cycle sets its own)

@@ -344,6 +344,56 @@ def voxel_cpu_baseline(prob, fields):
     return out
 
 
+def export_pipeline(dev, iters=100, cpu=True):
+    """The export steps on a real design: the 192x96x96 cantilever after its 100-iteration schedule, with a wall
+    anchor (x = 0 face) and a load anchor around the tip load; output_field -> choose_level -> keep_load_paths
+    on the device, and the same sequence in the oracle on one core."""
+    import numpy as np
+    import torch
+    from types import SimpleNamespace
+    from anneal_amd import export as X
+    from anneal_amd.optimize import simp
+    g, fill, active, frozen, fixed, F = cantilever(SHAPE)
+    res = simp(g, fill, active, frozen, fixed, {"tip": F}, {"tip": 1.0}, 1.0, 0.3, 0.3, iters=iters, r_mm=3.0,
+               tol=0.0, emin=1e-9, rtol=1e-6, log=lambda *a: None)
+    nx, ny, nz = SHAPE
+    anchors = [SimpleNamespace(name="wall", role="fixed", dofs=[0, 1, 2], loads={},
+                               region=_AxisBox([-1, -1, -1], [0.5, ny + 1, nz + 1])),
+               SimpleNamespace(name="tip", role="load", dofs=[0, 1, 2], loads={"tip": np.array([0, 0, -1.0])},
+                               region=_AxisBox([nx - 1.5, ny / 2 - 1.5, nz / 2 - 1.5], [nx + 1, ny / 2 + 1.5,
+                                                                                        nz / 2 + 1.5]))]
+    n = g.nelem
+    hp = SimpleNamespace(grid=g, anchors=anchors, fill=fill, active=active, frozen=frozen, solid_fill=np.zeros(n),
+                         void_fill=np.zeros(n), log=lambda *a: None)
+    dp = SimpleNamespace(**vars(hp))
+    for k in ("fill", "solid_fill", "void_fill"):
+        setattr(dp, k, torch.from_numpy(np.ascontiguousarray(getattr(hp, k), dtype=np.float64)).to(dev))
+    for k in ("active", "frozen"):
+        setattr(dp, k, torch.from_numpy(getattr(hp, k).astype(np.uint8)).to(dev))
+    x = torch.from_numpy(np.asarray(res["x"], dtype=np.float64)).to(dev)
+
+    def run():
+        f = X.output_field(dp, x)
+        lvl = X.choose_level(dp, f, x)
+        f2, lvl2 = X.keep_load_paths(dp, f, lvl, log=lambda *a: None)
+        return lvl, lvl2
+    ms, (lvl, lvl2) = _timed(run, 5)
+    out = {"level": lvl, "level_after_guard": lvl2, "device_ms": round(ms, 3)}
+    if cpu:
+        from oracle import voxel_ref as V
+        xs = np.asarray(res["x"], dtype=np.float64)
+        vp = V.Prob(g, [V.Anchor(a.name, a.role, V.Box(a.region.lo, a.region.hi), loads=a.loads) for a in anchors],
+                    fill, active=active)
+        t = time.perf_counter()
+        f = V.output_field(vp, xs)
+        lr = V.choose_level(vp, f, xs)
+        V.keep_load_paths(vp, f, lr, log=lambda *a: None)
+        out["cpu_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        out["same_level"] = lr == lvl
+    print(f"probe: export pipeline on the cantilever design {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def voxel_probe(reps=5, cpu=True):
     """Device timings of the voxel passes around the SIMP loop at 192x96x96 (wall clock: every pass ends in a
     host read of its counts), with the oracle's CPU time on the same inputs."""
@@ -380,6 +430,8 @@ def voxel_probe(reps=5, cpu=True):
     out = {"probe": "voxel passes", "grid": list(prob.grid.shape), "components": ncomp, "level": level,
            "sssp_sweeps": sweeps, "device_ms": {k: round(v, 3) for k, v in res.items()},
            "device_GBps": {k: round(v, 1) for k, v in gbs.items()}}
+    design = export_pipeline(dev, cpu=cpu)
+    out["cantilever_design"] = design
     if cpu:
         c = voxel_cpu_baseline(host_prob, fields)
         out["cpu_ms"] = {k: round(v, 1) for k, v in c.items()}
