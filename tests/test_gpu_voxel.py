@@ -253,3 +253,35 @@ def test_keep_load_paths_lowering_and_no_repair():
         ref, lvl_ref = V.keep_load_paths(prob, field, 0.5, log=logs_ref.append, repair=repair)
         assert (lvl, logs) == (lvl_ref, logs_ref)
         assert np.array_equal(got, ref)
+
+
+def test_voxel_passes_at_192x96x96_match_oracle():
+    """The full cantilever grid: boundary, connectivity, iso level and the repaired field, all identical."""
+    from anneal_amd.export import choose_level, keep_load_paths, output_field
+    from anneal_amd.problem import boundary, connected
+    shape = (192, 96, 96)
+    g = Grid(shape, 1.0, (0, 0, 0))
+    nx, ny, nz = shape
+    rng = np.random.default_rng(7)
+    anchors = [V.Anchor("wall", "fixed", V.Box([-1, -1, -1], [0.5, ny + 1, nz + 1])),
+               V.Anchor("tip", "load", V.Box([nx - 0.5, -1, -1], [nx + 1, ny + 1, nz + 1]), loads={"d": [0, 0, -1]}),
+               V.Anchor("top", "load", V.Sphere([nx / 2, ny, nz / 2], 3.0), loads={"p": [0, -1, 0]})]
+    active = rng.random(g.nelem) >= 0.03
+    prob = V.Prob(g, anchors, np.where(active, 1.0, 0.0), active=active)
+    print("voxel passes at 192x96x96", flush=True)
+    fixed, F, info = boundary(prob, active)
+    rf, rF, rinfo = V.boundary(prob, active)
+    assert info == rinfo and np.array_equal(fixed, rf)
+    assert all(np.array_equal(F[c], rF[c]) for c in rF)
+    mask = rng.random(g.nelem) < 0.35
+    assert np.array_equal(connected(prob, mask), V.connected(prob, mask))
+    x = rng.random(g.nelem)
+    f = output_field(prob, x)
+    assert np.array_equal(f, V.output_field(prob, x))
+    assert choose_level(prob, f, x) == V.choose_level(prob, f, x)
+    low = 0.3 * rng.random(g.nelem)          # every anchor cut: both repaired along unique shortest paths
+    logs, logs_ref = [], []
+    got, lvl = keep_load_paths(prob, low, 0.5, log=logs.append)
+    ref, lvl_ref = V.keep_load_paths(prob, low, 0.5, log=logs_ref.append)
+    assert (lvl, logs) == (lvl_ref, logs_ref)
+    assert np.array_equal(got, ref)
