@@ -41,7 +41,7 @@ SIGNATURES = {
     "anneal_filter_backward": [c_void_p, c_void_p, c_void_p],
     "anneal_filter_physical": [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "anneal_oc_bisect": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double,
-                         c_double, c_double, c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],
+                         c_double, c_double, c_double, c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],
     "anneal_stiffness_law": [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_int64, c_void_p,
                              c_void_p],
     "anneal_design_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,

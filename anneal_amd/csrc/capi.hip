@@ -344,14 +344,15 @@ int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const do
 
 int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
                      const double* phi_dev, double beta, double move, double lo, double hi, double vtarget,
-                     double hint, double* xnew_dev, int* steps_host, int* evals_host, double* lam_last_host) {
+                     double hint, double hint_width, double* xnew_dev, int* steps_host, int* evals_host,
+                     double* lam_last_host) {
   return guarded([&] {
     ANNEAL_CHECK(x_dev && dcx_dev && dvx_dev && phi_dev && xnew_dev, "null argument");
     ANNEAL_CHECK(lo > 0.0 && hi >= lo, "bisection bracket must satisfy 0 < lo <= hi");
     int ev = 0;
     double ll = 0.0;
     const int s = filt_of(ctx)->oc_bisect(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, lo, hi, vtarget, xnew_dev,
-                                          hint, &ev, &ll);
+                                          hint, hint_width, &ev, &ll);
     if (steps_host) *steps_host = s;
     if (evals_host) *evals_host = ev;
     if (lam_last_host) *lam_last_host = ll;

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
 // Sum the volume partials in a fixed order, record the evaluation in the
 // bracket, replay every bisection trial it decides, and choose the next
 // multiplier to evaluate.
-constexpr double kOcEps = 0.02;          // half-width of the warm bracket around a hint
+constexpr double kOcEps = 0.02;          // default half-width of the warm bracket around a hint
 constexpr double kOcSecantWidth = 0.25;  // log-width below which the root is interpolated
 
 __device__ void oc_record(double* oc, double lam, double f) {
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
     const double stage = oc[kOcStage];
     if (stage == 1.0) {
       // second point of the warm bracket around the hint
-      const double c = oc[kOcHint] * (1.0 + kOcEps);
+      const double c = oc[kOcHint] * (1.0 + oc[kOcWidth]);
       if (c > lo && c < hi) next = c;
       oc[kOcStage] = 2.0;
     } else if (stage == 2.0 && (hasA != hasB)) {
@@ -414,18 +414,21 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
 
 int FilterContext::oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
                              double move, double lo, double hi, double vtarget, double* xnew, double hint,
-                             int* evals, double* lam_last) {
+                             double width, int* evals, double* lam_last) {
+  const double eps = width > 0.0 ? std::min(width, 0.5) : kOcEps;
   double st[kOcLen] = {};
   st[kOcLo] = lo;
   st[kOcHi] = hi;
-  const bool warm = hint * (1.0 - kOcEps) > lo && hint * (1.0 + kOcEps) < hi;
-  st[kOcLam] = warm ? hint * (1.0 - kOcEps) : std::sqrt(lo * hi);
+  const bool warm = hint * (1.0 - eps) > lo && hint * (1.0 + eps) < hi;
+  st[kOcLam] = warm ? hint * (1.0 - eps) : std::sqrt(lo * hi);
   st[kOcStage] = warm ? 1.0 : 3.0;
   st[kOcHint] = hint;
+  st[kOcWidth] = eps;
   st[kOcLo0] = lo;
   st[kOcHi0] = hi;
   st[kOcVt] = vtarget;
-  st[kOcGrow] = 1.25;
+  // outward steps after a miss start a few bracket widths away and square each time
+  st[kOcGrow] = 1.0 + std::min(0.25, std::max(0.02, 10.0 * eps));
   ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
   const int64_t ne = g_.nelem();
   hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_);

@@ -13,7 +13,8 @@ constexpr uint8_t kActive = 1, kFrozen = 2;
 // [A, B] is a certified bracket of the root of vol(lam) = vt.
 constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7,
               kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcTries = 15,
-              kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19, kOcLo0 = 20, kOcHi0 = 21, kOcEnds = 22;
+              kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19, kOcLo0 = 20, kOcHi0 = 21, kOcEnds = 22,
+              kOcWidth = 23;
 constexpr int kOcLen = 24;
 
 class FilterContext {
@@ -35,11 +36,12 @@ class FilterContext {
   // evaluations: vol(lam) is non-increasing, so a certified bracket [A, B] of
   // its root decides every trial outside it; inside, Illinois steps in log(lam)
   // narrow the bracket first.  hint > 0 is a guess of the root (e.g. the
-  // previous iteration's) used for the first evaluation.  xnew receives the
-  // last trial design; returns the number of trials (the reference's count).
+  // previous iteration's): the first two evaluations sit at hint (1 -+ width)
+  // (width <= 0: 0.02).  xnew receives the last trial design; returns the
+  // number of trials (the reference's count).
   int oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
-                double lo, double hi, double vtarget, double* xnew, double hint = 0.0, int* evals = nullptr,
-                double* lam_last = nullptr);
+                double lo, double hi, double vtarget, double* xnew, double hint = 0.0, double width = 0.0,
+                int* evals = nullptr, double* lam_last = nullptr);
 
   const GridDims& grid() const { return g_; }
   const uint8_t* flags() const { return flags_; }

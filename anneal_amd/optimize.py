@@ -77,11 +77,13 @@ class DeviceFilter:
         _lib.call("anneal_filter_physical", self._ctx, _lib.ptr(x), float(beta), _lib.ptr(phi), _lib.ptr(xphys),
                   _lib.ptr(dxp), _lib.ptr(vol))
 
-    def oc_bisect(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint=0.0, info=None):
-        """Replayed OC bisection; returns the number of trials.  ``info`` (a dict) gets evals and lam_last."""
+    def oc_bisect(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint=0.0, info=None, width=0.0):
+        """Replayed OC bisection; returns the number of trials.  ``info`` (a dict) gets evals and lam_last.
+        ``hint`` / ``width``: guess of the root and relative half-width of the first bracket around it."""
         steps, evals, lam = _lib.ctypes.c_int(), _lib.ctypes.c_int(), _lib.c_double()
         _lib.call("anneal_oc_bisect", self._ctx, _lib.ptr(x), _lib.ptr(dcx), _lib.ptr(dvx), _lib.ptr(phi),
-                  float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), _lib.ptr(xnew),
+                  float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), float(width),
+                  _lib.ptr(xnew),
                   _lib.ctypes.byref(steps), _lib.ctypes.byref(evals), _lib.ctypes.byref(lam))
         if info is not None:
             info["evals"], info["lam_last"] = evals.value, lam.value
@@ -216,6 +218,8 @@ class SimpRun:
         self.it = 0
         self.converged = False
         self.oc_ratio = 0.0
+        self.oc_ratio_prev = 0.0
+        self.oc_extrapolate = True
         self.oc_evals = 0
 
     def _tick(self):
@@ -318,10 +322,16 @@ class SimpRun:
         ndes = st[3]
         scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
         info = {}
-        hint = scale * self.oc_ratio if self.oc_ratio else 0.0   # previous root relative to its scale
+        # where to look for the root: the previous roots relative to their scales, extrapolated
+        # geometrically from the last two (the multiplier drifts smoothly); only the choice of
+        # multipliers to evaluate depends on it, never the trial decisions or the result
+        ratio = self.oc_ratio
+        if self.oc_extrapolate and self.oc_ratio and self.oc_ratio_prev:
+            ratio = self.oc_ratio * (self.oc_ratio / self.oc_ratio_prev)
+        hint = scale * ratio if ratio else 0.0
         steps = filt.oc_bisect(self.x, self.dcx, self.dvx, self.phi, beta, self.move, scale * 1e-8, scale * 1e8,
                                self.Vtarget, self.xnew, hint=hint, info=info)
-        self.oc_ratio = info["lam_last"] / scale
+        self.oc_ratio_prev, self.oc_ratio = self.oc_ratio, info["lam_last"] / scale
         self.oc_evals += info["evals"]
         te = self._tick()
         T["oc"] += te - td

@@ -121,12 +121,16 @@ def test_oc_bisection(torch, beta, target):
         return torch.from_numpy(a).cuda()
     XN = torch.zeros(g.nelem, dtype=torch.float64, device="cuda")
     X, DCX, DVX, PHI = full(x), full(dcx), full(dvx), full(phi)
-    for hint in (0.0, scale * 0.37, scale * 1e6):
+    # hints far off, near and absent, with the default and with narrow or wide first brackets: only the
+    # number of evaluations may change, never the trials or the design
+    for hint, width in ((0.0, 0.0), (scale * 0.37, 0.0), (scale * 1e6, 0.0), (scale * 0.37, 1e-4),
+                        (scale * 0.37, 0.3)):
         info = {}
-        steps = df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, hint=hint, info=info)
+        steps = df.oc_bisect(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN, hint=hint, info=info,
+                             width=width)
         got = XN.cpu().numpy()[act]
-        print(f"oc beta={beta} {target} hint={hint / scale:.3g}*scale: {steps} trials from {info['evals']} "
-              f"volume evaluations", flush=True)
+        print(f"oc beta={beta} {target} hint={hint / scale:.3g}*scale width={width}: {steps} trials from "
+              f"{info['evals']} volume evaluations", flush=True)
         assert steps == steps_ref
         assert np.abs(got - want).max() <= 1e-10
         assert info["evals"] < steps_ref
