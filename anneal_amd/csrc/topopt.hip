@@ -1,15 +1,16 @@
 // Density filter, projection, sensitivities and OC bisection (FP64, gfx950).
 //
-// The filter is a direct LDS stencil: a workgroup stages a (TX+2R)(TY+2R)(TZ+2R)
-// tile of its input (computed on the fly: masked design, scaled sensitivity or
-// the OC trial design), and each thread produces TZ outputs along z, reading
-// each (dy, dx) column of the tile once and applying the 2R+1 weights of that
-// column to all its outputs.  The hat kernel max(0, r-|d|) is not separable
-// (its unfoldings have rank 2-3), so the direct form is the exact one.
+// The filter is a direct stencil, marched along z (see k_filter): each input
+// layer is staged in LDS once and scattered into register accumulators of the
+// output layers it reaches; the staged value is computed on the fly (masked
+// design, scaled sensitivity or the OC trial design).  The hat kernel
+// max(0, r-|d|) is not separable (its unfoldings have rank 2-3), so the direct
+// form is the exact one.
 #include "topopt.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace anneal {
@@ -27,12 +28,12 @@ struct FilterArgs {
   double* out2;
   double* part;
   const double* oc;
-  double w[125];
+  double wc[13];  // hat weight max(0, r - sqrt(d2)) for squared distance d2 = 0..3R^2
   double beta, move;
   int nx, ny, nz;
 };
 
-constexpr int FTX = 32, FTY = 8, FTZ = 16;
+constexpr int FTX = 32, FTY = 8, FZC = 8;
 
 __device__ __forceinline__ void project(double xt, double beta, double& xp, double& dxp) {
   if (beta <= 0.0) {
@@ -47,14 +48,39 @@ __device__ __forceinline__ void project(double xt, double beta, double& xp, doub
   dxp = beta * (1.0 - th * th) / (t0 + t1);
 }
 
+// z-marching filter.  A workgroup owns an FTX x FTY tile of element columns
+// and a chunk of FZC output layers.  It walks the input layers Z0-R ..
+// Z0+FZC-1+R once: each layer (tile plus an R-wide halo in x and y) is
+// written to one of two LDS buffers, and every thread sums the 25 (or 9)
+// values of its (dy, dx) window into rings of equal dy^2 + dx^2.  The hat
+// weight depends only on dz^2 + dy^2 + dx^2, so the layer's contribution to
+// the output layer |dz| away is sum_m w(dz^2 + m) * ring_m: 6 rings and 3
+// distinct |dz| for R = 2 (18 FMAs per input layer instead of 93).  A window
+// of 2R+1 register accumulators follows the march; output layer j is complete
+// after input layer j + 2R and is finished there (normalisation, projection,
+// volume).  Input layers are fetched two steps ahead in registers, the
+// epilogue operands of a layer one step ahead.  The summation order of every
+// output is fixed (input layers in +z; rings in order; taps in dy, dx order).
+
+// number of (dy', dx') offsets in [-R, R]^2 with dy'^2 + dx'^2 = m
+__host__ __device__ constexpr int ring_taps(int R, int m) {
+  int n = 0;
+  for (int dy = -R; dy <= R; ++dy)
+    for (int dx = -R; dx <= R; ++dx) n += (dy * dy + dx * dx == m) ? 1 : 0;
+  return n;
+}
+
 template <int R, int MODE>
 __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
-  constexpr int LX = FTX + 2 * R, LY = FTY + 2 * R, LZ = FTZ + 2 * R, K = 2 * R + 1;
-  __shared__ double tile[LZ * LY * LX];
-  __shared__ double sred[FTX * FTY / 64];
+  constexpr int K = 2 * R + 1, LX = FTX + 2 * R, LY = FTY + 2 * R, NP = LX * LY, NT = FTX * FTY;
+  constexpr int NLD = (NP + NT - 1) / NT;  // staging loads per thread per layer
+  constexpr int NS = FZC + 2 * R;          // input layers per chunk
+  __shared__ double buf[2][NP];
+  __shared__ double sred[NT / 64];
   if (MODE == kOc) {
     if (a.oc[kOcDone] != 0.0) return;
   }
+  const int tid = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
   int bid = xcd_logical(blockIdx.x, gridDim.x);  // neighbouring tiles share one XCD's L2
@@ -62,100 +88,163 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   bid /= ntx;
   const int by = bid % nty;
   const int bz = bid / nty;
-  const int X0 = bx * FTX, Y0 = by * FTY, Z0 = bz * FTZ;
+  const int X0 = bx * FTX, Y0 = by * FTY, Z0 = bz * FZC;
+  const int64_t exy = (int64_t)nx * ny;
   double rlam = 0.0, mv = 0.0;
   if (MODE == kOc) {
     rlam = 1.0 / sqrt(a.oc[kOcLam]);
     mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
   }
-  // fixed trip count, unrolled: every thread's tile loads are in flight together
-  constexpr int kStage = (LX * LY * LZ + FTX * FTY - 1) / (FTX * FTY);
+
+  // this thread's staging slots: in-plane element offset, or -1 outside the grid
+  int soff[NLD];
 #pragma unroll
-  for (int it = 0; it < kStage; ++it) {
-    const int idx = threadIdx.x + it * FTX * FTY;
-    if (idx >= LX * LY * LZ) break;
-    const int lx = idx % LX, ly = (idx / LX) % LY, lz = idx / (LX * LY);
-    const int gx = X0 - R + lx, gy = Y0 - R + ly, gz = Z0 - R + lz;
-    double v = 0.0;
-    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-      const int64_t e = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-      const uint8_t f = a.flags[e];
-      if (f & kActive) {
-        if (MODE == kHs) {
-          v = 1.0;
-        } else if (MODE == kFwd || MODE == kPhys) {
-          v = a.in[e];
-        } else if (MODE == kBwd) {
-          v = a.in[e] * a.invHs[e];
-        } else {  // kOc: trial design of optimize.py:150-153, with sqrt(c/dv) precomputed
-          if (f & kFrozen) {
-            v = 1.0;
-          } else {
-            const double x = a.in[e];
-            v = fmin(fmax(x * (a.dcx[e] * rlam), fmax(0.0, x - mv)), fmin(1.0, x + mv));
-          }
-        }
-      }
-    }
-    tile[idx] = v;
+  for (int k = 0; k < NLD; ++k) {
+    const int idx = tid + k * NT;
+    const int lx = idx % LX, ly = idx / LX;
+    const int gx = X0 - R + lx, gy = Y0 - R + ly;
+    soff[k] = (idx < NP && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gx + nx * gy : -1;
   }
-  __syncthreads();
-  const int tx = threadIdx.x % FTX, ty = threadIdx.x / FTX;
-  double acc[FTZ];
+  // raw operands of one staged value (flags, input, second input)
+  struct Raw {
+    uint8_t f;
+    double x, y;
+  };
+  Raw pre[2][NLD];
+  auto fetch = [&](int q, int slot) {
+    const bool qin = q >= 0 && q < nz;
 #pragma unroll
-  for (int z = 0; z < FTZ; ++z) acc[z] = 0.0;
+    for (int k = 0; k < NLD; ++k) {
+      Raw& v = pre[slot][k];
+      v.f = 0;
+      v.x = 0.0;
+      v.y = 0.0;
+      if (qin && soff[k] >= 0) {
+        const int64_t e = soff[k] + exy * q;
+        v.f = a.flags[e];
+        if (MODE != kHs) v.x = a.in[e];
+        if (MODE == kBwd) v.y = a.invHs[e];
+        if (MODE == kOc) v.y = a.dcx[e];
+      }
+    }
+  };
+  auto staged = [&](const Raw& v) -> double {
+    if (!(v.f & kActive)) return 0.0;
+    if (MODE == kHs) return 1.0;
+    if (MODE == kFwd || MODE == kPhys) return v.x;
+    if (MODE == kBwd) return v.x * v.y;
+    // kOc: trial design of optimize.py:150-153, with sqrt(c/dv) precomputed
+    if (v.f & kFrozen) return 1.0;
+    return fmin(fmax(v.x * (v.y * rlam), fmax(0.0, v.x - mv)), fmin(1.0, v.x + mv));
+  };
+
+  const int tx = tid % FTX, ty = tid / FTX;
+  const int gx = X0 + tx, gy = Y0 + ty;
+  const bool col = gx < nx && gy < ny;
+  const int64_t coff = gx + (int64_t)nx * gy;
+  struct Epi {
+    uint8_t f;
+    double ih, ph;
+  };
+  auto fetch_epi = [&](int j, Epi& E) {
+    const int z = Z0 + j;
+    E.f = 0;
+    E.ih = 0.0;
+    E.ph = 0.0;
+    if (col && j < FZC && z < nz) {
+      const int64_t e = coff + exy * z;
+      E.f = a.flags[e];
+      if (MODE == kFwd || MODE == kPhys || MODE == kOc) E.ih = a.invHs[e];
+      if (MODE == kPhys || MODE == kOc) E.ph = a.phi[e];
+    }
+  };
+
+  // win[i] accumulates output layer s - 2R + i while input layer s is processed
+  double win[K];
 #pragma unroll
-  for (int dy = 0; dy < K; ++dy)
+  for (int i = 0; i < K; ++i) win[i] = 0.0;
+  double vol = 0.0;
+  fetch(Z0 - R, 0);
+  fetch(Z0 - R + 1, 1);
+  Epi ep[2] = {};
+
+  // one input layer; SL = s & 1 selects the LDS buffer and the prefetch slots
+  auto step = [&](int s, auto slc) {
+    constexpr int SL = decltype(slc)::value;
+    double* b = buf[SL];
 #pragma unroll
-    for (int dx = 0; dx < K; ++dx) {
-      double col[LZ];
+    for (int k = 0; k < NLD; ++k) {
+      const int idx = tid + k * NT;
+      if (idx < NP) b[idx] = staged(pre[SL][k]);
+    }
+    if (s + 2 < NS) fetch(Z0 - R + s + 2, SL);
+    // output layer jo = s - 2R is finished in this step (its operands were fetched
+    // one step earlier, in slot jo & 1 = SL); fetch the next layer's
+    const int jo = s - 2 * R;
+    const Epi E = ep[SL];
+    if (jo + 1 >= 0 && jo + 1 < FZC) fetch_epi(jo + 1, ep[SL ^ 1]);
+    __syncthreads();
+    // ring sums of the (dy, dx) window by squared in-plane distance m = dy'^2 + dx'^2
+    double P[2 * R * R + 1];
 #pragma unroll
-      for (int z = 0; z < LZ; ++z) col[z] = tile[(z * LY + ty + dy) * LX + tx + dx];
+    for (int m = 0; m <= 2 * R * R; ++m) P[m] = 0.0;
 #pragma unroll
-      for (int dz = 0; dz < K; ++dz) {
-        const double w = a.w[(dz * K + dy) * K + dx];
-        if (w != 0.0) {
+    for (int dy = 0; dy < K; ++dy)
 #pragma unroll
-          for (int z = 0; z < FTZ; ++z) acc[z] += w * col[z + dz];
+      for (int dx = 0; dx < K; ++dx) P[(dy - R) * (dy - R) + (dx - R) * (dx - R)] += b[(ty + dy) * LX + tx + dx];
+    // the layer reaches output layers s - R +- o (window slots R +- o) with weight
+    // w(o^2 + m) on ring m
+#pragma unroll
+    for (int o = 0; o <= R; ++o) {
+      double q = 0.0;
+#pragma unroll
+      for (int m = 0; m <= 2 * R * R; ++m)
+        if (ring_taps(R, m) > 0) q += a.wc[o * o + m] * P[m];
+      win[R + o] += q;
+      if (o > 0) win[R - o] += q;
+    }
+    const int z = Z0 + jo;
+    if (jo >= 0 && col && z < nz) {
+      const int64_t e = coff + exy * z;
+      const bool act = E.f & kActive;
+      const double v = win[0];
+      if (MODE == kHs) {
+        a.out[e] = v;
+      } else if (MODE == kFwd) {
+        a.out[e] = act ? v * E.ih : 0.0;
+      } else if (MODE == kBwd) {
+        a.out[e] = act ? v : 0.0;
+      } else {
+        double xp = 0.0, dxp = 0.0;
+        if (act) {
+          project(v * E.ih, a.beta, xp, dxp);
+          if (E.f & kFrozen) xp = 1.0;
+          vol += E.ph * xp;
+        }
+        if (MODE == kPhys) {
+          a.out[e] = xp;
+          a.out2[e] = dxp;
         }
       }
     }
-  const int gx = X0 + tx, gy = Y0 + ty;
-  double vol = 0.0;
 #pragma unroll
-  for (int z = 0; z < FTZ; ++z) {
-    const int gz = Z0 + z;
-    if (gx >= nx || gy >= ny || gz >= nz) continue;
-    const int64_t e = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-    const uint8_t f = a.flags[e];
-    const bool act = f & kActive;
-    if (MODE == kHs) {
-      a.out[e] = acc[z];
-    } else if (MODE == kFwd) {
-      a.out[e] = act ? acc[z] * a.invHs[e] : 0.0;
-    } else if (MODE == kBwd) {
-      a.out[e] = act ? acc[z] : 0.0;
-    } else {
-      double xp = 0.0, dxp = 0.0;
-      if (act) {
-        project(acc[z] * a.invHs[e], a.beta, xp, dxp);
-        if (f & kFrozen) xp = 1.0;
-        vol += a.phi[e] * xp;
-      }
-      if (MODE == kPhys) {
-        a.out[e] = xp;
-        a.out2[e] = dxp;
-      }
-    }
+    for (int i = 0; i + 1 < K; ++i) win[i] = win[i + 1];
+    win[K - 1] = 0.0;
+  };
+  static_assert(NS % 2 == 0, "the step loop handles input layers in pairs");
+#pragma unroll 1
+  for (int s = 0; s < NS; s += 2) {
+    step(s, std::integral_constant<int, 0>{});
+    step(s + 1, std::integral_constant<int, 1>{});
   }
   if (MODE == kPhys || MODE == kOc) {
     for (int off = 32; off > 0; off >>= 1) vol += __shfl_down(vol, off, 64);
-    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = vol;
+    if ((tid & 63) == 0) sred[tid >> 6] = vol;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < FTX * FTY / 64; ++w) s += sred[w];
-      a.part[blockIdx.x] = s;
+    if (tid == 0) {
+      double sm = 0.0;
+      for (int w = 0; w < NT / 64; ++w) sm += sred[w];
+      a.part[blockIdx.x] = sm;
     }
   }
 }
@@ -296,14 +385,9 @@ __global__ void k_inv_hs(double* __restrict__ Hs, double* __restrict__ invHs, in
 }
 
 // ---------------------------------------------------------------------------
-static void cone_weights(double r, int R, double* w) {
-  const int K = 2 * R + 1;
-  for (int dz = 0; dz < K; ++dz)
-    for (int dy = 0; dy < K; ++dy)
-      for (int dx = 0; dx < K; ++dx) {
-        const double d = std::sqrt(double((dx - R) * (dx - R) + (dy - R) * (dy - R) + (dz - R) * (dz - R)));
-        w[(dz * K + dy) * K + dx] = std::max(0.0, r - d);
-      }
+// hat weight per squared voxel distance d2 (optimize.py:28-31: k = max(0, r - |d|))
+static void cone_weights(double r, double* wc) {
+  for (int d2 = 0; d2 < 13; ++d2) wc[d2] = std::max(0.0, r - std::sqrt(double(d2)));
 }
 
 template <int MODE>
@@ -331,10 +415,9 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_CHECK(r_vox > 0.0 && std::isfinite(r_vox), "filter radius must be positive");
   R_ = std::max((int)std::ceil(r_vox) - 1, 1);
   ANNEAL_CHECK(R_ <= 2, "filter radius above 3 voxels is not supported");
-  std::fill(w_, w_ + 125, 0.0);
-  cone_weights(r_vox, R_, w_);
+  cone_weights(r_vox, w_);
   const int64_t n = g_.nelem();
-  nblk_ = ((nx + FTX - 1) / FTX) * ((ny + FTY - 1) / FTY) * ((nz + FTZ - 1) / FTZ);
+  nblk_ = ((nx + FTX - 1) / FTX) * ((ny + FTY - 1) / FTY) * ((nz + FZC - 1) / FZC);
   ANNEAL_HIP(hipMalloc(&flags_, n));
   ANNEAL_HIP(hipMalloc(&Hs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
@@ -346,7 +429,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   FilterArgs a{};
   a.flags = flags_;
   a.out = Hs_;
-  std::copy(w_, w_ + 125, a.w);
+  std::copy(w_, w_ + 13, a.wc);
   a.nx = nx;
   a.ny = ny;
   a.nz = nz;
@@ -373,7 +456,7 @@ void FilterContext::forward(const double* x, double* y) {
   a.flags = flags_;
   a.invHs = invHs_;
   a.out = y;
-  std::copy(w_, w_ + 125, a.w);
+  std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
@@ -386,7 +469,7 @@ void FilterContext::backward(const double* d, double* y) {
   a.flags = flags_;
   a.invHs = invHs_;
   a.out = y;
-  std::copy(w_, w_ + 125, a.w);
+  std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
@@ -404,7 +487,7 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
   a.out2 = dxp;
   a.part = part_;
   a.beta = beta;
-  std::copy(w_, w_ + 125, a.w);
+  std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
@@ -444,7 +527,7 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   a.oc = oc_;
   a.beta = beta;
   a.move = move;
-  std::copy(w_, w_ + 125, a.w);
+  std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;

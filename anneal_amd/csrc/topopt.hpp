@@ -52,7 +52,7 @@ class FilterContext {
  private:
   GridDims g_;
   int R_;
-  double w_[125];
+  double w_[13];  // hat weight per squared voxel distance
   hipStream_t s_;
   uint8_t* flags_ = nullptr;
   double *Hs_ = nullptr, *invHs_ = nullptr, *r0_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
