@@ -7,8 +7,9 @@
 // back to the launch stream, so independent problems can run concurrently and
 // finished ones cost nothing, and the host waits once per iteration for the
 // whole batch instead of once per problem.  Within a problem, a right-hand
-// side that met its tolerance keeps its device "done" flag, so its x, r and p
-// stay untouched.  The FP64 true-residual refinement and its decisions are the
+// side that met its tolerance keeps its device "done" flag: the kernels of
+// the iteration skip its workgroups, so its x, r and p stay untouched and it
+// costs no bandwidth while the other right-hand sides keep iterating.  The FP64 true-residual refinement and its decisions are the
 // same as in FemContext::solve.
 #include "batch.hpp"
 
@@ -65,13 +66,17 @@ void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
   af.u = f.pp_;
   af.y = f.pq_;
   af.dot_part = f.part_;
+  af.skip = f.st_;
   launch_apply(af, kApply, nrhs, f.s_);
   finalize_st(f.part_, nblk_apply, nrhs, f.st_, kPQ, 1, f.s_);
-  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_);
+  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_, f.st_);
   ANNEAL_HIP(hipMemcpyAsync(f.host_red_ + 64, f.red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, f.s_));
-  f.vcycle(0, f.pr_, f.pz_, nrhs);
-  dot_partial(f.pr_, f.pz_, n, ld, nrhs, f.part_, f.s_);
-  finalize_st(f.part_, kDotBlocks, nrhs, f.st_, kRZnew, 2, f.s_);
+  // the cycle's last fine smoothing kernel forms the <r, z> partials (a
+  // one-level hierarchy is a dense solve only: separate dot)
+  const bool fused = f.lv_.size() > 1;
+  f.vcycle(0, f.pr_, f.pz_, nrhs, fused ? f.part_ : nullptr, f.st_);
+  if (!fused) dot_partial(f.pr_, f.pz_, n, ld, nrhs, f.part_, f.s_, f.st_);
+  finalize_st(f.part_, fused ? nblk_apply : kDotBlocks, nrhs, f.st_, kRZnew, 2, f.s_);
   pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nrhs, 0, f.s_);
 }
 
@@ -202,9 +207,13 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       ANNEAL_HIP(hipMemcpyAsync(f.st_, st.hst.data(), st.hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
       to_f32_scaled(f.r64_, f.pr_, f.scale_, n, ld, nr, s_);
       ANNEAL_HIP(hipMemsetAsync(f.px_, 0, (size_t)nr * 3 * ld * sizeof(float), s_));
-      f.vcycle_pcg(nr);
-      dot_partial(f.pr_, f.pz_, n, ld, nr, f.part_, s_);
-      finalize_st(f.part_, kDotBlocks, nr, f.st_, kRZ, 0, s_);
+      f.vcycle_pcg(nr);  // skips the rhs that already meet the tolerance
+      if (f.lv_.size() > 1) {
+        finalize_st(f.part_, apply_blocks(g0.nx, g0.ny, g0.nz, f.lv_[0].tile), nr, f.st_, kRZ, 0, s_);
+      } else {
+        dot_partial(f.pr_, f.pz_, n, ld, nr, f.part_, s_, f.st_);
+        finalize_st(f.part_, kDotBlocks, nr, f.st_, kRZ, 0, s_);
+      }
       pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nr, 1, s_);
       st.inner = true;
       active.push_back(p);

@@ -10,8 +10,10 @@ constexpr int kRedThreads = 256;
 
 template <typename TA, typename TB>
 __global__ __launch_bounds__(kRedThreads) void k_dot_partial(const TA* __restrict__ a, const TB* __restrict__ b,
-                                                             int64_t n, int64_t ld, double* __restrict__ part) {
+                                                             int64_t n, int64_t ld, double* __restrict__ part,
+                                                             const double* __restrict__ skip) {
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   const TA* pa = a + (int64_t)3 * r * ld;
   const TB* pb = b + (int64_t)3 * r * ld;
   double s = 0.0;
@@ -51,7 +53,8 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(const double* __restri
 
 template <typename TA, typename TB>
 void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s) {
-  hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld, part);
+  hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld, part,
+                     (const double*)nullptr);
   ANNEAL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out);
   ANNEAL_HIP(hipGetLastError());
@@ -66,8 +69,10 @@ void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s
 
 // x = M * omega * dinv * b  (first Jacobi sweep from a zero guess)
 __global__ void k_jacobi0(const float* __restrict__ b, float* __restrict__ x, const float* __restrict__ dinv,
-                          const uint8_t* __restrict__ mask, float omega, int64_t n, int64_t ld) {
+                          const uint8_t* __restrict__ mask, float omega, int64_t n, int64_t ld,
+                          const double* __restrict__ skip) {
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float d = omega * dinv[i];
@@ -78,17 +83,19 @@ __global__ void k_jacobi0(const float* __restrict__ b, float* __restrict__ x, co
   }
 }
 void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
-             int nrhs, hipStream_t s) {
+             int nrhs, hipStream_t s, const double* skip) {
   hipLaunchKernelGGL(k_jacobi0, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, s, b, x, dinv, mask, omega, n,
-                     ld);
+                     ld, skip);
   ANNEAL_HIP(hipGetLastError());
 }
 
 // Dense coarse solve x = Ainv b with Ainv in FP64 (n x n, row major).
 __global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __restrict__ b, float* __restrict__ x,
-                              int nnode, int64_t ld, const OpEntry* __restrict__ ops, int rpo) {
+                              int nnode, int64_t ld, const OpEntry* __restrict__ ops, int rpo,
+                              const double* __restrict__ skip) {
   const int n = 3 * nnode;
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   if (ops) Ainv = ops[r / rpo].Ainv;
   const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
@@ -107,9 +114,9 @@ __global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __re
   }
 }
 void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s,
-                 const OpEntry* ops, int rpo) {
+                 const OpEntry* ops, int rpo, const double* skip) {
   const int n = 3 * nnode;
-  hipLaunchKernelGGL(k_dense_solve, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, Ainv, b, x, nnode, ld, ops, rpo);
+  hipLaunchKernelGGL(k_dense_solve, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, Ainv, b, x, nnode, ld, ops, rpo, skip);
   ANNEAL_HIP(hipGetLastError());
 }
 
@@ -274,9 +281,10 @@ void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, 
   hipLaunchKernelGGL(k_finalize_st, dim3(nrhs), dim3(kRedThreads), 0, s, part, nblk, st, field, op);
   ANNEAL_HIP(hipGetLastError());
 }
-void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s) {
+void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s,
+                 const double* skip) {
   hipLaunchKernelGGL((k_dot_partial<float, float>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld,
-                     part);
+                     part, skip);
   ANNEAL_HIP(hipGetLastError());
 }
 
@@ -284,8 +292,10 @@ void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs
 __global__ __launch_bounds__(kRedThreads) void k_pcg_update(float* __restrict__ x, float* __restrict__ rv,
                                                              const float* __restrict__ p, const float* __restrict__ q,
                                                              const double* __restrict__ st, int64_t n, int64_t ld,
-                                                             double* __restrict__ part) {
+                                                             double* __restrict__ part,
+                                                             const double* __restrict__ skip) {
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   const float al = (float)st[r * kSt + kAlpha];
   double s = 0.0;
   for (int c = 0; c < 3; ++c) {
@@ -308,8 +318,8 @@ __global__ __launch_bounds__(kRedThreads) void k_pcg_update(float* __restrict__ 
   }
 }
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
-                double* part, double* out_rr, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, x, r, p, q, st, n, ld, part);
+                double* part, double* out_rr, hipStream_t s, const double* skip) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, x, r, p, q, st, n, ld, part, skip);
   ANNEAL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out_rr);
   ANNEAL_HIP(hipGetLastError());

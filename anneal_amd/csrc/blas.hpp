@@ -10,6 +10,14 @@ constexpr int kDotBlocks = 512;  // fixed reduction grid: results do not depend 
 constexpr int kSt = 8;
 constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kRR = 3, kAlpha = 4, kBeta = 5, kDone = 6, kZQ = 7;
 
+// Kernels of a PCG inner loop take an optional `skip` pointer (the per-rhs
+// scalar block above): the workgroups of a right-hand side whose done flag is
+// set exit at entry and write nothing, so a converged rhs costs no bandwidth
+// while the others of its block keep iterating.  Null means every rhs runs.
+__device__ __forceinline__ bool rhs_skipped(const double* skip, int r) {
+  return skip != nullptr && skip[r * kSt + kDone] != 0.0;
+}
+
 template <typename T>
 struct KeCoef;
 template <typename T>
@@ -27,27 +35,28 @@ void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const Grid
 // ops[r / rpo].mask instead of the mask argument.
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
-                     hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1);
+                     hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1, const double* skip = nullptr);
 template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
-                        hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1);
+                        hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1, const double* skip = nullptr);
 
 template <typename TA, typename TB>
 void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s);
 void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s);
 void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
-             int nrhs, hipStream_t s);
+             int nrhs, hipStream_t s, const double* skip = nullptr);
 // With ops set, rhs r uses ops[r / rpo].Ainv.
 void dense_solve(const double* Ainv, const float* b, float* x, int nnode, int64_t ld, int nrhs, hipStream_t s,
-                 const OpEntry* ops = nullptr, int rpo = 1);
+                 const OpEntry* ops = nullptr, int rpo = 1, const double* skip = nullptr);
 void dense_assemble_invert(const float* Ee, const uint8_t* mask, const double* ke, double* A, double* B, int nx,
                            int ny, int nz, hipStream_t s);
 // st[field] = sum of the partials, then op: 0 none; 1 alpha = rz / t; 2 beta = t / rz, rz = t;
 // 3 (flexible) beta = -t / pq with t = <z_new, K p>; 4 rz = t.
 void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, int op, hipStream_t s);
-void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s);
+void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s,
+                 const double* skip = nullptr);
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
-                double* part, double* out_rr, hipStream_t s);
+                double* part, double* out_rr, hipStream_t s, const double* skip = nullptr);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
 void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
 void axpy_f32_to_f64(const float* x, double* y, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);

@@ -481,7 +481,7 @@ void FemContext::learned_pair(const double* F, const double* U, int nrhs, int le
 
 void FemContext::vcycle_pcg(int nrhs) {
   if (!use_graphs) {
-    vcycle(0, pr_, pz_, nrhs, part_);
+    vcycle(0, pr_, pz_, nrhs, part_, st_);
     return;
   }
   if (!vgraph_[nrhs]) {
@@ -490,7 +490,7 @@ void FemContext::vcycle_pcg(int nrhs) {
     hipGraph_t g = nullptr;
     try {
       ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
-      vcycle(0, pr_, pz_, nrhs, part_);
+      vcycle(0, pr_, pz_, nrhs, part_, st_);
       ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
     } catch (...) {
       s_ = saved;
@@ -646,10 +646,10 @@ int FemContext::autotune(int variant, float* us_out) {
   return lv_[0].tile;
 }
 
-void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_part) {
+void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_part, const double* skip) {
   Level& L = lv_[l];
   if (l == (int)lv_.size() - 1) {
-    dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_, optab(l), rpo_);
+    dense_solve(Ainv_, b, x, (int)L.g.nnode(), L.g.ldn(), nrhs, s_, optab(l), rpo_, skip);
     return;
   }
   const int64_t n = L.g.nnode(), ld = L.g.ldn();
@@ -670,6 +670,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
   float* nxt = x;
   ApplyArgs<float> a = args<float>(l);
   a.b = b;
+  a.skip = skip;
   if (nu >= 2 && l >= fuse_first_from) {
     // steps 1 and 2 fused: x1 = omega D^-1 b is formed inside the kernel,
     // x2 lands where the unfused sequence would put it; step 3 forms its
@@ -696,7 +697,7 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
     a.prev_j0 = 0;
   } else {
     ANNEAL_CHECK(!is_group(), "a group smooths with the fused first step (degree >= 2)");
-    jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_);
+    jacobi0(b, cur, L.dinv, L.mask, float(1.0 / th), n, ld, nrhs, s_, skip);
     for (int k = 1; k < nu; ++k) {
       a.u = cur;
       a.prev = (k == 1) ? nullptr : nxt;
@@ -711,9 +712,9 @@ void FemContext::vcycle(int l, const float* b, float* x, int nrhs, double* dot_p
   a.y = L.r;
   launch_apply(a, kResid, nrhs, s_);
   Level& C = lv_[l + 1];
-  launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_, optab(l + 1), rpo_);
-  vcycle(l + 1, C.b, C.x, nrhs);
-  launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_, optab(l), rpo_);
+  launch_restrict(L.r, C.b, C.mask, L.g, C.g, nrhs, s_, optab(l + 1), rpo_, skip);
+  vcycle(l + 1, C.b, C.x, nrhs, nullptr, skip);
+  launch_prolong_add(C.x, cur, L.mask, L.g, C.g, nrhs, s_, optab(l), rpo_, skip);
   a.u = cur;
   a.y = nxt;
   a.omega = float(1.0 / th);
@@ -808,9 +809,10 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       af.u = pp_;
       af.y = pq_;
       af.dot_part = part_;
+      af.skip = st_;
       launch_apply(af, kApply, nrhs, s_);
       finalize_st(part_, nblk_apply, nrhs, st_, kPQ, 1, s_);  // pq, alpha
-      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_);
+      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_);
       ANNEAL_HIP(hipMemcpyAsync(host_red_ + 64, red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
       ANNEAL_HIP(hipEventRecord(ev_, s_));
       bool speculate = it >= 2;
@@ -822,7 +824,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       const bool flexible = net_ && net_mode_ == 2;
       auto next_direction = [&]() {
         const bool fused = precondition(nrhs);
-        if (!fused) dot_partial(pr_, pz_, n, ld, nrhs, part_, s_);
+        if (!fused) dot_partial(pr_, pz_, n, ld, nrhs, part_, s_, st_);
         const int nblk_rz = fused ? nblk_apply : kDotBlocks;
         if (flexible) {
           finalize_st(part_, nblk_rz, nrhs, st_, kRZnew, 4, s_);  // rz for the next alpha

@@ -79,7 +79,9 @@ class FemContext {
   int solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out);
   // x = M b; with dot_part the last fine-level smoothing kernel also writes the
   // per-block partials of <b, x> (apply grid, [nrhs][blocks]).
-  void vcycle(int l, const float* b, float* x, int nrhs, double* dot_part = nullptr);
+  // skip: the PCG scalar block whose done flags mark right-hand sides to leave
+  // untouched (null outside a PCG inner loop: every rhs is smoothed)
+  void vcycle(int l, const float* b, float* x, int nrhs, double* dot_part = nullptr, const double* skip = nullptr);
   // Tile shape of the FP32 apply: variant >= 0 sets it on every level,
   // variant < 0 times every shape on every multigrid level and keeps the
   // fastest per level (coarse levels are latency-bound and prefer short z

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   const int bxy = lid / a.nzc;
   const int bx = bxy % a.ntx, by = bxy / a.ntx;
   const int r = blockIdx.y;
+  if (rhs_skipped(a.skip, r)) return;
   const int X0 = bx * (TX - 1), Y0 = by * (TY - 1), Z0 = bz * ZC;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int sx = nx + 1;
@@ -385,13 +386,14 @@ __device__ __forceinline__ void fw_stencil(int I, int n, int (&idx)[3], float (&
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ maskc, int nx,
                            int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs,
-                           const OpEntry* __restrict__ ops, int rpo) {
+                           const OpEntry* __restrict__ ops, int rpo, const double* __restrict__ skip) {
   const int64_t nc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= 3 * nc) return;
   const int c = int(t / nc);
   const int64_t N = t - (int64_t)c * nc;
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   if (ops) maskc = ops[r / rpo].mask;
   const int I = N % (cx + 1), J = (N / (cx + 1)) % (cy + 1), K = N / ((int64_t)(cx + 1) * (cy + 1));
   const uint8_t mk = maskc[N];
@@ -418,11 +420,12 @@ __global__ void k_restrict(const T* __restrict__ rf, T* __restrict__ bc, const u
 template <typename T>
 __global__ void k_prolong_add(const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ maskf, int nx,
                               int ny, int nz, int cx, int cy, int cz, int64_t ldf, int64_t ldc, int nrhs,
-                              const OpEntry* __restrict__ ops, int rpo) {
+                              const OpEntry* __restrict__ ops, int rpo, const double* __restrict__ skip) {
   const int64_t nf = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (n >= nf) return;
   const int r = blockIdx.y;
+  if (rhs_skipped(skip, r)) return;
   if (ops) maskf = ops[r / rpo].mask;
   const uint8_t mk = maskf[n];
   if (!mk) return;
@@ -534,21 +537,21 @@ void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const Grid
 }
 template <typename T>
 void launch_restrict(const T* rf, T* bc, const uint8_t* maskc, const GridDims& f, const GridDims& c, int nrhs,
-                     hipStream_t s, const OpEntry* ops, int rpo) {
+                     hipStream_t s, const OpEntry* ops, int rpo, const double* skip) {
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(3 * c.nnode(), 256), nrhs), dim3(256), 0, s, rf, bc, maskc, f.nx, f.ny, f.nz,
-                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo);
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo, skip);
   ANNEAL_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
-                        hipStream_t s, const OpEntry* ops, int rpo) {
+                        hipStream_t s, const OpEntry* ops, int rpo, const double* skip) {
   hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(f.nnode(), 256), nrhs), dim3(256), 0, s, xc, xf, maskf, f.nx, f.ny, f.nz,
-                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo);
+                     c.nx, c.ny, c.nz, f.ldn(), c.ldn(), nrhs, ops, rpo, skip);
   ANNEAL_HIP(hipGetLastError());
 }
 template void launch_restrict<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
-                                     hipStream_t, const OpEntry*, int);
+                                     hipStream_t, const OpEntry*, int, const double*);
 template void launch_prolong_add<float>(const float*, float*, const uint8_t*, const GridDims&, const GridDims&, int,
-                                        hipStream_t, const OpEntry*, int);
+                                        hipStream_t, const OpEntry*, int, const double*);
 
 }  // namespace anneal

@@ -39,6 +39,7 @@ struct ApplyArgs {
   int variant;         // tile shape id (host side)
   const OpEntry* ops;  // per-problem operators (device table) or null: Ee, mask, dinv above serve every rhs
   int rpo;             // right-hand sides per problem when ops is set
+  const double* skip;  // PCG scalar block: rhs with the done flag set are skipped (null: none)
 };
 
 // Face (2-D) Walsh transform of 4 corner values a=(i,j) b=(i+1,j) c=(i,j+1) d=(i+1,j+1)
