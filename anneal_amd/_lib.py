@@ -30,6 +30,7 @@ SIGNATURES = {
     "anneal_fem_apply_f64": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_solve": [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_intp],
     "anneal_fem_autotune": [c_void_p, c_int, c_intp, c_void_p, c_intp],
+    "anneal_fem_set_level_tile": [c_void_p, c_int, c_int],
     "anneal_fem_set_option": [c_void_p, ctypes.c_char_p, c_double],
     "anneal_fem_vcycle": [c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_element_energy": [c_void_p, c_void_p, c_int, c_void_p],

@@ -196,6 +196,13 @@ int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host
   });
 }
 
+int anneal_fem_set_level_tile(void* ctx, int level, int variant) {
+  return guarded([&] {
+    fem_of(ctx)->fem->set_level_tile(level, variant);
+    return 0;
+  });
+}
+
 int anneal_fem_set_option(void* ctx, const char* name, double value) {
   return guarded([&] {
     ANNEAL_CHECK(name, "null option name");

@@ -598,6 +598,15 @@ void FemContext::apply_f64(const double* u, double* y, int nrhs) {
   launch_apply(a, kApply, nrhs, s_);
 }
 
+void FemContext::set_level_tile(int level, int variant) {
+  ANNEAL_CHECK(!is_group(), "a group uses the tile shapes of its first problem");
+  ANNEAL_CHECK(level >= 0 && level < (int)lv_.size(), "level out of range");
+  ANNEAL_CHECK(variant >= 0 && variant < apply_variants(), "unknown apply tile variant");
+  drop_graphs();
+  lv_[level].tile = variant;
+  tuned_ = true;  // a pinned shape is kept: no sweep when the first operator is set
+}
+
 int FemContext::autotune(int variant, float* us_out) {
   ANNEAL_CHECK(!is_group(), "a group uses the tile shapes of its first problem");
   const int nv = apply_variants();

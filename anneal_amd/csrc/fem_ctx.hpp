@@ -88,6 +88,10 @@ class FemContext {
   // chunks).  us_out (may be null) receives the fine level's per-shape
   // microseconds.  Returns the fine level's shape.
   int autotune(int variant, float* us_out);
+  // pin the FP32 apply tile shape of one multigrid level (what autotune picks
+  // per level; captured V-cycles are dropped; the automatic sweep on the first
+  // set_Ee is skipped from then on)
+  void set_level_tile(int level, int variant);
   // The fine-level V-cycle z = M r on the PCG buffers, replayed from a hipGraph
   // captured once per rhs count (pointers and smoother coefficients are fixed;
   // Ee, dinv and the coarse inverse are read from fixed buffers).

@@ -229,6 +229,11 @@ class FEM:
             out.append(tuple(x.value for x in v))
         return out
 
+    def set_level_tiles(self, tiles):
+        """Pin the FP32 apply tile shape id of each multigrid level, finest first (ids as in level_info)."""
+        for l, t in enumerate(tiles):
+            _lib.call("anneal_fem_set_level_tile", self._ctx, int(l), int(t))
+
     def set_learned(self, net, level=1, e_ref=1.0, mode=2):
         """Attach a U-Net (anneal_amd.learned.UNet) as a preconditioner stage of every solve.
 

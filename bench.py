@@ -562,6 +562,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2])
     ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--tiles", default=None,
+                    help="comma-separated FP32 apply tile shape ids per multigrid level, finest first, instead of "
+                         "the autotuner's choice (a profiler's per-dispatch overhead hides the differences it times)")
     ap.add_argument("--probe", choices=["apply-traffic", "unet", "voxel"], default=None,
                     help="apply-traffic: FP32 applies on rotating buffers at 192x96x96 plus four 256 MB copies "
                          "(the workload of the FETCH_SIZE / WRITE_SIZE counter runs); unet: U-Net forward and "
@@ -619,8 +622,11 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)
 
     def new_run(timers=None):
-        return SimpRun(g, fill, active, frozen, fixed, loads, weights, 1.0, 0.3, 0.3, iters=iters, r_mm=3.0,
-                       tol=0.0, emin=1e-9, rtol=1e-6, log=log, timers=timers)
+        r = SimpRun(g, fill, active, frozen, fixed, loads, weights, 1.0, 0.3, 0.3, iters=iters, r_mm=3.0,
+                    tol=0.0, emin=1e-9, rtol=1e-6, log=log, timers=timers)
+        if args.tiles:
+            r.fem.set_level_tiles([int(t) for t in args.tiles.split(",")])
+        return r
     print(f"bench: warm-up run on the {shape} {args.workload}", file=sys.stderr, flush=True)
     warm = new_run()
     for _ in range(args.warmup):
@@ -660,6 +666,8 @@ def main():
         "data": "synthetic (no dataset)",
         "config": {"workload": workload, "solver": "plain MG-PCG", "iterations_timed": f"1-{steps}",
                    "setup_s": round(setup_s, 3), "pcg_iters_total": int(sum(cg)), "pcg_iters_max": int(max(cg)),
+                   "level_tiles": {"ids": [lv[3] for lv in run.fem.level_info()[:-1]],
+                                   "shapes": [TILE_SHAPES[lv[3]] for lv in run.fem.level_info()[:-1]]},
                    "oc_volume_evals": int(run.oc_evals), "oc_trials": int(sum(x["ocSteps"] for x in h)),
                    "final_compliance": h[-1]["compliance"], "final_volume_fraction": h[-1]["volumeFraction"],
                    "learned": lrn},

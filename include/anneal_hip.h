@@ -86,6 +86,12 @@ int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, do
  * shape in use; us_host (may be NULL; room for 16 floats) receives *n_host per-shape times in
  * microseconds, where *n_host is the number of shapes. */
 int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host);
+/* Pin the FP32 apply tile shape (an id as reported by anneal_fem_level_info)
+ * of multigrid level `level` (no reference counterpart): reproduces a tile
+ * choice of the autotuner, e.g. under a profiler whose per-dispatch overhead
+ * hides the differences the autotuner measures.  A context with a pinned
+ * shape skips the sweep it otherwise runs when its first operator is set. */
+int anneal_fem_set_level_tile(void* ctx, int level, int variant);
 /* Solver options (no reference counterpart; the reference's solver has none):
  * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch
  * eagerly), "smoother_degree" (Chebyshev degree per side on the fine level,
