@@ -142,8 +142,10 @@ class FemContext {
   // lmax is the element bound lambda_max(ke) / ke_ii, an upper bound of
   // lambda_max(D^-1 K) for any stiffness scales (x^T K x = sum_e Ee x_e^T ke x_e
   // <= bound * sum_e Ee x_e^T diag(ke) x_e = bound * x^T D x), so the V-cycle
-  // stays symmetric positive definite however heterogeneous Ee is.
-  int nu_pre = 2;
+  // stays symmetric positive definite however heterogeneous Ee is.  Degree 3
+  // on the fine level: 531 PCG iterations over the 100-iteration 192x96x96
+  // schedule against 683 at degree 2 and 638 at degree 4, 5 % less time.
+  int nu_pre = 3;
   // coarse Ee = power mean of the children, (mean E^q)^(1/q); takes effect at the next set_Ee
   double coarsen_power = 0.5;
   int nu_coarse = 3;  // degree on the coarse levels (l >= 1): cheaper per cell, fewer PCG iterations
@@ -153,8 +155,9 @@ class FemContext {
   float omega = 1.0f;      // default Jacobi weight in the apply arguments (the V-cycle sets its own)
   double inner_tol_floor = 1e-5;
   // the inner FP32 PCG stops at inner_tol_factor * rtol * |F| / |r| (relative to its start), so one
-  // refinement pass usually meets rtol on the FP64 residual
-  double inner_tol_factor = 0.7;
+  // refinement pass usually meets rtol on the FP64 residual (0.85: 513 PCG
+  // iterations on that schedule against 531 at 0.7)
+  double inner_tol_factor = 0.85;
   int inner_max = 400;
 
  private:

@@ -95,13 +95,14 @@ int anneal_fem_set_level_tile(void* ctx, int level, int variant);
 /* Solver options (no reference counterpart; the reference's solver has none):
  * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch
  * eagerly), "smoother_degree" (Chebyshev degree per side on the fine level,
- * >= 1), "smoother_degree_coarse" (levels >= 1), "smoother_degree_deep"
- * (levels >= 2; 0 = as the coarse levels), "cheb_ratio" (lambda_max /
- * lambda_min of the smoothed interval), "coarsen_power" (q of the power mean
+ * >= 1; default 3), "smoother_degree_coarse" (levels >= 1; default 3),
+ * "smoother_degree_deep" (levels >= 2; 0 = as the coarse levels), "cheb_ratio"
+ * (lambda_max / lambda_min of the smoothed interval; default 15),
+ * "coarsen_power" (q of the power mean
  * that coarsens the stiffness scales, (0, 1]), "fuse_first_step" (1 = the first
  * two smoothing steps in one kernel on every level, 0 = never),
  * "fuse_first_level" (fuse from this level down), "inner_tol_factor" (the
- * FP32 inner PCG stops at factor * rtol * |F| / |r|), "inner_tol_floor"
+ * FP32 inner PCG stops at factor * rtol * |F| / |r|; default 0.85), "inner_tol_floor"
  * (largest relative residual reduction asked of one inner PCG), "inner_max"
  * (iterations per inner PCG), "rebuild_each_solve" (1 = every solve after a
  * new Ee refreshes the coarse inverse, 0 = only anneal_fem_rebuild does).
