@@ -102,8 +102,23 @@ __global__ void k_dense_solve(const double* __restrict__ Ainv, const float* __re
   if (row >= n) return;
   const double* a = Ainv + (int64_t)row * n;
   const float* br = b + (int64_t)3 * r * ld;
+  // the row is read in groups of 8 loads per lane so several are in flight;
+  // the summation order (j ascending per lane) is the same as one at a time
   double s = 0.0;
-  for (int j = lane; j < n; j += 64) {
+  int j = lane;
+  for (; j + 7 * 64 < n; j += 8 * 64) {
+    double av[8], bv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int jj = j + u * 64;
+      const int node = jj / 3, c = jj - 3 * node;
+      av[u] = a[jj];
+      bv[u] = (double)br[c * ld + node];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += av[u] * bv[u];
+  }
+  for (; j < n; j += 64) {
     const int node = j / 3, c = j - 3 * node;
     s += a[j] * (double)br[c * ld + node];
   }
