@@ -54,8 +54,12 @@ __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t v
 }
 
 
+// FP32 variants are held to 5 waves per SIMD (at most 96 VGPRs with gfx950's
+// 8-register granule): at 97-104 VGPRs the smoothing modes ran at 4 waves, so
+// the 1274 workgroups of the 192x96x96 fine level no longer fit in one round
+// on 256 CUs (25-29 us instead of 20-23 us per launch).
 template <typename T, int TX, int TY, int ZC, int MODE>
-__global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
+__global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyArgs<T> a) {
   constexpr int LX = TX + 1, LY = TY + 1, NT = TX * TY, NP = LX * LY;
   constexpr int NLD = (NP + NT - 1) / NT;  // node loads per thread per plane
   __shared__ T su[3][3][NP];               // three planes: p is written while p-1 is still read
@@ -199,8 +203,14 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
   fetch(Z0 - 1, 0);
   fetch(Z0, 1);
   T eq[2] = {fetch_e(Z0 - 1), fetch_e(Z0)};  // layers used at steps 1 and 2
-  Epi ep[2];
-  fetch_epi(Z0, ep[0]);  // plane finished at step 2
+  // Light modes (K u, residual) fetch the epilogue operands one step ahead;
+  // the smoothing modes read four more arrays there, and a second operand set
+  // would push them past 96 VGPRs (4 instead of 5 waves per SIMD), so they
+  // fetch at the top of the step that uses them (the step's compute covers
+  // the latency).
+  constexpr bool kEpiAhead = (MODE == kApply || MODE == kResid);
+  Epi ep[kEpiAhead ? 2 : 1];
+  if (kEpiAhead) fetch_epi(Z0, ep[0]);  // plane finished at step 2
 
 #pragma unroll
   for (int s = 0; s <= ZC + 1; ++s) {
@@ -225,8 +235,12 @@ __global__ __launch_bounds__(TX* TY) void k_apply(ApplyArgs<T> a) {
     const bool outp = (s >= 2) && (L <= nz);
     Epi E;
     if (s >= 2) {
-      E = ep[s & 1];
-      if (s + 1 <= ZC + 1) fetch_epi(L + 1, ep[(s + 1) & 1]);
+      if constexpr (kEpiAhead) {
+        E = ep[s & 1];
+        if (s + 1 <= ZC + 1) fetch_epi(L + 1, ep[(s + 1) & 1]);
+      } else {
+        fetch_epi(L, E);
+      }
     }
     const int64_t lb = sxy * (outp ? L : 0);  // uniform base of the output plane
     const uint32_t nb_out = outp ? pbytes : 0u;

@@ -191,6 +191,29 @@ def test_autotune_shapes_agree(torch):
     assert 0 <= chosen < 11 and len(us) == 11
 
 
+def test_pinned_level_tiles(torch):
+    """Tile shapes pinned per level survive the first operator (no sweep) and solve to the same answer."""
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem((40, 20, 20), 11, 0.05)
+    fem = _product_fem(g, active, fixed)
+    nlev = len(fem.level_info()) - 1  # the coarsest level is a dense solve
+    pins = [7, 8, 10, 9][:nlev]
+    fem.set_level_tiles(pins)
+    K = fem.assemble(Ee[fem.act])  # 16 000 elements: an unpinned context would sweep here
+    assert [lv[3] for lv in fem.level_info()[:nlev]] == pins
+    f = np.random.default_rng(3).standard_normal(fem.nfree)
+    u1 = fem.solve(K, [f], rtol=1e-10)[0]
+    fem.set_level_tiles([2] * nlev)
+    assert [lv[3] for lv in fem.level_info()[:nlev]] == [2] * nlev
+    u2 = fem.solve(K, [f], rtol=1e-10)[0]
+    # both meet rtol 1e-10 on the FP64 residual: their compliances agree to ~1e-10
+    assert abs(f @ u1 - f @ u2) <= 1e-8 * abs(f @ u1)
+    with pytest.raises(_lib.AnnealLibError):
+        fem.set_level_tiles([11])
+    with pytest.raises(_lib.AnnealLibError):
+        _lib.call("anneal_fem_set_level_tile", fem._ctx, nlev + 1, 0)
+
+
 @pytest.mark.parametrize("shape,inactive", [((40, 20, 20), 0.0), ((33, 17, 11), 0.2)])
 def test_vcycle_is_symmetric_positive_definite(torch, shape, inactive):
     """The preconditioner must be SPD for PCG: <r1, M r2> = <r2, M r1> and <r, M r> > 0.
