@@ -199,7 +199,7 @@ class SimpRun:
         self.F = fem._to_device(Fr)
         self.U = torch.zeros_like(self.F)
         self.extrapolate = bool(extrapolate)
-        self._u_prev = self._u_tmp = None
+        self._u_prev = None
         self.cs_dev = torch.zeros(self.ncase, **f64)
         self.coef_dev = torch.zeros(self.ncase, **f64)
         self.stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -275,11 +275,11 @@ class SimpRun:
             return
         if self._u_prev is None:          # second solve: remember u_1, start from it
             self._u_prev = self.U.clone()
-            self._u_tmp = self.torch.empty_like(self.U)
             return
-        self._u_tmp.copy_(self.U)
-        self.U.mul_(2.0).sub_(self._u_prev)
-        self._u_prev, self._u_tmp = self._u_tmp, self._u_prev
+        # u_{k-2} <- u_{k-2} + 2 (u_{k-1} - u_{k-2}) = 2 u_{k-1} - u_{k-2} in one pass, then the
+        # buffers swap roles: U is the start of this solve, _u_prev keeps u_{k-1}
+        self._u_prev.lerp_(self.U, 2.0)
+        self.U, self._u_prev = self._u_prev, self.U
 
     def _post_solve(self, its):
         """Compliance, sensitivities, OC update and the history entry (optimize.py:125-175)."""
