@@ -157,8 +157,9 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       a.u = U[p];
       a.b = F[p];
       a.y = f.r64_;
+      a.dot_part = f.part_;  // |r|^2 partials from the residual kernel itself
       launch_apply(a, kResid, nrhs[p], s_);
-      dot(f.r64_, f.r64_, g0.nnode(), g0.ldn(), nrhs[p], f.part_, f.red_, s_);
+      finalize(f.part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs[p], f.red_, s_);
       ANNEAL_HIP(hipMemcpyAsync(f.host_red_, f.red_, nrhs[p] * sizeof(double), hipMemcpyDeviceToHost, s_));
     }
     ANNEAL_HIP(hipStreamSynchronize(s_));

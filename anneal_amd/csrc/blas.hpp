@@ -26,6 +26,7 @@ struct ApplyArgs;
 template <typename T>
 void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s);
 int apply_blocks(int nx, int ny, int nz, int variant);
+constexpr int kApplyVariant64 = 0;  // the one tile shape of the FP64 apply (register pressure)
 int apply_variants();
 
 void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,

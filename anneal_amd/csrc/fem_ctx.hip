@@ -769,8 +769,9 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     a.u = U;
     a.b = F;
     a.y = r64_;
+    a.dot_part = part_;  // |r|^2 partials from the residual kernel itself
     launch_apply(a, kResid, nrhs, s_);
-    dot(r64_, r64_, n, ld, nrhs, part_, red_, s_);
+    finalize(part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs, red_, s_);
     ANNEAL_HIP(hipMemcpyAsync(host_red_, red_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
     ANNEAL_HIP(hipStreamSynchronize(s_));
     bool all = true;

@@ -308,6 +308,9 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
             } else if (MODE == kCheb || MODE == kJacobi) {
               // <b, y>: the PCG's <r, z> when this step ends the fine-level V-cycle
               if (a.dot_part && ncol) dot += (double)E.bv[c] * (double)out;
+            } else if (MODE == kResid) {
+              // <y, y>: the squared norm of the (true) residual
+              if (a.dot_part && ncol) dot += (double)out * (double)out;
             }
           }
         }
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
 #pragma unroll
       for (int m = 0; m < 4; ++m) Fprev[c][m] = Fcur[c][m];
   }
-  if ((MODE == kApply || MODE == kCheb || MODE == kJacobi) && a.dot_part) {
+  if ((MODE == kApply || MODE == kCheb || MODE == kJacobi || MODE == kResid) && a.dot_part) {
     // fixed-order block reduction
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((tid & 63) == 0) sred[tid >> 6] = dot;
@@ -510,7 +513,7 @@ static void launch_tile(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
 
 template <typename T, int MODE>
 static void launch_apply_mode(const ApplyArgs<T>& a, int nrhs, int variant, hipStream_t s) {
-  if (sizeof(T) == 8) variant = 0;  // FP64: one shape (register pressure)
+  if (sizeof(T) == 8) variant = kApplyVariant64;  // FP64: one shape (register pressure)
   switch (variant) {
 #define ANNEAL_TILE_LAUNCH(id, a_, b_, c_) \
   case id:                                 \

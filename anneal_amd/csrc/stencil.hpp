@@ -26,7 +26,8 @@ struct ApplyArgs {
   const T* Ee;         // [nelem] (FP64 operator keeps FP64 stiffness scales)
   const uint8_t* mask; // [nnode]
   const float* dinv;   // [nnode] (kJacobi)
-  double* dot_part;    // optional per-block partials [nrhs][nblk_xyz]: kApply sum u.y, kCheb / kJacobi sum b.y
+  double* dot_part;    // optional per-block partials [nrhs][nblk_xyz]: kApply sum u.y, kCheb / kJacobi sum b.y,
+                       // kResid sum y.y
   KeCoef<T> k;
   T omega;
   const T* prev;       // kCheb: previous iterate (read at the node only, so it may alias y)
