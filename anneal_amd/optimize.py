@@ -202,6 +202,12 @@ class SimpRun:
         self._u_prev = None
         self.cs_dev = torch.zeros(self.ncase, **f64)
         self.coef_dev = torch.zeros(self.ncase, **f64)
+        # the weighted sum (and a single case) combines the sensitivities with the constant
+        # weights: they are uploaded once, and the compliances are read with the end-of-step
+        # statistics instead of in a host wait of their own
+        self._fixed_coef = combine == "sum" or self.ncase == 1
+        if self._fixed_coef:
+            self.coef_dev.copy_(torch.from_numpy(self.w.copy()))
         self.stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
         self.timing = timers is not None      # per-phase timing synchronises the device five times per step
         self.T = timers if timers is not None else {}
@@ -294,20 +300,17 @@ class SimpRun:
             self.on_solve(self)
         _lib.call("anneal_dot_f64", _lib.ptr(self.F), _lib.ptr(self.U), self.grid.nnode, fem.ldn, self.ncase,
                   _lib.ptr(self.cs_dev), _lib.ptr(self.work), stream)
-        cs = self.cs_dev.cpu().numpy()
         tc = self._tick()
         T["solve"] += tc - tb
-        if self.c0 is None:
-            self.c0 = cs.copy()
         w = self.w
-        if self.combine == "sum" or self.ncase == 1:                          # optimize.py:132-139
-            obj = float(w @ cs)
-            coef = w
-        else:
+        if not self._fixed_coef:                                               # optimize.py:132-139
+            cs = self.cs_dev.cpu().numpy()
+            if self.c0 is None:
+                self.c0 = cs.copy()
             r = w * cs / self.c0
             obj = float((r ** PNORM).sum() ** (1 / PNORM))
             coef = (r / obj) ** (PNORM - 1) * w / self.c0
-        self.coef_dev.copy_(torch.from_numpy(np.asarray(coef, float)))
+            self.coef_dev.copy_(torch.from_numpy(np.asarray(coef, float)))
         _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(self.U), self.ncase, _lib.ptr(flags),
                   _lib.ptr(self.xphys), _lib.ptr(self.dxp), _lib.ptr(self.phi), _lib.ptr(self.coef_dev), float(p),
                   self.emin, self.E0, _lib.ptr(self.dcdx), _lib.ptr(self.dvdx))
@@ -340,6 +343,11 @@ class SimpRun:
                   _lib.ptr(self.phi), None, None, _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work),
                   stream)
         st = self.stats.cpu().numpy()
+        if self._fixed_coef:                                                   # optimize.py:132-134
+            cs = self.cs_dev.cpu().numpy()
+            if self.c0 is None:
+                self.c0 = cs.copy()
+            obj = float(w @ cs)
         self.x, self.xnew = self.xnew, self.x
         change = float(st[2]) if ndes > 0 else 0.0
         grey = float(st[1] / ndes) if ndes > 0 else 0.0
