@@ -119,7 +119,10 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
 
   // Node planes are prefetched two steps ahead (two register slots), so the
   // load latency of a plane is covered by two steps of compute.
-  T pre[2][NLD][3];
+  // (the fused first smoothing steps stage mask, D^-1 and b per node, so they
+  // prefetch one plane ahead to stay within 96 VGPRs)
+  constexpr int kDepth = (MODE == kCheb0) ? 1 : 2;
+  T pre[kDepth][NLD][3];
   auto fetch = [&](int p, int slot) {
     const bool pin = (p >= 0) && (p <= nz);
     const int64_t pb = sxy * (pin ? p : 0);
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     }
   double dot = 0.0;
   fetch(Z0 - 1, 0);
-  fetch(Z0, 1);
+  if (kDepth == 2) fetch(Z0, 1 % kDepth);
   T eq[2] = {fetch_e(Z0 - 1), fetch_e(Z0)};  // layers used at steps 1 and 2
   // Light modes (K u, residual) fetch the epilogue operands one step ahead;
   // the smoothing modes read four more arrays there, and a second operand set
@@ -221,10 +224,10 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
       const int idx = tid + k * NT;
       if (idx < NP) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[s & 1][k][c];
+        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[s % kDepth][k][c];
       }
     }
-    if (s + 2 <= ZC + 1) fetch(p + 2, s & 1);  // in flight for the next two steps
+    if (s + kDepth <= ZC + 1) fetch(p + kDepth, s % kDepth);  // in flight for the next kDepth steps
     T e_cur = T(0);
     if (s >= 1) {
       e_cur = eq[(s - 1) & 1];  // layer p-1
