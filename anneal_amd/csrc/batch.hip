@@ -286,6 +286,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
   for (int p = 0; p < np; ++p) {
     status[p] = S[p].finished ? S[p].status : 1;
     for (int r = 0; r < 64; ++r) iters[p * 64 + r] = r < nrhs[p] ? S[p].its[r] : 0;
+    f_[p]->maybe_escalate(S[p].its.data(), nrhs[p]);  // a switch bumps the graph generation: re-recorded
   }
 }
 

@@ -222,6 +222,12 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
       ANNEAL_CHECK(value >= 0 && value <= 15, "smoother_degree_deep must be in [0, 15] (0: as the coarse levels)");
       f->nu_deep = (int)value;
       f->invalidate_graphs();
+    } else if (n == "deep_escalate_iters") {
+      ANNEAL_CHECK(value >= 0, "deep_escalate_iters must be >= 0 (0: off)");
+      f->deep_escalate_iters = (int)value;
+    } else if (n == "deep_escalate_degree") {
+      ANNEAL_CHECK(value >= 1 && value <= 15, "deep_escalate_degree must be in [1, 15]");
+      f->deep_escalate_degree = (int)value;
     } else if (n == "coarsen_power") {
       ANNEAL_CHECK(value > 0.0 && value <= 1.0, "coarsen_power must be in (0, 1]");
       f->coarsen_power = value;

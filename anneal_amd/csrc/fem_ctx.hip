@@ -598,6 +598,16 @@ void FemContext::apply_f64(const double* u, double* y, int nrhs) {
   launch_apply(a, kApply, nrhs, s_);
 }
 
+bool FemContext::maybe_escalate(const int* its, int nrhs) {
+  if (deep_escalate_iters <= 0 || nu_deep != 0 || lv_.size() < 4 || nrhs < 1) return false;
+  long sum = 0;
+  for (int r = 0; r < nrhs; ++r) sum += its[r];
+  if (sum <= (long)deep_escalate_iters * nrhs) return false;
+  nu_deep = deep_escalate_degree;
+  drop_graphs();  // the smoothing degree is baked into captured V-cycles
+  return true;
+}
+
 void FemContext::set_level_tile(int level, int variant) {
   ANNEAL_CHECK(!is_group(), "a group uses the tile shapes of its first problem");
   ANNEAL_CHECK(level >= 0 && level < (int)lv_.size(), "level out of range");
@@ -879,6 +889,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
   if (bad) status = 2;
   ANNEAL_HIP(hipStreamSynchronize(s_));
   for (int r = 0; r < nrhs; ++r) iters_out[r] = its[r];
+  maybe_escalate(its.data(), nrhs);
   return status;
 }
 

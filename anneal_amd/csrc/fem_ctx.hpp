@@ -150,6 +150,15 @@ class FemContext {
   double coarsen_power = 0.5;
   int nu_coarse = 3;  // degree on the coarse levels (l >= 1): cheaper per cell, fewer PCG iterations
   int nu_deep = 0;    // degree on levels l >= 2 when > 0 (there every dispatch is latency-bound)
+  // When a solve needs more than deep_escalate_iters PCG iterations per right-hand
+  // side (on average) while nu_deep is 0, the deep levels switch to degree
+  // deep_escalate_degree for the rest of the context's life: a slowly converging
+  // cycle (high stiffness contrast: keep-outs next to solid) gains from more
+  // sweeps there, and they cost little.  0 disables it.
+  int deep_escalate_iters = 20;
+  int deep_escalate_degree = 5;
+  // applies the rule above after a solve; returns true when it switched
+  bool maybe_escalate(const int* its, int nrhs);
   double cheb_ratio = 15.0;
   double cheb_lmax = 0.0;  // set from ke at construction
   float omega = 1.0f;      // default Jacobi weight in the apply arguments (the V-cycle sets its own)

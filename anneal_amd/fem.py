@@ -207,7 +207,8 @@ class FEM:
 
     def set_option(self, name, value):
         """Solver knobs of the device context (anneal_fem_set_option): graphs, smoother_degree (fine level),
-        smoother_degree_coarse (levels >= 1), smoother_degree_deep (levels >= 2, 0 = as coarse), cheb_ratio,
+        smoother_degree_coarse (levels >= 1), smoother_degree_deep (levels >= 2, 0 = as coarse),
+        deep_escalate_iters / deep_escalate_degree (slow solves raise the deep-level degree once), cheb_ratio,
         coarsen_power, fuse_first_step, fuse_first_level, inner_tol_factor, inner_tol_floor, inner_max,
         rebuild_each_solve."""
         _lib.call("anneal_fem_set_option", self._ctx, name.encode(), float(value))

@@ -96,7 +96,11 @@ int anneal_fem_set_level_tile(void* ctx, int level, int variant);
  * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch
  * eagerly), "smoother_degree" (Chebyshev degree per side on the fine level,
  * >= 1; default 3), "smoother_degree_coarse" (levels >= 1; default 3),
- * "smoother_degree_deep" (levels >= 2; 0 = as the coarse levels), "cheb_ratio"
+ * "smoother_degree_deep" (levels >= 2; 0 = as the coarse levels),
+ * "deep_escalate_iters" (a solve averaging more PCG iterations per rhs than
+ * this, while smoother_degree_deep is 0, sets the deep levels to
+ * "deep_escalate_degree" for the rest of the context; defaults 20 and 5,
+ * 0 = off), "cheb_ratio"
  * (lambda_max / lambda_min of the smoothed interval; default 15),
  * "coarsen_power" (q of the power mean
  * that coarsens the stiffness scales, (0, 1]), "fuse_first_step" (1 = the first

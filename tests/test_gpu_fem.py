@@ -214,6 +214,32 @@ def test_pinned_level_tiles(torch):
         _lib.call("anneal_fem_set_level_tile", fem._ctx, nlev + 1, 0)
 
 
+def test_slow_solves_raise_the_deep_smoothing_degree(torch):
+    """A solve above deep_escalate_iters PCG iterations per rhs switches levels >= 2 to the escalation degree."""
+    g, active, fixed, Ee = random_problem((96, 48, 48), 12, 0.05)
+    a = _product_fem(g, active, fixed)
+    b = _product_fem(g, active, fixed)
+    nlev = len(a.level_info()) - 1
+    assert nlev >= 3  # a smoothed level below level 1 exists
+    for fem in (a, b):  # same tile shapes: same reduction partitions, comparable bits
+        fem.set_level_tiles([2, 10, 8, 8][:nlev])
+    a.set_option("deep_escalate_iters", 1)      # any solve escalates
+    b.set_option("smoother_degree_deep", 5)     # the escalated configuration from the start
+    Ka, Kb = a.assemble(Ee[a.act]), b.assemble(Ee[b.act])
+    f = np.random.default_rng(4).standard_normal(a.nfree)
+    a.solve(Ka, [f], rtol=1e-8)                 # degree 3 on every level, then the switch
+    ua = a.solve(Ka, [f], rtol=1e-8)[0]
+    ub = b.solve(Kb, [f], rtol=1e-8)[0]
+    torch.cuda.synchronize()
+    assert a.cg_iters[-1] == b.cg_iters[-1]
+    assert np.array_equal(ua, ub)
+    # off by default for fast solves: a context left at the defaults keeps degree 3
+    c = _product_fem(g, active, fixed)
+    c.set_option("deep_escalate_iters", 0)
+    uc = c.solve(c.assemble(Ee[c.act]), [f], rtol=1e-8)[0]
+    assert abs(f @ uc - f @ ua) <= 1e-6 * abs(f @ ua)
+
+
 @pytest.mark.parametrize("shape,inactive", [((40, 20, 20), 0.0), ((33, 17, 11), 0.2)])
 def test_vcycle_is_symmetric_positive_definite(torch, shape, inactive):
     """The preconditioner must be SPD for PCG: <r1, M r2> = <r2, M r1> and <r, M r> > 0.
