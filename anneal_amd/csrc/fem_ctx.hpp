@@ -156,7 +156,7 @@ class FemContext {
   // cycle (high stiffness contrast: keep-outs next to solid) gains from more
   // sweeps there, and they cost little.  0 disables it.
   int deep_escalate_iters = 20;
-  int deep_escalate_degree = 5;
+  int deep_escalate_degree = 13;  // large part: 185 / 161 / 150 / 136 / 137 ms per iteration at 3 / 5 / 7 / 13 / 15
   // applies the rule above after a solve; returns true when it switched
   bool maybe_escalate(const int* its, int nrhs);
   double cheb_ratio = 15.0;

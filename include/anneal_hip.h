@@ -99,7 +99,7 @@ int anneal_fem_set_level_tile(void* ctx, int level, int variant);
  * "smoother_degree_deep" (levels >= 2; 0 = as the coarse levels),
  * "deep_escalate_iters" (a solve averaging more PCG iterations per rhs than
  * this, while smoother_degree_deep is 0, sets the deep levels to
- * "deep_escalate_degree" for the rest of the context; defaults 20 and 5,
+ * "deep_escalate_degree" for the rest of the context; defaults 20 and 13,
  * 0 = off), "cheb_ratio"
  * (lambda_max / lambda_min of the smoothed interval; default 15),
  * "coarsen_power" (q of the power mean

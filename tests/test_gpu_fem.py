@@ -224,7 +224,8 @@ def test_slow_solves_raise_the_deep_smoothing_degree(torch):
     for fem in (a, b):  # same tile shapes: same reduction partitions, comparable bits
         fem.set_level_tiles([2, 10, 8, 8][:nlev])
     a.set_option("deep_escalate_iters", 1)      # any solve escalates
-    b.set_option("smoother_degree_deep", 5)     # the escalated configuration from the start
+    a.set_option("deep_escalate_degree", 7)
+    b.set_option("smoother_degree_deep", 7)     # the escalated configuration from the start
     Ka, Kb = a.assemble(Ee[a.act]), b.assemble(Ee[b.act])
     f = np.random.default_rng(4).standard_normal(a.nfree)
     a.solve(Ka, [f], rtol=1e-8)                 # degree 3 on every level, then the switch
