@@ -306,6 +306,8 @@ FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hi
     nu_pre = std::max(2, m0.nu_pre);
     nu_coarse = std::max(2, m0.nu_coarse);
     nu_deep = m0.nu_deep > 0 ? std::max(2, m0.nu_deep) : 0;
+    deep_escalate_iters = m0.deep_escalate_iters;
+    deep_escalate_degree = std::max(2, m0.deep_escalate_degree);
     cheb_ratio = m0.cheb_ratio;
     coarsen_power = m0.coarsen_power;
     omega = m0.omega;
