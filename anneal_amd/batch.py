@@ -203,7 +203,25 @@ class BatchSimp:
         its = self.solver.solve([r.F for r in self.runs], [r.U for r in self.runs], [r.ncase for r in self.runs],
                                 self.rtol, self.maxiter, use_x0=use_x0)
         self.it += 1
-        return [r._post_solve(i) for r, i in zip(self.runs, its)]
+        # the density side of every run in phases, one host wait per phase for the whole batch;
+        # the launches are those of SimpRun.step
+        runs = self.runs
+        for r, i in zip(runs, its):
+            r._sensitivities(i)
+        self._sync()
+        for r in runs:
+            r._oc_begin()
+        pending = runs
+        while pending:
+            pending = [r for r in pending if not r.filt.oc_poll()]
+        for r in runs:
+            r._oc_end()
+        self._sync()
+        return [r._record() for r in runs]
+
+    def _sync(self):
+        for st in {r.stream.value: r._torch_stream for r in self.runs}.values():
+            st.synchronize()
 
     def run(self, iters):
         for _ in range(iters):

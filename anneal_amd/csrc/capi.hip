@@ -373,6 +373,37 @@ int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, cons
   });
 }
 
+int anneal_oc_begin(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                    const double* phi_dev, double beta, double move, double lo, double hi, double vtarget, double hint,
+                    double hint_width, double* xnew_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && dcx_dev && dvx_dev && phi_dev && xnew_dev, "null argument");
+    ANNEAL_CHECK(lo > 0.0 && hi >= lo, "bisection bracket must satisfy 0 < lo <= hi");
+    filt_of(ctx)->oc_begin(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, lo, hi, vtarget, xnew_dev, hint, hint_width);
+    return 0;
+  });
+}
+
+int anneal_oc_poll(void* ctx, int* done_host) {
+  return guarded([&] {
+    ANNEAL_CHECK(done_host, "null argument");
+    *done_host = filt_of(ctx)->oc_poll() ? 1 : 0;
+    return 0;
+  });
+}
+
+int anneal_oc_end(void* ctx, int* steps_host, int* evals_host, double* lam_last_host) {
+  return guarded([&] {
+    int ev = 0;
+    double ll = 0.0;
+    const int s = filt_of(ctx)->oc_end(&ev, &ll);
+    if (steps_host) *steps_host = s;
+    if (evals_host) *evals_host = ev;
+    if (lam_last_host) *lam_last_host = ll;
+    return 0;
+  });
+}
+
 int anneal_stiffness_law(const double* xphys_dev, const double* phi_dev, const uint8_t* flags_dev, double E0,
                          double emin, double p, int64_t n, double* Ee_dev, void* stream) {
   return guarded([&] {

@@ -424,7 +424,8 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&r0_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&part_, std::max(nblk_, 6 * 512) * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&oc_, kOcLen * sizeof(double)));
-  ANNEAL_HIP(hipHostMalloc((void**)&host_, kOcLen * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_, 2 * kOcLen * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipEventCreateWithFlags(&oc_ready_, hipEventDisableTiming));
   ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
   FilterArgs a{};
   a.flags = flags_;
@@ -448,6 +449,7 @@ FilterContext::~FilterContext() {
   (void)hipFree(part_);
   (void)hipFree(oc_);
   (void)hipHostFree(host_);
+  (void)hipEventDestroy(oc_ready_);
 }
 
 void FilterContext::forward(const double* x, double* y) {
@@ -495,11 +497,14 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
   if (vol_dev) finalize(part_, nblk_, 1, vol_dev, s_);
 }
 
-int FilterContext::oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
+void FilterContext::oc_begin(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
                              double move, double lo, double hi, double vtarget, double* xnew, double hint,
-                             double width, int* evals, double* lam_last) {
+                             double width) {
   const double eps = width > 0.0 ? std::min(width, 0.5) : kOcEps;
-  double st[kOcLen] = {};
+  // the initial state goes up from the pinned upper half of host_: the copy may
+  // still be queued when this call returns
+  double* st = host_ + kOcLen;
+  std::fill(st, st + kOcLen, 0.0);
   st[kOcLo] = lo;
   st[kOcHi] = hi;
   const bool warm = hint * (1.0 - eps) > lo && hint * (1.0 + eps) < hi;
@@ -512,46 +517,71 @@ int FilterContext::oc_bisect(const double* x, const double* dcx, const double* d
   st[kOcVt] = vtarget;
   // outward steps after a miss start a few bracket widths away and square each time
   st[kOcGrow] = 1.0 + std::min(0.25, std::max(0.02, 10.0 * eps));
-  ANNEAL_HIP(hipMemcpyAsync(oc_, st, sizeof st, hipMemcpyHostToDevice, s_));
+  ANNEAL_HIP(hipMemcpyAsync(oc_, st, kOcLen * sizeof(double), hipMemcpyHostToDevice, s_));
   const int64_t ne = g_.nelem();
   hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_);
   ANNEAL_HIP(hipGetLastError());
+  run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true};
+  oc_launch(12);
+}
+
+void FilterContext::oc_launch(int count) {
   FilterArgs a{};
-  a.in = x;
+  a.in = run_.x;
   a.dcx = r0_;
   a.dvx = nullptr;
   a.flags = flags_;
   a.invHs = invHs_;
-  a.phi = phi;
+  a.phi = run_.phi;
   a.part = part_;
   a.oc = oc_;
-  a.beta = beta;
-  a.move = move;
+  a.beta = run_.beta;
+  a.move = run_.move;
   std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
-  int launched = 0;
-  while (true) {
-    // evaluations launched after the stop are no-ops that read the device flag
-    const int batch = launched == 0 ? 12 : 8;
-    for (int i = 0; i < batch && launched < 240; ++i, ++launched) {
-      launch_filter(kOc, R_, a, nblk_, s_);
-      hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_, nblk_, oc_);
-      ANNEAL_HIP(hipGetLastError());
-    }
-    ANNEAL_HIP(hipMemcpyAsync(host_, oc_, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
-    ANNEAL_HIP(hipStreamSynchronize(s_));
-    if (host_[kOcDone] != 0.0) break;
-    if (launched >= 240) throw std::runtime_error("OC bisection did not terminate");
+  // evaluations launched after the stop are no-ops that read the device flag
+  for (int i = 0; i < count && run_.launched < 240; ++i, ++run_.launched) {
+    launch_filter(kOc, R_, a, nblk_, s_);
+    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_, nblk_, oc_);
+    ANNEAL_HIP(hipGetLastError());
   }
+  ANNEAL_HIP(hipMemcpyAsync(host_, oc_, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
+  ANNEAL_HIP(hipEventRecord(oc_ready_, s_));
+}
+
+bool FilterContext::oc_poll() {
+  if (!run_.active) throw std::runtime_error("oc_poll without oc_begin");
+  ANNEAL_HIP(hipEventSynchronize(oc_ready_));
+  if (host_[kOcDone] != 0.0) return true;
+  if (run_.launched >= 240) {
+    run_.active = false;
+    throw std::runtime_error("OC bisection did not terminate");
+  }
+  oc_launch(8);
+  return false;
+}
+
+int FilterContext::oc_end(int* evals, double* lam_last) {
+  if (!run_.active || host_[kOcDone] == 0.0) throw std::runtime_error("oc_end before the bisection has stopped");
+  run_.active = false;
   const int64_t n = g_.nelem();
-  hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, x, dcx, dvx, flags_, oc_, beta,
-                     move, n, xnew);
+  hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, run_.x, run_.dcx, run_.dvx,
+                     flags_, oc_, run_.beta, run_.move, n, run_.xnew);
   ANNEAL_HIP(hipGetLastError());
   if (evals) *evals = (int)host_[kOcEvals];
   if (lam_last) *lam_last = host_[kOcLamLast];
   return (int)host_[kOcSteps];
+}
+
+int FilterContext::oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
+                             double move, double lo, double hi, double vtarget, double* xnew, double hint,
+                             double width, int* evals, double* lam_last) {
+  oc_begin(x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint, width);
+  while (!oc_poll()) {
+  }
+  return oc_end(evals, lam_last);
 }
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,16 @@ class FilterContext {
   int oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
                 double lo, double hi, double vtarget, double* xnew, double hint = 0.0, double width = 0.0,
                 int* evals = nullptr, double* lam_last = nullptr);
+  // oc_bisect in three calls, so that one host thread can keep several
+  // contexts' bisections in flight: oc_begin queues the first evaluations and
+  // the state read-back; oc_poll waits for that read-back only, returns true
+  // once the bisection has stopped and otherwise queues the next evaluations;
+  // oc_end queues the final design and returns the trial count.  oc_bisect is
+  // these calls in sequence (same launches).
+  void oc_begin(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
+                double lo, double hi, double vtarget, double* xnew, double hint = 0.0, double width = 0.0);
+  bool oc_poll();
+  int oc_end(int* evals = nullptr, double* lam_last = nullptr);
 
   const GridDims& grid() const { return g_; }
   const uint8_t* flags() const { return flags_; }
@@ -57,6 +67,17 @@ class FilterContext {
   uint8_t* flags_ = nullptr;
   double *Hs_ = nullptr, *invHs_ = nullptr, *r0_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
   int nblk_ = 0;
+  // the bisection in flight (oc_begin .. oc_end)
+  struct OcRun {
+    const double *x, *phi;
+    double* xnew;
+    const double *dcx, *dvx;
+    double beta, move;
+    int launched;
+    bool active;
+  } run_{};
+  hipEvent_t oc_ready_ = nullptr;  // recorded after each state read-back
+  void oc_launch(int count);
 };
 
 // ce_r(e) = u_e^T ke u_e for E = 1 (anneal/fem.py:239-242), out [nrhs][nelem].

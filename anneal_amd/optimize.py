@@ -89,6 +89,27 @@ class DeviceFilter:
             info["evals"], info["lam_last"] = evals.value, lam.value
         return steps.value
 
+    def oc_begin(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint=0.0, width=0.0):
+        """First part of :meth:`oc_bisect`: queue the first evaluations and return."""
+        _lib.call("anneal_oc_begin", self._ctx, _lib.ptr(x), _lib.ptr(dcx), _lib.ptr(dvx), _lib.ptr(phi),
+                  float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), float(width),
+                  _lib.ptr(xnew))
+
+    def oc_poll(self):
+        """Wait for the last queued evaluations; True once the bisection has stopped (else queue more)."""
+        done = _lib.ctypes.c_int()
+        _lib.call("anneal_oc_poll", self._ctx, _lib.ctypes.byref(done))
+        return bool(done.value)
+
+    def oc_end(self, info=None):
+        """Queue the final design; returns the number of trials (as :meth:`oc_bisect`)."""
+        steps, evals, lam = _lib.ctypes.c_int(), _lib.ctypes.c_int(), _lib.c_double()
+        _lib.call("anneal_oc_end", self._ctx, _lib.ctypes.byref(steps), _lib.ctypes.byref(evals),
+                  _lib.ctypes.byref(lam))
+        if info is not None:
+            info["evals"], info["lam_last"] = evals.value, lam.value
+        return steps.value
+
 
 class Filter:
     """Linear hat filter normalised over active voxels (optimize.py:24-47), on the GPU.
@@ -208,7 +229,11 @@ class SimpRun:
         self._fixed_coef = combine == "sum" or self.ncase == 1
         if self._fixed_coef:
             self.coef_dev.copy_(torch.from_numpy(self.w.copy()))
-        self.stream = _lib.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self._torch_stream = torch.cuda.current_stream()
+        self.stream = _lib.c_void_p(self._torch_stream.cuda_stream)
+        # pinned read-back buffers of the per-iteration statistics and compliances
+        self.stats_host = torch.zeros(8, dtype=torch.float64, pin_memory=True)
+        self.cs_host = torch.zeros(self.ncase, dtype=torch.float64, pin_memory=True)
         self.timing = timers is not None      # per-phase timing synchronises the device five times per step
         self.T = timers if timers is not None else {}
         for k in ("physical", "solve", "sensitivity", "oc", "stats"):
@@ -289,42 +314,61 @@ class SimpRun:
 
     def _post_solve(self, its):
         """Compliance, sensitivities, OC update and the history entry (optimize.py:125-175)."""
+        self._sensitivities(its)
+        self._sync()
+        self._oc_begin()
+        while not self.filt.oc_poll():
+            pass
+        self._oc_end()
+        self._sync()
+        return self._record()
+
+    # _post_solve in phases that only queue device work, separated by host waits (``_sync``,
+    # ``filt.oc_poll``), so that a batch of runs can share each wait
+
+    def _sync(self):
+        self._torch_stream.synchronize()
+
+    def _sensitivities(self, its):
+        """Compliance, sensitivities and the statistics that scale the OC bracket (optimize.py:125-145)."""
         torch, fem, filt, T = self.torch, self.fem, self.filt, self.T
-        it, p, beta, rebuild, t0, tb = self.it, self._p, self.beta, self._rebuild, self._t0, self._tb
         nel, flags, stream = self.nel, filt.flags, self.stream
         fem.cg_iters.extend(its)
         self.have_u = True
-        if rebuild:
+        if self._rebuild:
             self.base_iters = fem.cg_iters[-1]
         if self.on_solve is not None:
             self.on_solve(self)
         _lib.call("anneal_dot_f64", _lib.ptr(self.F), _lib.ptr(self.U), self.grid.nnode, fem.ldn, self.ncase,
                   _lib.ptr(self.cs_dev), _lib.ptr(self.work), stream)
         tc = self._tick()
-        T["solve"] += tc - tb
+        T["solve"] += tc - self._tb
         w = self.w
         if not self._fixed_coef:                                               # optimize.py:132-139
             cs = self.cs_dev.cpu().numpy()
             if self.c0 is None:
                 self.c0 = cs.copy()
             r = w * cs / self.c0
-            obj = float((r ** PNORM).sum() ** (1 / PNORM))
-            coef = (r / obj) ** (PNORM - 1) * w / self.c0
+            self._obj = float((r ** PNORM).sum() ** (1 / PNORM))
+            self._cs = cs
+            coef = (r / self._obj) ** (PNORM - 1) * w / self.c0
             self.coef_dev.copy_(torch.from_numpy(np.asarray(coef, float)))
         _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(self.U), self.ncase, _lib.ptr(flags),
-                  _lib.ptr(self.xphys), _lib.ptr(self.dxp), _lib.ptr(self.phi), _lib.ptr(self.coef_dev), float(p),
-                  self.emin, self.E0, _lib.ptr(self.dcdx), _lib.ptr(self.dvdx))
+                  _lib.ptr(self.xphys), _lib.ptr(self.dxp), _lib.ptr(self.phi), _lib.ptr(self.coef_dev),
+                  float(self._p), self.emin, self.E0, _lib.ptr(self.dcdx), _lib.ptr(self.dvdx))
         filt.backward(self.dcdx, self.dcx)
         filt.backward(self.dvdx, self.dvx)
-        td = self._tick()
-        T["sensitivity"] += td - tc
-        # OC (optimize.py:144-163)
+        self._td = self._tick()
+        T["sensitivity"] += self._td - tc
         _lib.call("anneal_design_stats", None, None, None, _lib.ptr(self.phi), _lib.ptr(self.dcx), _lib.ptr(self.dvx),
                   _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work), stream)
-        st = self.stats.cpu().numpy()
-        ndes = st[3]
-        scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
-        info = {}
+        self.stats_host.copy_(self.stats, non_blocking=True)
+
+    def _oc_begin(self):
+        """Bracket scale and root guess from the statistics (read after a wait), then the first OC trials."""
+        st = self.stats_host.numpy()
+        ndes = self._ndes = st[3]
+        scale = self._scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
         # where to look for the root: the previous roots relative to their scales, extrapolated
         # geometrically from the last two (the multiplier drifts smoothly); only the choice of
         # multipliers to evaluate depends on it, never the trial decisions or the result
@@ -332,32 +376,48 @@ class SimpRun:
         if self.oc_extrapolate and self.oc_ratio and self.oc_ratio_prev:
             ratio = self.oc_ratio * (self.oc_ratio / self.oc_ratio_prev)
         hint = scale * ratio if ratio else 0.0
-        steps = filt.oc_bisect(self.x, self.dcx, self.dvx, self.phi, beta, self.move, scale * 1e-8, scale * 1e8,
-                               self.Vtarget, self.xnew, hint=hint, info=info)
-        self.oc_ratio_prev, self.oc_ratio = self.oc_ratio, info["lam_last"] / scale
+        self.filt.oc_begin(self.x, self.dcx, self.dvx, self.phi, self.beta, self.move, scale * 1e-8, scale * 1e8,
+                           self.Vtarget, self.xnew, hint=hint)
+
+    def _oc_end(self):
+        """The OC design (optimize.py:144-163), its physical density and the statistics of the history entry."""
+        filt, flags, nel, stream = self.filt, self.filt.flags, self.nel, self.stream
+        info = self._oc_info = {}
+        self._oc_steps = filt.oc_end(info)
+        self.oc_ratio_prev, self.oc_ratio = self.oc_ratio, info["lam_last"] / self._scale
         self.oc_evals += info["evals"]
-        te = self._tick()
-        T["oc"] += te - td
-        filt.physical(self.xnew, beta, self.phi, self.xphys, self.dxp)
+        self._te = self._tick()
+        self.T["oc"] += self._te - self._td
+        filt.physical(self.xnew, self.beta, self.phi, self.xphys, self.dxp)
         _lib.call("anneal_design_stats", _lib.ptr(self.x), _lib.ptr(self.xnew), _lib.ptr(self.xphys),
                   _lib.ptr(self.phi), None, None, _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work),
                   stream)
-        st = self.stats.cpu().numpy()
+        self.stats_host.copy_(self.stats, non_blocking=True)
+        if self._fixed_coef:
+            self.cs_host.copy_(self.cs_dev, non_blocking=True)
+
+    def _record(self):
+        """History entry and convergence test (optimize.py:162-175), from the statistics read after a wait."""
+        fem, T = self.fem, self.T
+        it, p, beta, rebuild, ndes = self.it, self._p, self.beta, self._rebuild, self._ndes
+        st = self.stats_host.numpy()
         if self._fixed_coef:                                                   # optimize.py:132-134
-            cs = self.cs_dev.cpu().numpy()
+            cs = self.cs_host.numpy().copy()
             if self.c0 is None:
                 self.c0 = cs.copy()
-            obj = float(w @ cs)
+            obj = float(self.w @ cs)
+        else:
+            cs, obj = self._cs, self._obj
         self.x, self.xnew = self.xnew, self.x
         change = float(st[2]) if ndes > 0 else 0.0
         grey = float(st[1] / ndes) if ndes > 0 else 0.0
         volfrac = float(st[0] / self.Vtot)
-        T["stats"] += self._tick() - te
+        T["stats"] += self._tick() - self._te
         ncase = self.ncase
         h = {"iter": it, "objective": obj, "compliance": cs.tolist(), "p": p, "beta": beta,
              "volumeFraction": volfrac, "change": change, "grey": grey,
              "cgIters": fem.cg_iters[-ncase:], "amgRebuilt": bool(rebuild),
-             "seconds": time.time() - t0, "ocSteps": steps, "ocEvals": info["evals"]}
+             "seconds": time.time() - self._t0, "ocSteps": self._oc_steps, "ocEvals": self._oc_info["evals"]}
         self.hist.append(h)
         self.log(f"  it {it:3d}  p {p:.2f}  b {beta:.0f}  obj {obj:.4g}  vol {volfrac:.3f}  "
                  f"change {change:.3f}  grey {grey:.2f}  cg {fem.cg_iters[-1]}{' amg' if rebuild else ''}  "

@@ -153,6 +153,21 @@ int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, cons
                      double hint, double hint_width, double* xnew_dev, int* steps_host, int* evals_host,
                      double* lam_last_host);
 
+/* anneal_oc_bisect in three calls (optimize.py:144-161), so that one host
+ * thread can keep the bisections of several filter contexts in flight:
+ * anneal_oc_begin queues the first volume evaluations and a read-back of the
+ * bisection state on the context's stream and returns; anneal_oc_poll waits for
+ * that read-back only and sets *done_host = 1 once the bisection has stopped,
+ * else queues the next evaluations and sets 0; once done, anneal_oc_end
+ * queues the final design into xnew_dev and returns the counts as
+ * anneal_oc_bisect does.  Launches and results equal anneal_oc_bisect's; one
+ * bisection per context at a time. */
+int anneal_oc_begin(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                    const double* phi_dev, double beta, double move, double lo, double hi, double vtarget, double hint,
+                    double hint_width, double* xnew_dev);
+int anneal_oc_poll(void* ctx, int* done_host);
+int anneal_oc_end(void* ctx, int* steps_host, int* evals_host, double* lam_last_host);
+
 /* ---- element-wise pieces of simp() ---------------------------------------- */
 
 /* Ee = E0 phi (emin + xPhys^p (1-emin)) on active voxels, else 0 (optimize.py:118). */

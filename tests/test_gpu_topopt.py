@@ -140,6 +140,21 @@ def test_oc_bisection(torch, beta, target):
                  PHI, beta, scale, vt, XN), info=info)
     if target == "inside":
         assert info["evals"] <= 12
+    # the split calls with two contexts in flight at once (as a batch of runs uses them): same trials
+    # and design, the second context started from the warm hint
+    df2 = DeviceFilter(Grid(shape, 1.0, (0, 0, 0)), active, r, frozen)
+    XN2 = torch.zeros_like(XN)
+    XN.zero_()
+    df.oc_begin(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN)
+    df2.oc_begin(X, DCX, DVX, PHI, beta, 0.2, scale * 1e-8, scale * 1e8, vt, XN2, hint=scale * 0.37)
+    pending = [df, df2]
+    while pending:
+        pending = [d for d in pending if not d.oc_poll()]
+    infos = [{}, {}]
+    assert df.oc_end(infos[0]) == steps_ref and df2.oc_end(infos[1]) == steps_ref
+    for got in (XN, XN2):
+        assert np.abs(got.cpu().numpy()[act] - want).max() <= 1e-10
+    df2.close()
 
 
 def info_lam(df, X, DCX, DVX, PHI, beta, scale, vt, XN):
