@@ -174,68 +174,119 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
 // pivot block K of up to kGJB rows, out of place between two buffers:
 //   new_KK = P = M_KK^-1, new_KJ = P M_KJ, new_IK = -M_IK P,
 //   new_IJ = M_IJ - (M_IK P) M_KJ.
-// A workgroup owns kGJB rows x kGJC columns: it inverts the small pivot block
-// in LDS (redundantly), forms T = M_IK P for its rows, stages the pivot rows of
-// its columns in LDS, and writes its tile with coalesced row segments.
-constexpr int kGJB = 16, kGJC = 256;
+// A workgroup owns kGJB rows x kGJC columns.  It inverts the small pivot block
+// in LDS (redundantly in every workgroup), forms N = -M_IK P for its rows (N = P
+// on the pivot rows, whose tile starts from 0 instead of M_IJ), stages the pivot
+// rows M_KJ of its columns in LDS, and updates its tile as a register-blocked
+// product: each thread owns 4 rows x 8 columns and per pivot index l reads 4
+// values of N and 8 of M_KJ for 32 FMAs (a thread per column would read one
+// LDS value per FMA, and LDS bandwidth would bound the step).  Blocks are
+// zero-padded to kGJB, so the last, partial pivot block runs the same unrolled
+// loops.  All global loads are issued first from clamped, in-bounds addresses
+// (padding selected afterwards), so they are in flight during the inversion.
+constexpr int kGJB = 32, kGJC = 256, kGJR = 4, kGJM = 8;
+static_assert(kGJB * kGJC == 256 * kGJR * kGJM, "one 4x8 register block per thread");
 __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k0,
                                                   int bk) {
-  __shared__ double P[kGJB * kGJB];
-  __shared__ double T[kGJB * kGJB];
-  __shared__ double MK[kGJB * kGJC];
+  constexpr int kPer = kGJB * kGJB / 256;
+  __shared__ double P[kGJB * kGJB];    // pivot block, then its inverse
+  __shared__ double Nt[kGJB * kGJB];   // N transposed: Nt[l][row]
+  __shared__ double MK[kGJB * kGJC];   // M_KJ; its first kGJB * kGJB entries hold M_IK until N is formed
+  double* MI = MK;
   const int tid = threadIdx.x;
   const int r0 = blockIdx.y * kGJB, c0 = blockIdx.x * kGJC;
   const int nr = min(kGJB, n - r0), nc = min(kGJC, n - c0);
-  const int pi = tid / kGJB, pj = tid % kGJB;
-  const bool inP = pi < bk && pj < bk;
-  if (inP) P[pi * kGJB + pj] = A[(int64_t)(k0 + pi) * n + k0 + pj];
-  for (int idx = tid; idx < bk * kGJC; idx += 256) {
-    const int l = idx / kGJC, cj = idx - l * kGJC;
-    MK[idx] = cj < nc ? A[(int64_t)(k0 + l) * n + c0 + cj] : 0.0;
+  const bool pivot_rows = r0 == k0;
+  // loads: pivot block and M_IK entries, this thread's column of M_KJ, its 4x8 tile block
+  double pv[kPer], mv[kPer], mk[kGJB], acc[kGJR][kGJM];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * 256, i = idx / kGJB, jj = min(idx % kGJB, bk - 1);
+    pv[q] = A[(int64_t)(k0 + min(i, bk - 1)) * n + k0 + jj];
+    mv[q] = A[(int64_t)(r0 + min(i, nr - 1)) * n + k0 + jj];
+  }
+  const int jc = c0 + min(tid, nc - 1);
+#pragma unroll
+  for (int l = 0; l < kGJB; ++l) mk[l] = A[(int64_t)(k0 + min(l, bk - 1)) * n + jc];
+  const int tr = tid / 32, tc = tid % 32;  // rows tr*4 + i, columns tc + 32*m
+#pragma unroll
+  for (int i = 0; i < kGJR; ++i)
+#pragma unroll
+    for (int m = 0; m < kGJM; ++m)
+      acc[i][m] = A[(int64_t)(r0 + min(tr * kGJR + i, nr - 1)) * n + c0 + min(tc + 32 * m, nc - 1)];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * 256, i = idx / kGJB, jj = idx % kGJB;
+    P[idx] = (i < bk && jj < bk) ? pv[q] : 0.0;
+    MI[idx] = (i < nr && jj < bk) ? mv[q] : 0.0;
   }
   __syncthreads();
+  // Gauss-Jordan on the pivot block, ping-ponging between P and Nt (free
+  // until N is formed) so that each pivot takes one barrier
+  double *Pc = P, *Pn = Nt;
   for (int k = 0; k < bk; ++k) {
-    const double piv = P[k * kGJB + k];
-    const double aik = inP ? P[pi * kGJB + k] : 0.0, akj = inP ? P[k * kGJB + pj] : 0.0;
-    const double aij = inP ? P[pi * kGJB + pj] : 0.0;
-    __syncthreads();
-    if (inP) {
+    const double rp = 1.0 / Pc[k * kGJB + k];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = tid + q * 256, pi = idx / kGJB, pj = idx % kGJB;
+      const double aik = Pc[pi * kGJB + k], akj = Pc[k * kGJB + pj] * rp;
       double v;
-      if (pi == k && pj == k) v = 1.0 / piv;
-      else if (pi == k) v = akj / piv;
-      else if (pj == k) v = -aik / piv;
-      else v = aij - aik * akj / piv;
-      P[pi * kGJB + pj] = v;
+      if (pi == k) v = pj == k ? rp : akj;
+      else if (pj == k) v = -aik * rp;
+      else v = Pc[idx] - aik * akj;
+      Pn[idx] = v;
     }
     __syncthreads();
+    double* t = Pc;
+    Pc = Pn;
+    Pn = t;
   }
-  if (pi < nr && pj < bk) {
-    const int64_t row = r0 + pi;
+  if (Pc != P) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) P[tid + q * 256] = Pc[tid + q * 256];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * 256, ri = idx / kGJB, pj = idx % kGJB;
     double t = 0.0;
-    for (int l = 0; l < bk; ++l) t += A[row * n + k0 + l] * P[l * kGJB + pj];
-    T[pi * kGJB + pj] = t;
+#pragma unroll
+    for (int l = 0; l < kGJB; ++l) t += MI[ri * kGJB + l] * P[l * kGJB + pj];
+    Nt[pj * kGJB + ri] = pivot_rows ? P[idx] : -t;
   }
+  __syncthreads();  // M_IK is consumed: its space takes M_KJ
+#pragma unroll
+  for (int l = 0; l < kGJB; ++l) MK[l * kGJC + tid] = (tid < nc && l < bk) ? mk[l] : 0.0;
   __syncthreads();
-  const int cj = tid;  // one column of the tile per thread, rows in turn
-  if (cj >= nc) return;
-  const int j = c0 + cj;
-  const bool ck = j >= k0 && j < k0 + bk;
-  for (int ri = 0; ri < nr; ++ri) {
-    const int row = r0 + ri;
-    const bool rk = row >= k0 && row < k0 + bk;
-    double v;
-    if (rk && ck) {
-      v = P[(row - k0) * kGJB + (j - k0)];
-    } else if (rk) {
-      v = 0.0;
-      for (int l = 0; l < bk; ++l) v += P[(row - k0) * kGJB + l] * MK[l * kGJC + cj];
-    } else if (ck) {
-      v = -T[ri * kGJB + (j - k0)];
-    } else {
-      v = A[(int64_t)row * n + j];
-      for (int l = 0; l < bk; ++l) v -= T[ri * kGJB + l] * MK[l * kGJC + cj];
+#pragma unroll
+  for (int i = 0; i < kGJR; ++i)
+#pragma unroll
+    for (int m = 0; m < kGJM; ++m)
+      if (pivot_rows) acc[i][m] = 0.0;
+#pragma unroll 4
+  for (int l = 0; l < kGJB; ++l) {
+    double nv[kGJR], kv[kGJM];
+#pragma unroll
+    for (int i = 0; i < kGJR; ++i) nv[i] = Nt[l * kGJB + tr * kGJR + i];
+#pragma unroll
+    for (int m = 0; m < kGJM; ++m) kv[m] = MK[l * kGJC + tc + 32 * m];
+#pragma unroll
+    for (int i = 0; i < kGJR; ++i)
+#pragma unroll
+      for (int m = 0; m < kGJM; ++m) acc[i][m] += nv[i] * kv[m];
+  }
+#pragma unroll
+  for (int i = 0; i < kGJR; ++i) {
+    const int ri = tr * kGJR + i;
+    if (ri >= nr) continue;
+#pragma unroll
+    for (int m = 0; m < kGJM; ++m) {
+      const int cj = tc + 32 * m, j = c0 + cj;
+      if (cj >= nc) continue;
+      // pivot columns: -M_IK P, or P itself on the pivot rows
+      const double v = (j >= k0 && j < k0 + bk) ? Nt[(j - k0) * kGJB + ri] : acc[i][m];
+      B[(int64_t)(r0 + ri) * n + j] = v;
     }
-    B[(int64_t)row * n + j] = v;
   }
 }
 

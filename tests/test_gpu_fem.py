@@ -103,16 +103,23 @@ def test_solve_matches_oracle(torch, shape, inactive, seed, rtol):
         assert abs(c - cref) <= 1e2 * rtol * abs(cref) + 1e-300
 
 
-def test_solve_dense_bruteforce(torch):
-    """Tiny grid: against numpy.linalg.solve of the assembled matrix."""
-    g, active, fixed, Ee = random_problem((4, 2, 2), 3, 0.0, "clamp")
+@pytest.mark.parametrize("shape,inactive", [((4, 2, 2), 0.0), ((8, 4, 4), 0.15)])
+def test_solve_dense_bruteforce(torch, shape, inactive):
+    """Single-level grids (135 and 675 dofs: one and three column blocks of the Gauss-Jordan
+    inverse, the last pivot block partial): against numpy.linalg.solve of the assembled
+    matrix.  The dense inverse is the whole preconditioner here, so PCG needs only the few
+    iterations that FP32 vectors take to reach 1e-12; a wrong inverse needs many more."""
+    g, active, fixed, Ee = random_problem(shape, 3, inactive, "clamp")
     ref = oracle_fem(g, active, fixed)
     A = ref.assemble(Ee[ref.act]).toarray()
     f = np.random.default_rng(1).standard_normal(ref.nfree)
     exact = np.linalg.solve(A, f)
     fem = _product_fem(g, active, fixed)
+    assert fem.levels == 1
     u = fem.solve(fem.assemble(Ee[fem.act]), [f], rtol=1e-12)[0]
+    print(f"dense {shape}: {fem.cg_iters[-1]} PCG iterations", flush=True)
     assert np.abs(u - exact).max() <= 1e-8 * np.abs(exact).max()
+    assert fem.cg_iters[-1] <= 8
 
 
 def test_warm_start_and_errors(torch):
