@@ -27,7 +27,10 @@ struct FilterArgs {
   double* out;
   double* out2;
   double* part;
-  const double* oc;
+  const double* oc;       // kOc: bisection state before this launch's step
+  double* oc_out;         // kOc: state after it (written by workgroup 0)
+  const double* part_in;  // kOc with prev: volume partials of the previous evaluation, not yet in oc
+  int prev;
   double wc[13];  // hat weight max(0, r - sqrt(d2)) for squared distance d2 = 0..3R^2
   double beta, move;
   int nx, ny, nz;
@@ -62,6 +65,104 @@ __device__ __forceinline__ void project(double xt, double beta, double& xp, doub
 // epilogue operands of a layer one step ahead.  The summation order of every
 // output is fixed (input layers in +z; rings in order; taps in dy, dx order).
 
+// Sum the volume partials in a fixed order, record the evaluation in the
+// bracket, replay every bisection trial it decides, and choose the next
+// multiplier to evaluate.
+constexpr double kOcEps = 0.02;          // default half-width of the warm bracket around a hint
+constexpr double kOcSecantWidth = 0.25;  // log-width below which the root is interpolated
+
+__device__ void oc_record(double* oc, double lam, double f) {
+  if (f > 0.0) {
+    if (oc[kOcHasA] == 0.0 || lam > oc[kOcA]) {
+      oc[kOcA] = lam;
+      oc[kOcFA] = f;
+      oc[kOcHasA] = 1.0;
+      if (oc[kOcSide] > 0.0 && oc[kOcHasB] != 0.0) oc[kOcFB] *= 0.5;  // Illinois
+    }
+    oc[kOcSide] = 1.0;
+  } else {
+    if (oc[kOcHasB] == 0.0 || lam < oc[kOcB]) {
+      oc[kOcB] = lam;
+      oc[kOcFB] = f;
+      oc[kOcHasB] = 1.0;
+      if (oc[kOcSide] < 0.0 && oc[kOcHasA] != 0.0) oc[kOcFA] *= 0.5;
+    }
+    oc[kOcSide] = -1.0;
+  }
+}
+
+// Fixed-order sum of nblk volume partials over a 256-thread workgroup (thread
+// 0 holds the result): every caller gets the same bits from the same partials.
+__device__ double oc_volume(const double* __restrict__ part, int nblk, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// One bisection step for the evaluation vol = vol(oc[kOcLam]) (one thread).
+__device__ void oc_step(double* oc, double vol) {
+  oc[kOcVol] = vol;
+  oc[kOcEvals] += 1.0;
+  oc_record(oc, oc[kOcLam], vol - oc[kOcVt]);
+  double lo = oc[kOcLo], hi = oc[kOcHi];
+  for (int guard = 0; guard < 200; ++guard) {
+    const double lam = sqrt(lo * hi);  // the reference's next trial
+    int dec = 0;                        // +1: vol(lam) > vt, -1: vol(lam) <= vt
+    if (oc[kOcHasA] != 0.0 && lam <= oc[kOcA]) dec = 1;
+    else if (oc[kOcHasB] != 0.0 && lam >= oc[kOcB]) dec = -1;
+    if (dec != 0) {
+      if (dec > 0) lo = lam;
+      else hi = lam;
+      oc[kOcLamLast] = lam;
+      oc[kOcSteps] += 1.0;
+      oc[kOcTries] = 0.0;
+      if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) {
+        oc[kOcDone] = 1.0;
+        break;
+      }
+      continue;
+    }
+    // undecided: pick the next multiplier to evaluate
+    double next = lam;
+    const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
+    const double stage = oc[kOcStage];
+    if (stage == 1.0) {
+      // second point of the warm bracket around the hint
+      const double c = oc[kOcHint] * (1.0 + oc[kOcWidth]);
+      if (c > lo && c < hi) next = c;
+      oc[kOcStage] = 2.0;
+    } else if (stage == 2.0 && (hasA != hasB)) {
+      // one side known: step outward geometrically from it
+      const double g = oc[kOcGrow];
+      const double c = hasA ? oc[kOcA] * g : oc[kOcB] / g;
+      if (c > lo && c < hi) next = c;
+      else oc[kOcStage] = 3.0;
+      oc[kOcGrow] = g * g;
+    } else if ((hasA != hasB) && oc[kOcSteps] >= 4.0 && oc[kOcEnds] == 0.0) {
+      // still one-sided after several trials: test the far end of the initial
+      // bracket; if it is on the same side, every remaining trial is decided
+      next = hasA ? oc[kOcHi0] : oc[kOcLo0];
+      oc[kOcEnds] = 1.0;
+    } else if (hasA && hasB && log(oc[kOcB] / oc[kOcA]) < kOcSecantWidth && oc[kOcTries] < 3.0) {
+      // narrow bracket: Illinois (regula falsi) step in log(lam)
+      const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
+      const double fa = oc[kOcFA], fb = oc[kOcFB];
+      double t = ta - fa * (tb - ta) / (fb - fa);
+      if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
+      const double c = exp(t);
+      if (c > oc[kOcA] && c < oc[kOcB]) next = c;
+      oc[kOcTries] += 1.0;
+    }
+    oc[kOcLam] = next;
+    break;
+  }
+  oc[kOcLo] = lo;
+  oc[kOcHi] = hi;
+}
+
 // number of (dy', dx') offsets in [-R, R]^2 with dy'^2 + dx'^2 = m
 __host__ __device__ constexpr int ring_taps(int R, int m) {
   int n = 0;
@@ -77,10 +178,29 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   constexpr int NS = FZC + 2 * R;          // input layers per chunk
   __shared__ double buf[2][NP];
   __shared__ double sred[NT / 64];
-  if (MODE == kOc) {
-    if (a.oc[kOcDone] != 0.0) return;
-  }
   const int tid = threadIdx.x;
+  double rlam = 0.0, mv = 0.0;
+  if (MODE == kOc) {
+    // the bisection step of the previous evaluation (identical in every
+    // workgroup), then this launch's trial multiplier
+    static_assert(NT == 256, "oc_volume sums over 256 threads");
+    __shared__ double st[kOcLen];
+    if (a.prev && a.oc[kOcDone] == 0.0) {
+      const double vol = oc_volume(a.part_in, gridDim.x, sred);
+      if (tid == 0) {
+        for (int i = 0; i < kOcLen; ++i) st[i] = a.oc[i];
+        oc_step(st, vol);
+      }
+    } else if (tid < kOcLen) {
+      st[tid] = a.oc[tid];
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid < kOcLen) a.oc_out[tid] = st[tid];
+    if (st[kOcDone] != 0.0) return;
+    rlam = 1.0 / sqrt(st[kOcLam]);
+    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
+    __syncthreads();  // sred is reused by the volume sum at the end
+  }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
   int bid = xcd_logical(blockIdx.x, gridDim.x);  // neighbouring tiles share one XCD's L2
@@ -90,11 +210,6 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   const int bz = bid / nty;
   const int X0 = bx * FTX, Y0 = by * FTY, Z0 = bz * FZC;
   const int64_t exy = (int64_t)nx * ny;
-  double rlam = 0.0, mv = 0.0;
-  if (MODE == kOc) {
-    rlam = 1.0 / sqrt(a.oc[kOcLam]);
-    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
-  }
 
   // this thread's staging slots: in-plane element offset, or -1 outside the grid
   int soff[NLD];
@@ -249,99 +364,12 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   }
 }
 
-// Sum the volume partials in a fixed order, record the evaluation in the
-// bracket, replay every bisection trial it decides, and choose the next
-// multiplier to evaluate.
-constexpr double kOcEps = 0.02;          // default half-width of the warm bracket around a hint
-constexpr double kOcSecantWidth = 0.25;  // log-width below which the root is interpolated
-
-__device__ void oc_record(double* oc, double lam, double f) {
-  if (f > 0.0) {
-    if (oc[kOcHasA] == 0.0 || lam > oc[kOcA]) {
-      oc[kOcA] = lam;
-      oc[kOcFA] = f;
-      oc[kOcHasA] = 1.0;
-      if (oc[kOcSide] > 0.0 && oc[kOcHasB] != 0.0) oc[kOcFB] *= 0.5;  // Illinois
-    }
-    oc[kOcSide] = 1.0;
-  } else {
-    if (oc[kOcHasB] == 0.0 || lam < oc[kOcB]) {
-      oc[kOcB] = lam;
-      oc[kOcFB] = f;
-      oc[kOcHasB] = 1.0;
-      if (oc[kOcSide] < 0.0 && oc[kOcHasA] != 0.0) oc[kOcFA] *= 0.5;
-    }
-    oc[kOcSide] = -1.0;
-  }
-}
-
+// The last evaluation of a launch batch: its step, in place (one workgroup).
 __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc) {
   if (oc[kOcDone] != 0.0) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) s += part[i];
   __shared__ double sh[4];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double vol = sh[0] + sh[1] + sh[2] + sh[3];
-  oc[kOcVol] = vol;
-  oc[kOcEvals] += 1.0;
-  oc_record(oc, oc[kOcLam], vol - oc[kOcVt]);
-  double lo = oc[kOcLo], hi = oc[kOcHi];
-  for (int guard = 0; guard < 200; ++guard) {
-    const double lam = sqrt(lo * hi);  // the reference's next trial
-    int dec = 0;                        // +1: vol(lam) > vt, -1: vol(lam) <= vt
-    if (oc[kOcHasA] != 0.0 && lam <= oc[kOcA]) dec = 1;
-    else if (oc[kOcHasB] != 0.0 && lam >= oc[kOcB]) dec = -1;
-    if (dec != 0) {
-      if (dec > 0) lo = lam;
-      else hi = lam;
-      oc[kOcLamLast] = lam;
-      oc[kOcSteps] += 1.0;
-      oc[kOcTries] = 0.0;
-      if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) {
-        oc[kOcDone] = 1.0;
-        break;
-      }
-      continue;
-    }
-    // undecided: pick the next multiplier to evaluate
-    double next = lam;
-    const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
-    const double stage = oc[kOcStage];
-    if (stage == 1.0) {
-      // second point of the warm bracket around the hint
-      const double c = oc[kOcHint] * (1.0 + oc[kOcWidth]);
-      if (c > lo && c < hi) next = c;
-      oc[kOcStage] = 2.0;
-    } else if (stage == 2.0 && (hasA != hasB)) {
-      // one side known: step outward geometrically from it
-      const double g = oc[kOcGrow];
-      const double c = hasA ? oc[kOcA] * g : oc[kOcB] / g;
-      if (c > lo && c < hi) next = c;
-      else oc[kOcStage] = 3.0;
-      oc[kOcGrow] = g * g;
-    } else if ((hasA != hasB) && oc[kOcSteps] >= 4.0 && oc[kOcEnds] == 0.0) {
-      // still one-sided after several trials: test the far end of the initial
-      // bracket; if it is on the same side, every remaining trial is decided
-      next = hasA ? oc[kOcHi0] : oc[kOcLo0];
-      oc[kOcEnds] = 1.0;
-    } else if (hasA && hasB && log(oc[kOcB] / oc[kOcA]) < kOcSecantWidth && oc[kOcTries] < 3.0) {
-      // narrow bracket: Illinois (regula falsi) step in log(lam)
-      const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
-      const double fa = oc[kOcFA], fb = oc[kOcFB];
-      double t = ta - fa * (tb - ta) / (fb - fa);
-      if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
-      const double c = exp(t);
-      if (c > oc[kOcA] && c < oc[kOcB]) next = c;
-      oc[kOcTries] += 1.0;
-    }
-    oc[kOcLam] = next;
-    break;
-  }
-  oc[kOcLo] = lo;
-  oc[kOcHi] = hi;
+  const double vol = oc_volume(part, nblk, sh);
+  if (threadIdx.x == 0) oc_step(oc, vol);
 }
 
 // sqrt(max(-dcx, 0) / dvx) per element, so a trial needs one multiply by
@@ -422,8 +450,8 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&Hs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&r0_, n * sizeof(double)));
-  ANNEAL_HIP(hipMalloc(&part_, std::max(nblk_, 6 * 512) * sizeof(double)));
-  ANNEAL_HIP(hipMalloc(&oc_, kOcLen * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&part_, std::max(2 * nblk_, 6 * 512) * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&oc_, 2 * kOcLen * sizeof(double)));
   ANNEAL_HIP(hipHostMalloc((void**)&host_, 2 * kOcLen * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipEventCreateWithFlags(&oc_ready_, hipEventDisableTiming));
   ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
@@ -521,7 +549,7 @@ void FilterContext::oc_begin(const double* x, const double* dcx, const double* d
   const int64_t ne = g_.nelem();
   hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_);
   ANNEAL_HIP(hipGetLastError());
-  run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true};
+  run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true, oc_};
   oc_launch(12);
 }
 
@@ -533,21 +561,34 @@ void FilterContext::oc_launch(int count) {
   a.flags = flags_;
   a.invHs = invHs_;
   a.phi = run_.phi;
-  a.part = part_;
-  a.oc = oc_;
   a.beta = run_.beta;
   a.move = run_.move;
   std::copy(w_, w_ + 13, a.wc);
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
-  // evaluations launched after the stop are no-ops that read the device flag
-  for (int i = 0; i < count && run_.launched < 240; ++i, ++run_.launched) {
+  // Evaluation e reads the state in oc_[e % 2], first takes the step of
+  // evaluation e - 1 from the partials in part_[(e - 1) % 2] (except the first
+  // of a batch, whose predecessor the batch finisher took), evaluates and
+  // writes its partials to part_[e % 2] and the state to oc_[(e + 1) % 2].
+  // Evaluations launched after the stop only carry the state forward.
+  int batch = 0;
+  for (; batch < count && run_.launched < 240; ++batch, ++run_.launched) {
+    const int e = run_.launched;
+    a.oc = oc_ + (e % 2) * kOcLen;
+    a.oc_out = oc_ + ((e + 1) % 2) * kOcLen;
+    a.part = part_ + (e % 2) * nblk_;
+    a.part_in = part_ + ((e + 1) % 2) * nblk_;
+    a.prev = batch > 0;
     launch_filter(kOc, R_, a, nblk_, s_);
-    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_, nblk_, oc_);
-    ANNEAL_HIP(hipGetLastError());
   }
-  ANNEAL_HIP(hipMemcpyAsync(host_, oc_, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (batch == 0) return;
+  // the batch finisher: the last evaluation's step, in place
+  const int e = run_.launched;
+  run_.state = oc_ + (e % 2) * kOcLen;
+  hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_ + ((e + 1) % 2) * nblk_, nblk_, run_.state);
+  ANNEAL_HIP(hipGetLastError());
+  ANNEAL_HIP(hipMemcpyAsync(host_, run_.state, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
   ANNEAL_HIP(hipEventRecord(oc_ready_, s_));
 }
 
@@ -568,7 +609,7 @@ int FilterContext::oc_end(int* evals, double* lam_last) {
   run_.active = false;
   const int64_t n = g_.nelem();
   hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, run_.x, run_.dcx, run_.dvx,
-                     flags_, oc_, run_.beta, run_.move, n, run_.xnew);
+                     flags_, run_.state, run_.beta, run_.move, n, run_.xnew);
   ANNEAL_HIP(hipGetLastError());
   if (evals) *evals = (int)host_[kOcEvals];
   if (lam_last) *lam_last = host_[kOcLamLast];

@@ -75,6 +75,7 @@ class FilterContext {
     double beta, move;
     int launched;
     bool active;
+    double* state;  // the state after the last batch (one of the two buffers in oc_)
   } run_{};
   hipEvent_t oc_ready_ = nullptr;  // recorded after each state read-back
   void oc_launch(int count);
