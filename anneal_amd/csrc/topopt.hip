@@ -94,8 +94,16 @@ __device__ void oc_record(double* oc, double lam, double f) {
 // Fixed-order sum of nblk volume partials over a 256-thread workgroup (thread
 // 0 holds the result): every caller gets the same bits from the same partials.
 __device__ double oc_volume(const double* __restrict__ part, int nblk, double* sh) {
+  // partials threadIdx.x + 256 i in increasing i; four loads in flight at a time
+  // (clamped addresses, the padding added as 0)
   double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) s += part[i];
+  for (int i0 = threadIdx.x; i0 < nblk; i0 += 4 * 256) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = part[min(i0 + u * 256, nblk - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += i0 + u * 256 < nblk ? v[u] : 0.0;
+  }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -180,27 +188,6 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   __shared__ double sred[NT / 64];
   const int tid = threadIdx.x;
   double rlam = 0.0, mv = 0.0;
-  if (MODE == kOc) {
-    // the bisection step of the previous evaluation (identical in every
-    // workgroup), then this launch's trial multiplier
-    static_assert(NT == 256, "oc_volume sums over 256 threads");
-    __shared__ double st[kOcLen];
-    if (a.prev && a.oc[kOcDone] == 0.0) {
-      const double vol = oc_volume(a.part_in, gridDim.x, sred);
-      if (tid == 0) {
-        for (int i = 0; i < kOcLen; ++i) st[i] = a.oc[i];
-        oc_step(st, vol);
-      }
-    } else if (tid < kOcLen) {
-      st[tid] = a.oc[tid];
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && tid < kOcLen) a.oc_out[tid] = st[tid];
-    if (st[kOcDone] != 0.0) return;
-    rlam = 1.0 / sqrt(st[kOcLam]);
-    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
-    __syncthreads();  // sred is reused by the volume sum at the end
-  }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
   int bid = xcd_logical(blockIdx.x, gridDim.x);  // neighbouring tiles share one XCD's L2
@@ -282,6 +269,28 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   fetch(Z0 - R, 0);
   fetch(Z0 - R + 1, 1);
   Epi ep[2] = {};
+  if (MODE == kOc) {
+    // the bisection step of the previous evaluation (identical in every
+    // workgroup; the first two layers' loads are in flight meanwhile), then
+    // this launch's trial multiplier
+    static_assert(NT == 256, "oc_volume sums over 256 threads");
+    __shared__ double st[kOcLen];
+    if (a.prev && a.oc[kOcDone] == 0.0) {
+      const double vol = oc_volume(a.part_in, gridDim.x, sred);
+      if (tid == 0) {
+        for (int i = 0; i < kOcLen; ++i) st[i] = a.oc[i];
+        oc_step(st, vol);
+      }
+    } else if (tid < kOcLen) {
+      st[tid] = a.oc[tid];
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid < kOcLen) a.oc_out[tid] = st[tid];
+    if (st[kOcDone] != 0.0) return;
+    rlam = 1.0 / sqrt(st[kOcLam]);
+    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
+    __syncthreads();  // sred is reused by the volume sum at the end
+  }
 
   // one input layer; SL = s & 1 selects the LDS buffer and the prefetch slots
   auto step = [&](int s, auto slc) {
