@@ -69,8 +69,8 @@ void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
   af.skip = f.st_;
   launch_apply(af, kApply, nrhs, f.s_);
   finalize_st(f.part_, nblk_apply, nrhs, f.st_, kPQ, 1, f.s_);
-  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_, f.st_);
-  ANNEAL_HIP(hipMemcpyAsync(f.host_red_ + 64, f.red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, f.s_));
+  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_, f.st_,
+             f.host_red_dev_ + 64);
   // the cycle's last fine smoothing kernel forms the <r, z> partials (a
   // one-level hierarchy is a dense solve only: separate dot)
   const bool fused = f.lv_.size() > 1;
@@ -131,8 +131,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
     if (!f.coarse_valid_ || (f.coarse_stale_ && f.rebuild_each_solve)) f.rebuild_coarse();
     const GridDims& g0 = f.lv_[0].g;
     if (!use_x0) ANNEAL_HIP(hipMemsetAsync(U[p], 0, (size_t)nr * 3 * g0.ldn() * sizeof(double), s_));
-    dot(F[p], F[p], g0.nnode(), g0.ldn(), nr, f.part_, f.red_, s_);
-    ANNEAL_HIP(hipMemcpyAsync(f.host_red_, f.red_, nr * sizeof(double), hipMemcpyDeviceToHost, s_));
+    dot(F[p], F[p], g0.nnode(), g0.ldn(), nr, f.part_, f.red_, s_, f.host_red_dev_);
     St& st = S[p];
     st.fnorm.assign(nr, 0.0);
     st.rnorm.assign(nr, 0.0);
@@ -159,8 +158,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       a.y = f.r64_;
       a.dot_part = f.part_;  // |r|^2 partials from the residual kernel itself
       launch_apply(a, kResid, nrhs[p], s_);
-      finalize(f.part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs[p], f.red_, s_);
-      ANNEAL_HIP(hipMemcpyAsync(f.host_red_, f.red_, nrhs[p] * sizeof(double), hipMemcpyDeviceToHost, s_));
+      finalize(f.part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs[p], f.red_, s_, f.host_red_dev_);
     }
     ANNEAL_HIP(hipStreamSynchronize(s_));
     std::vector<int> active;

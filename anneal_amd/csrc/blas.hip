@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(const TA* __restric
 
 // Sum nblk partials per rhs in a fixed order; out[r] = sum (or out[r] = scale*sum).
 __global__ __launch_bounds__(kRedThreads) void k_finalize(const double* __restrict__ part, int nblk,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out, double* __restrict__ out_host) {
   const int r = blockIdx.x;
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += kRedThreads) s += part[(int64_t)r * nblk + i];
@@ -48,22 +48,26 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(const double* __restri
     double t = 0.0;
     for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
     out[r] = t;
+    if (out_host) out_host[r] = t;  // pinned host memory, read after the caller's wait
   }
 }
 
 template <typename TA, typename TB>
-void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s) {
+void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s,
+         double* out_host) {
   hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, a, b, n, ld, part,
                      (const double*)nullptr);
   ANNEAL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out);
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out, out_host);
   ANNEAL_HIP(hipGetLastError());
 }
-template void dot<float, float>(const float*, const float*, int64_t, int64_t, int, double*, double*, hipStream_t);
-template void dot<double, double>(const double*, const double*, int64_t, int64_t, int, double*, double*, hipStream_t);
+template void dot<float, float>(const float*, const float*, int64_t, int64_t, int, double*, double*, hipStream_t,
+                                double*);
+template void dot<double, double>(const double*, const double*, int64_t, int64_t, int, double*, double*, hipStream_t,
+                                  double*);
 
-void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, nblk, out);
+void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s, double* out_host) {
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, nblk, out, out_host);
   ANNEAL_HIP(hipGetLastError());
 }
 
@@ -384,10 +388,10 @@ __global__ __launch_bounds__(kRedThreads) void k_pcg_update(float* __restrict__ 
   }
 }
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
-                double* part, double* out_rr, hipStream_t s, const double* skip) {
+                double* part, double* out_rr, hipStream_t s, const double* skip, double* out_host) {
   hipLaunchKernelGGL(k_pcg_update, dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, x, r, p, q, st, n, ld, part, skip);
   ANNEAL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out_rr);
+  hipLaunchKernelGGL(k_finalize, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, out_rr, out_host);
   ANNEAL_HIP(hipGetLastError());
 }
 

@@ -41,9 +41,12 @@ template <typename T>
 void launch_prolong_add(const T* xc, T* xf, const uint8_t* maskf, const GridDims& f, const GridDims& c, int nrhs,
                         hipStream_t s, const OpEntry* ops = nullptr, int rpo = 1, const double* skip = nullptr);
 
+// Reductions into out[r] (device); out_host, when set, is a device alias of
+// pinned host memory that receives the same values (read after a wait on s).
 template <typename TA, typename TB>
-void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s);
-void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s);
+void dot(const TA* a, const TB* b, int64_t n, int64_t ld, int nrhs, double* part, double* out, hipStream_t s,
+         double* out_host = nullptr);
+void finalize(const double* part, int nblk, int nrhs, double* out, hipStream_t s, double* out_host = nullptr);
 void jacobi0(const float* b, float* x, const float* dinv, const uint8_t* mask, float omega, int64_t n, int64_t ld,
              int nrhs, hipStream_t s, const double* skip = nullptr);
 // With ops set, rhs r uses ops[r / rpo].Ainv.
@@ -57,7 +60,8 @@ void finalize_st(const double* part, int nblk, int nrhs, double* st, int field, 
 void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs, double* part, hipStream_t s,
                  const double* skip = nullptr);
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
-                double* part, double* out_rr, hipStream_t s, const double* skip = nullptr);
+                double* part, double* out_rr, hipStream_t s, const double* skip = nullptr,
+                double* out_host = nullptr);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
 void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
 void axpy_f32_to_f64(const float* x, double* y, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);

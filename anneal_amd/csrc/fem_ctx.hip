@@ -274,6 +274,7 @@ void FemContext::alloc_solver(int nrhs_max) {
   red_ = (double*)alloc(4 * 64 * sizeof(double));
   scale_ = (double*)alloc(64 * sizeof(double));
   ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_red_dev_, host_red_, 0));
   ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
   ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   vgraph_.assign(nrhs_max + 1, nullptr);
@@ -764,8 +765,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
   const size_t v64 = (size_t)nrhs * 3 * ld * sizeof(double);
   if (!use_x0) ANNEAL_HIP(hipMemsetAsync(U, 0, v64, s_));
   // |F| is read after the first residual's wait: one host round trip for both
-  dot(F, F, n, ld, nrhs, part_, red_ + 128, s_);
-  ANNEAL_HIP(hipMemcpyAsync(host_red_ + 128, red_ + 128, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+  dot(F, F, n, ld, nrhs, part_, red_ + 128, s_, host_red_dev_ + 128);
   std::vector<double> fnorm(nrhs), rnorm(nrhs), tol2(nrhs), sc(nrhs);
   std::vector<int> its(nrhs, 0);
   std::vector<char> done(nrhs, 0), idone(nrhs, 0);
@@ -782,8 +782,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     a.y = r64_;
     a.dot_part = part_;  // |r|^2 partials from the residual kernel itself
     launch_apply(a, kResid, nrhs, s_);
-    finalize(part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs, red_, s_);
-    ANNEAL_HIP(hipMemcpyAsync(host_red_, red_, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+    finalize(part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs, red_, s_, host_red_dev_);
     ANNEAL_HIP(hipStreamSynchronize(s_));
     if (outer == 0)
       for (int r = 0; r < nrhs; ++r) fnorm[r] = std::sqrt(host_red_[128 + r]);
@@ -835,8 +834,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       af.skip = st_;
       launch_apply(af, kApply, nrhs, s_);
       finalize_st(part_, nblk_apply, nrhs, st_, kPQ, 1, s_);  // pq, alpha
-      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_);
-      ANNEAL_HIP(hipMemcpyAsync(host_red_ + 64, red_ + 64, nrhs * sizeof(double), hipMemcpyDeviceToHost, s_));
+      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
       ANNEAL_HIP(hipEventRecord(ev_, s_));
       bool speculate = it >= 2;
       for (int r = 0; r < nrhs && speculate; ++r) {

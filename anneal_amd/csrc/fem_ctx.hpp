@@ -194,7 +194,8 @@ class FemContext {
   float *px_ = nullptr, *pr_ = nullptr, *pz_ = nullptr, *pp_ = nullptr, *pq_ = nullptr;
   double *x64_ = nullptr, *r64_ = nullptr;
   double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
-  double* host_red_ = nullptr;  // pinned
+  double* host_red_ = nullptr;      // pinned
+  double* host_red_dev_ = nullptr;  // its device alias: reductions are stored there directly
   bool tuned_ = false;
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
