@@ -374,17 +374,24 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
 }
 
 // The last evaluation of a launch batch: its step, in place (one workgroup).
-__global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc) {
-  if (oc[kOcDone] != 0.0) return;
+// The state also goes to out_host (device alias of pinned host memory).
+__global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc,
+                                                   double* __restrict__ out_host) {
   __shared__ double sh[4];
-  const double vol = oc_volume(part, nblk, sh);
-  if (threadIdx.x == 0) oc_step(oc, vol);
+  if (oc[kOcDone] == 0.0) {
+    const double vol = oc_volume(part, nblk, sh);
+    if (threadIdx.x == 0) oc_step(oc, vol);
+    __syncthreads();
+  }
+  if (threadIdx.x < kOcLen) out_host[threadIdx.x] = oc[threadIdx.x];
 }
 
 // sqrt(max(-dcx, 0) / dvx) per element, so a trial needs one multiply by
 // lam^-1/2 (dvx = 0 with -dcx > 0 gives +inf, which the move limit clips).
+// Block 0 also brings the initial bisection state in from pinned host memory.
 __global__ void k_oc_prep(const double* __restrict__ dcx, const double* __restrict__ dvx, int64_t n,
-                          double* __restrict__ r0) {
+                          double* __restrict__ r0, const double* __restrict__ st_host, double* __restrict__ st) {
+  if (blockIdx.x == 0 && threadIdx.x < kOcLen) st[threadIdx.x] = st_host[threadIdx.x];
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double c = fmax(-dcx[e], 0.0);
@@ -462,6 +469,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&part_, std::max(2 * nblk_, 6 * 512) * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&oc_, 2 * kOcLen * sizeof(double)));
   ANNEAL_HIP(hipHostMalloc((void**)&host_, 2 * kOcLen * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_dev_, host_, 0));
   ANNEAL_HIP(hipEventCreateWithFlags(&oc_ready_, hipEventDisableTiming));
   ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
   FilterArgs a{};
@@ -538,8 +546,8 @@ void FilterContext::oc_begin(const double* x, const double* dcx, const double* d
                              double move, double lo, double hi, double vtarget, double* xnew, double hint,
                              double width) {
   const double eps = width > 0.0 ? std::min(width, 0.5) : kOcEps;
-  // the initial state goes up from the pinned upper half of host_: the copy may
-  // still be queued when this call returns
+  // the initial state goes up from the pinned upper half of host_ (k_oc_prep
+  // reads it, possibly after this call returns)
   double* st = host_ + kOcLen;
   std::fill(st, st + kOcLen, 0.0);
   st[kOcLo] = lo;
@@ -554,9 +562,9 @@ void FilterContext::oc_begin(const double* x, const double* dcx, const double* d
   st[kOcVt] = vtarget;
   // outward steps after a miss start a few bracket widths away and square each time
   st[kOcGrow] = 1.0 + std::min(0.25, std::max(0.02, 10.0 * eps));
-  ANNEAL_HIP(hipMemcpyAsync(oc_, st, kOcLen * sizeof(double), hipMemcpyHostToDevice, s_));
   const int64_t ne = g_.nelem();
-  hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_);
+  hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_,
+                     host_dev_ + kOcLen, oc_);
   ANNEAL_HIP(hipGetLastError());
   run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true, oc_};
   oc_launch(12);
@@ -595,9 +603,9 @@ void FilterContext::oc_launch(int count) {
   // the batch finisher: the last evaluation's step, in place
   const int e = run_.launched;
   run_.state = oc_ + (e % 2) * kOcLen;
-  hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_ + ((e + 1) % 2) * nblk_, nblk_, run_.state);
+  hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(256), 0, s_, part_ + ((e + 1) % 2) * nblk_, nblk_, run_.state,
+                     host_dev_);
   ANNEAL_HIP(hipGetLastError());
-  ANNEAL_HIP(hipMemcpyAsync(host_, run_.state, kOcLen * sizeof(double), hipMemcpyDeviceToHost, s_));
   ANNEAL_HIP(hipEventRecord(oc_ready_, s_));
 }
 

@@ -66,6 +66,7 @@ class FilterContext {
   hipStream_t s_;
   uint8_t* flags_ = nullptr;
   double *Hs_ = nullptr, *invHs_ = nullptr, *r0_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
+  double* host_dev_ = nullptr;  // device alias of host_ (pinned): state read-back and upload without copies
   int nblk_ = 0;
   // the bisection in flight (oc_begin .. oc_end)
   struct OcRun {
