@@ -176,7 +176,9 @@ class BatchSimp:
 
     Each dict holds grid, fill, active, frozen, fixed, loads, weights, E0, nu,
     vf and optional keyword arguments of SimpRun (iters, r_mm, tol, emin,
-    rtol, ...).  All runs must use the same rtol.
+    rtol, ...).  All runs must use the same rtol.  A run leaves the batch when
+    it has done its own ``iters`` or met its stopping test (optimize.py:173-175),
+    so each run's history and design equal those of its own ``simp()``.
     """
 
     def __init__(self, problems, log=lambda m: None):
@@ -192,23 +194,42 @@ class BatchSimp:
             raise ValueError("all problems of a batch must use the same rtol")
         self.rtol = rt.pop()
         self.maxiter = max(r.maxiter for r in self.runs)
-        self.solver = BatchSolve([r.fem for r in self.runs], group_rhs=max(r.ncase for r in self.runs))
+        self.solver = None
+        self._members = None
+        self._solver_for(self.runs)
         self.it = 0
 
+    def active(self):
+        """The runs that still iterate: fewer than their iters done and not converged."""
+        return [r for r in self.runs if not r.converged and r.it < r.iters]
+
+    def _solver_for(self, runs):
+        """The batch solver over the FEMs of ``runs``, rebuilt when a run has left the batch."""
+        key = tuple(id(r) for r in runs)
+        if key != self._members:
+            if self.solver is not None:
+                self.solver.close()
+            self.solver = BatchSolve([r.fem for r in runs], group_rhs=max(r.ncase for r in runs))
+            self._members = key
+        return self.solver
+
     def step(self):
-        """One SIMP iteration of every problem; returns their history entries."""
-        for r in self.runs:
+        """One SIMP iteration of every active run; returns one history entry per run (None for runs that
+        have finished)."""
+        runs = self.active()
+        if not runs:
+            return [None] * len(self.runs)
+        for r in runs:
             r._pre_solve()
-        use_x0 = all(r.have_u for r in self.runs)   # every run starts cold on the first iteration
-        its = self.solver.solve([r.F for r in self.runs], [r.U for r in self.runs], [r.ncase for r in self.runs],
-                                self.rtol, self.maxiter, use_x0=use_x0)
+        use_x0 = all(r.have_u for r in runs)   # every run starts cold on the first iteration
+        its = self._solver_for(runs).solve([r.F for r in runs], [r.U for r in runs], [r.ncase for r in runs],
+                                           self.rtol, self.maxiter, use_x0=use_x0)
         self.it += 1
         # the density side of every run in phases, one host wait per phase for the whole batch;
         # the launches are those of SimpRun.step
-        runs = self.runs
         for r, i in zip(runs, its):
             r._sensitivities(i)
-        self._sync()
+        self._sync(runs)
         for r in runs:
             r._oc_begin()
         pending = runs
@@ -216,19 +237,26 @@ class BatchSimp:
             pending = [r for r in pending if not r.filt.oc_poll()]
         for r in runs:
             r._oc_end()
-        self._sync()
-        return [r._record() for r in runs]
+        self._sync(runs)
+        entries = {id(r): r._record() for r in runs}
+        return [entries.get(id(r)) for r in self.runs]
 
-    def _sync(self):
-        for st in {r.stream.value: r._torch_stream for r in self.runs}.values():
+    @staticmethod
+    def _sync(runs):
+        for st in {r.stream.value: r._torch_stream for r in runs}.values():
             st.synchronize()
 
-    def run(self, iters):
-        for _ in range(iters):
+    def run(self, iters=None):
+        """Step until every run has finished (or for at most ``iters`` batch iterations); returns each
+        run's ``simp()`` result."""
+        n = 0
+        while self.active() and (iters is None or n < iters):
             self.step()
+            n += 1
         return [r.result() for r in self.runs]
 
     def close(self):
-        self.solver.close()
+        if self.solver is not None:
+            self.solver.close()
         for r in self.runs:
             r.fem.close()

@@ -27,6 +27,9 @@ EMIN = 1e-6        # optimize.py:18
 PNORM = 8.0        # optimize.py:19
 ETA = 0.5          # optimize.py:20 (the projection kernel uses eta = 0.5)
 BETA_MAX = 4.0     # optimize.py:21
+MAX_BLOCK = 64          # right-hand sides per device solve (anneal_fem_create's nrhs_max limit)
+STATS_WORK = 3072       # doubles of scratch for anneal_design_stats
+DOT_WORK_PER_RHS = 512  # doubles of scratch per right-hand side for anneal_dot_f64
 
 
 def _flags(grid, active, frozen=None):
@@ -170,8 +173,10 @@ class SimpRun:
         frozen = np.asarray(frozen).astype(bool).ravel()
         self.active = active
         self.cases = list(loads)
+        self.ncase = len(self.cases)
         if fem is None:
-            fem = FEM(grid, active, fixed, nu, log=log, nrhs_max=max(1, len(self.cases)))
+            # one solve takes at most 64 right-hand sides; more load cases are solved in blocks
+            fem = FEM(grid, active, fixed, nu, log=log, nrhs_max=min(MAX_BLOCK, max(1, self.ncase)))
         self.fem = fem
         if learned is not None:
             fem.set_learned(learned, level=learned_level, e_ref=float(E0), mode=learned_mode)
@@ -215,8 +220,8 @@ class SimpRun:
         self.dcx, self.dvx = torch.zeros(nel, **f64), torch.zeros(nel, **f64)
         self.xnew = torch.zeros(nel, **f64)
         self.stats = torch.zeros(8, **f64)
-        self.work = torch.zeros(4096, **f64)
-        self.ncase = len(self.cases)
+        # scratch of anneal_design_stats (3072 doubles) and anneal_dot_f64 (512 per load case)
+        self.work = torch.zeros(max(STATS_WORK, DOT_WORK_PER_RHS * self.ncase), **f64)
         self.F = fem._to_device(Fr)
         self.U = torch.zeros_like(self.F)
         self.extrapolate = bool(extrapolate)
@@ -261,7 +266,12 @@ class SimpRun:
     def step(self):
         """One SIMP iteration (optimize.py:105-175); returns its history entry."""
         self._pre_solve()
-        its = self.fem.solve_device(self.F, self.U, self.ncase, self.rtol, self.maxiter, use_x0=self.have_u)
+        its = []
+        nb = self.fem.nrhs_max
+        for s in range(0, self.ncase, nb):          # load cases in blocks of at most nrhs_max
+            e = min(self.ncase, s + nb)
+            its += self.fem.solve_device(self.F[s:e], self.U[s:e], e - s, self.rtol, self.maxiter,
+                                         use_x0=self.have_u)
         return self._post_solve(its)
 
     def _pre_solve(self):
@@ -339,6 +349,7 @@ class SimpRun:
             self.base_iters = fem.cg_iters[-1]
         if self.on_solve is not None:
             self.on_solve(self)
+        assert self.work.numel() >= DOT_WORK_PER_RHS * self.ncase
         _lib.call("anneal_dot_f64", _lib.ptr(self.F), _lib.ptr(self.U), self.grid.nnode, fem.ldn, self.ncase,
                   _lib.ptr(self.cs_dev), _lib.ptr(self.work), stream)
         tc = self._tick()

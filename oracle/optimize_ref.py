@@ -76,6 +76,28 @@ def penalty_schedule(it, iters, p_final, warm):
     return p_final if warm else min(p_final, 1.0 + 0.25 * ((it - 1) // hold))
 
 
+def element_sensitivity(fem, u_red, xPhys, p, E0, phi, emin=EMIN):
+    """d(f.u)/d(xPhys) of one load case: -p xPhys^(p-1) (1-emin) E0 phi u_e^T ke u_e (optimize.py:118, 126-129)."""
+    return -p * xPhys ** (p - 1) * (1 - emin) * E0 * phi * fem.element_energy(fem.expand(u_red))
+
+
+def combine_cases(cs, dce, w, c0, combine):
+    """Objective and clamped combined sensitivity over the load cases (optimize.py:130-140).
+
+    "sum" (or one case): obj = w.cs, dc = sum w_i dce_i.  "worst": r = w cs / c0,
+    obj = ||r||_8, dc = sum (r_i/obj)^7 w_i / c0_i dce_i.  Then dc = min(dc, 0).
+    """
+    if combine == "sum" or len(cs) == 1:
+        obj = float(w @ cs)
+        dc = sum(wi * d for wi, d in zip(w, dce))
+    else:
+        r = w * cs / c0
+        obj = float((r ** PNORM).sum() ** (1 / PNORM))
+        coef = (r / obj) ** (PNORM - 1) * w / c0
+        dc = sum(ci * d for ci, d in zip(coef, dce))
+    return obj, np.minimum(dc, 0.0)
+
+
 def oc_update(x, dcx, dvx, phi, frz, des, vtarget, move, beta, physical):
     """Geometric bisection on the OC multiplier, returns the last trial design (optimize.py:144-161)."""
     if des.any():
@@ -165,18 +187,10 @@ def simp_ref(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
         if rebuild:
             base_iters = fem.cg_iters[-1]
         cs = np.array([f @ u for f, u in zip(Fr, U)])
-        dce = [-p * xPhys ** (p - 1) * (1 - emin) * E0 * phi * fem.element_energy(fem.expand(u)) for u in U]
+        dce = [element_sensitivity(fem, u, xPhys, p, E0, phi, emin) for u in U]
         if c0 is None:
             c0 = cs.copy()
-        if combine == "sum" or len(cases) == 1:
-            obj = float(w @ cs)
-            dc = sum(wi * d for wi, d in zip(w, dce))
-        else:
-            r = w * cs / c0
-            obj = float((r ** PNORM).sum() ** (1 / PNORM))
-            coef = (r / obj) ** (PNORM - 1) * w / c0
-            dc = sum(ci * d for ci, d in zip(coef, dce))
-        dc = np.minimum(dc, 0.0)
+        obj, dc = combine_cases(cs, dce, w, c0, combine)
         dcx = filt.backward(dc * dxp)
         dvx = filt.backward(phi * dxp)
         xd, steps = oc_update(x, dcx, dvx, phi, frz, des, Vtarget, move, beta[0], physical)

@@ -170,3 +170,31 @@ def test_batch_simp_of_one_shape_matches_individual_runs():
         assert np.abs(rb.result()["x"] - ri.result()["x"]).max() < 1e-6
         ri.fem.close()
     b.close()
+
+
+def test_batch_simp_runs_stop_on_their_own():
+    """Runs with their own iteration budgets and stopping tests leave the batch when simp() would stop them
+    (optimize.py:105, 173-175): histories and designs equal those of separate runs."""
+    import bench
+    from anneal_amd.batch import BatchSimp
+    from anneal_amd.optimize import SimpRun
+    probs = bench.batch_problems(seed=5, n=3, ncase=2, iters=14, shapes=((12, 6, 6), (16, 8, 8)))
+    for p in probs:
+        p["rtol"] = 1e-10
+    probs[0]["iters"] = 9                                    # a shorter budget
+    probs[1].update(beta_max=1.0, tol=1.0)                   # converges at iteration 11 (p = 3 and beta = 1 from 9)
+    b = BatchSimp(probs)
+    res = b.run()
+    assert [len(r["history"]) for r in res] == [9, 11, 14]
+    assert not b.active()
+    keys = ("grid", "fill", "active", "frozen", "fixed", "loads", "weights", "E0", "nu", "vf")
+    for p, rb, want_len in zip(probs, res, (9, 11, 14)):
+        ri = SimpRun(*[p[k] for k in keys], **{k: v for k, v in p.items() if k not in keys}, log=lambda m: None)
+        hi = ri.run()
+        assert len(hi["history"]) == want_len
+        for hb, hs in zip(rb["history"], hi["history"]):
+            assert np.allclose(hb["compliance"], hs["compliance"], rtol=1e-7, atol=0)
+            assert hb["beta"] == hs["beta"] and hb["p"] == hs["p"]
+        assert np.abs(rb["x"] - hi["x"]).max() < 1e-6
+        ri.fem.close()
+    b.close()
