@@ -17,6 +17,12 @@ namespace anneal {
 
 enum FilterMode : int { kHs = 0, kFwd = 1, kBwd = 2, kPhys = 3, kOc = 4 };
 
+// Radii R <= kRingMaxR run the z-marching ring-sum kernel (k_filter<R, MODE>);
+// larger ones stage the filtered quantity in a scratch grid and gather each
+// output's (2R+1)^3 cube from it (k_filter_stage / k_filter_gather).
+constexpr int kRingMaxR = 5;
+constexpr int kWc = 3 * kRingMaxR * kRingMaxR + 1;  // squared distances 0 .. 3R^2
+
 struct FilterArgs {
   const double* in;
   const double* dcx;
@@ -31,7 +37,10 @@ struct FilterArgs {
   double* oc_out;         // kOc: state after it (written by workgroup 0)
   const double* part_in;  // kOc with prev: volume partials of the previous evaluation, not yet in oc
   int prev;
-  double wc[13];  // hat weight max(0, r - sqrt(d2)) for squared distance d2 = 0..3R^2
+  double wc[kWc];    // hat weight max(0, r - sqrt(d2)) for squared distance d2 = 0..3R^2 (R <= kRingMaxR)
+  const double* wt;  // the same weights in device memory (any R; the gather kernel reads them)
+  double* stg;       // staged values of the gather path (nelem doubles)
+  int R;
   double beta, move;
   int nx, ny, nz;
 };
@@ -179,6 +188,31 @@ __host__ __device__ constexpr int ring_taps(int R, int m) {
   return n;
 }
 
+// kOc prologue of a filter launch: the step of the previous evaluation (the
+// same bits in every workgroup, from the fixed-order sum of its partials),
+// the state written out by workgroup 0, and this launch's trial parameters.
+// Returns false once the bisection has stopped (the launch then only carries
+// the state forward).  256 threads.
+__device__ __forceinline__ bool oc_prologue(const FilterArgs& a, double* st, double* sred, double& rlam, double& mv) {
+  const int tid = threadIdx.x;
+  if (a.prev && a.oc[kOcDone] == 0.0) {
+    const double vol = oc_volume(a.part_in, gridDim.x, sred);
+    if (tid == 0) {
+      for (int i = 0; i < kOcLen; ++i) st[i] = a.oc[i];
+      oc_step(st, vol);
+    }
+  } else if (tid < kOcLen) {
+    st[tid] = a.oc[tid];
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < kOcLen) a.oc_out[tid] = st[tid];
+  if (st[kOcDone] != 0.0) return false;
+  rlam = 1.0 / sqrt(st[kOcLam]);
+  mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
+  __syncthreads();  // sred is reused by the volume sum at the end
+  return true;
+}
+
 template <int R, int MODE>
 __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   constexpr int K = 2 * R + 1, LX = FTX + 2 * R, LY = FTY + 2 * R, NP = LX * LY, NT = FTX * FTY;
@@ -275,21 +309,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
     // this launch's trial multiplier
     static_assert(NT == 256, "oc_volume sums over 256 threads");
     __shared__ double st[kOcLen];
-    if (a.prev && a.oc[kOcDone] == 0.0) {
-      const double vol = oc_volume(a.part_in, gridDim.x, sred);
-      if (tid == 0) {
-        for (int i = 0; i < kOcLen; ++i) st[i] = a.oc[i];
-        oc_step(st, vol);
-      }
-    } else if (tid < kOcLen) {
-      st[tid] = a.oc[tid];
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && tid < kOcLen) a.oc_out[tid] = st[tid];
-    if (st[kOcDone] != 0.0) return;
-    rlam = 1.0 / sqrt(st[kOcLam]);
-    mv = a.beta <= 1.0 ? a.move : 0.5 * a.move;
-    __syncthreads();  // sred is reused by the volume sum at the end
+    if (!oc_prologue(a, st, sred, rlam, mv)) return;
   }
 
   // one input layer; SL = s & 1 selects the LDS buffer and the prefetch slots
@@ -373,6 +393,103 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   }
 }
 
+// Gather path for radii above kRingMaxR, first kernel: the staged value of
+// every element (what k_filter writes to LDS) into a.stg, grid-stride over
+// the same number of workgroups as the gather kernel, so that an OC
+// evaluation's prologue sums as many volume partials as the previous
+// evaluation's gather wrote.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_filter_stage(FilterArgs a) {
+  __shared__ double st[kOcLen];
+  __shared__ double sred[4];
+  double rlam = 0.0, mv = 0.0;
+  if (MODE == kOc && !oc_prologue(a, st, sred, rlam, mv)) return;
+  const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const uint8_t f = a.flags[e];
+    double v = 0.0;
+    if (f & kActive) {
+      if (MODE == kHs) v = 1.0;
+      else if (MODE == kFwd || MODE == kPhys) v = a.in[e];
+      else if (MODE == kBwd) v = a.in[e] * a.invHs[e];
+      else if (f & kFrozen) v = 1.0;
+      else {
+        const double x = a.in[e];
+        v = fmin(fmax(x * (a.dcx[e] * rlam), fmax(0.0, x - mv)), fmin(1.0, x + mv));
+      }
+    }
+    a.stg[e] = v;
+  }
+}
+
+// Gather path, second kernel: each output sums w(dz^2 + dy^2 + dx^2) times
+// the staged values of its (2R+1)^3 cube (inside the grid) in a fixed order
+// (dz, dy, dx ascending), then the epilogue of k_filter.  A workgroup covers
+// an FTX x FTY x FZC tile (one column per thread), as k_filter does.
+template <int MODE>
+__global__ __launch_bounds__(FTX* FTY) void k_filter_gather(FilterArgs a) {
+  constexpr int NT = FTX * FTY;
+  __shared__ double sred[NT / 64];
+  if (MODE == kOc && a.oc_out[kOcDone] != 0.0) return;  // stopped: the stage launch carried the state
+  const int tid = threadIdx.x, R = a.R;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int ntx = (nx + FTX - 1) / FTX, nty = (ny + FTY - 1) / FTY;
+  int bid = xcd_logical(blockIdx.x, gridDim.x);
+  const int bx = bid % ntx;
+  bid /= ntx;
+  const int by = bid % nty;
+  const int bz = bid / nty;
+  const int gx = bx * FTX + tid % FTX, gy = by * FTY + tid / FTX;
+  const bool col = gx < nx && gy < ny;
+  const int64_t exy = (int64_t)nx * ny;
+  const int x0 = max(gx - R, 0), x1 = min(gx + R, nx - 1), y0 = max(gy - R, 0), y1 = min(gy + R, ny - 1);
+  double vol = 0.0;
+  for (int j = 0; j < FZC; ++j) {
+    const int z = bz * FZC + j;
+    if (!col || z >= nz) break;
+    double v = 0.0;
+    for (int zz = max(z - R, 0); zz <= min(z + R, nz - 1); ++zz) {
+      const int dz2 = (zz - z) * (zz - z);
+      for (int yy = y0; yy <= y1; ++yy) {
+        const int d2 = dz2 + (yy - gy) * (yy - gy);
+        const double* row = a.stg + exy * zz + (int64_t)nx * yy;
+        for (int xx = x0; xx <= x1; ++xx) v += a.wt[d2 + (xx - gx) * (xx - gx)] * row[xx];
+      }
+    }
+    const int64_t e = gx + (int64_t)nx * gy + exy * z;
+    const uint8_t f = a.flags[e];
+    const bool act = f & kActive;
+    if (MODE == kHs) {
+      a.out[e] = v;
+    } else if (MODE == kFwd) {
+      a.out[e] = act ? v * a.invHs[e] : 0.0;
+    } else if (MODE == kBwd) {
+      a.out[e] = act ? v : 0.0;
+    } else {
+      double xp = 0.0, dxp = 0.0;
+      if (act) {
+        project(v * a.invHs[e], a.beta, xp, dxp);
+        if (f & kFrozen) xp = 1.0;
+        vol += a.phi[e] * xp;
+      }
+      if (MODE == kPhys) {
+        a.out[e] = xp;
+        a.out2[e] = dxp;
+      }
+    }
+  }
+  if (MODE == kPhys || MODE == kOc) {
+    for (int off = 32; off > 0; off >>= 1) vol += __shfl_down(vol, off, 64);
+    if ((tid & 63) == 0) sred[tid >> 6] = vol;
+    __syncthreads();
+    if (tid == 0) {
+      double sm = 0.0;
+      for (int w = 0; w < NT / 64; ++w) sm += sred[w];
+      a.part[blockIdx.x] = sm;
+    }
+  }
+}
+
 // The last evaluation of a launch batch: its step, in place (one workgroup).
 // The state also goes to out_host (device alias of pinned host memory).
 __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ part, int nblk, double* __restrict__ oc,
@@ -429,9 +546,11 @@ __global__ void k_inv_hs(double* __restrict__ Hs, double* __restrict__ invHs, in
 }
 
 // ---------------------------------------------------------------------------
-// hat weight per squared voxel distance d2 (optimize.py:28-31: k = max(0, r - |d|))
-static void cone_weights(double r, double* wc) {
-  for (int d2 = 0; d2 < 13; ++d2) wc[d2] = std::max(0.0, r - std::sqrt(double(d2)));
+// hat weight per squared voxel distance d2 = 0 .. 3R^2 (optimize.py:28-31: k = max(0, r - |d|))
+static std::vector<double> cone_weights(double r, int R) {
+  std::vector<double> w(3 * R * R + 1);
+  for (int d2 = 0; d2 < (int)w.size(); ++d2) w[d2] = std::max(0.0, r - std::sqrt(double(d2)));
+  return w;
 }
 
 template <int MODE>
@@ -439,7 +558,15 @@ static void launch_filter_mode(int R, const FilterArgs& a, int nblk, hipStream_t
   switch (R) {
     case 1: hipLaunchKernelGGL((k_filter<1, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
     case 2: hipLaunchKernelGGL((k_filter<2, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
-    default: throw std::invalid_argument("filter radius above 3 voxels is not supported");
+    case 3: hipLaunchKernelGGL((k_filter<3, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_filter<4, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_filter<5, MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a); break;
+    default:
+      static_assert(kRingMaxR == 5, "one case per ring-kernel radius");
+      hipLaunchKernelGGL((k_filter_stage<MODE>), dim3(nblk), dim3(256), 0, s, a);
+      ANNEAL_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_filter_gather<MODE>), dim3(nblk), dim3(FTX * FTY), 0, s, a);
+      break;
   }
   ANNEAL_HIP(hipGetLastError());
 }
@@ -458,8 +585,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be positive");
   ANNEAL_CHECK(r_vox > 0.0 && std::isfinite(r_vox), "filter radius must be positive");
   R_ = std::max((int)std::ceil(r_vox) - 1, 1);
-  ANNEAL_CHECK(R_ <= 2, "filter radius above 3 voxels is not supported");
-  cone_weights(r_vox, w_);
+  w_ = cone_weights(r_vox, R_);
   const int64_t n = g_.nelem();
   nblk_ = ((nx + FTX - 1) / FTX) * ((ny + FTY - 1) / FTY) * ((nz + FZC - 1) / FZC);
   ANNEAL_HIP(hipMalloc(&flags_, n));
@@ -471,14 +597,12 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipHostMalloc((void**)&host_, 2 * kOcLen * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_dev_, host_, 0));
   ANNEAL_HIP(hipEventCreateWithFlags(&oc_ready_, hipEventDisableTiming));
+  ANNEAL_HIP(hipMalloc(&wt_, w_.size() * sizeof(double)));
+  ANNEAL_HIP(hipMemcpy(wt_, w_.data(), w_.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (R_ > kRingMaxR) ANNEAL_HIP(hipMalloc(&stg_, n * sizeof(double)));
   ANNEAL_HIP(hipMemcpy(flags_, flags_host, n, hipMemcpyHostToDevice));
-  FilterArgs a{};
-  a.flags = flags_;
+  FilterArgs a = args();
   a.out = Hs_;
-  std::copy(w_, w_ + 13, a.wc);
-  a.nx = nx;
-  a.ny = ny;
-  a.nz = nz;
   launch_filter(kHs, R_, a, nblk_, s_);
   hipLaunchKernelGGL(k_inv_hs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, Hs_, invHs_, n);
   ANNEAL_HIP(hipGetLastError());
@@ -493,51 +617,49 @@ FilterContext::~FilterContext() {
   (void)hipFree(r0_);
   (void)hipFree(part_);
   (void)hipFree(oc_);
+  (void)hipFree(wt_);
+  (void)hipFree(stg_);
   (void)hipHostFree(host_);
   (void)hipEventDestroy(oc_ready_);
 }
 
-void FilterContext::forward(const double* x, double* y) {
+FilterArgs FilterContext::args() const {
   FilterArgs a{};
-  a.in = x;
   a.flags = flags_;
   a.invHs = invHs_;
-  a.out = y;
-  std::copy(w_, w_ + 13, a.wc);
+  std::copy(w_.begin(), w_.begin() + std::min<size_t>(w_.size(), kWc), a.wc);
+  a.wt = wt_;
+  a.stg = stg_;
+  a.R = R_;
   a.nx = g_.nx;
   a.ny = g_.ny;
   a.nz = g_.nz;
+  return a;
+}
+
+void FilterContext::forward(const double* x, double* y) {
+  FilterArgs a = args();
+  a.in = x;
+  a.out = y;
   launch_filter(kFwd, R_, a, nblk_, s_);
 }
 
 void FilterContext::backward(const double* d, double* y) {
-  FilterArgs a{};
+  FilterArgs a = args();
   a.in = d;
-  a.flags = flags_;
-  a.invHs = invHs_;
   a.out = y;
-  std::copy(w_, w_ + 13, a.wc);
-  a.nx = g_.nx;
-  a.ny = g_.ny;
-  a.nz = g_.nz;
   launch_filter(kBwd, R_, a, nblk_, s_);
 }
 
 void FilterContext::physical(const double* x, double beta, const double* phi, double* xphys, double* dxp,
                              double* vol_dev) {
-  FilterArgs a{};
+  FilterArgs a = args();
   a.in = x;
-  a.flags = flags_;
-  a.invHs = invHs_;
   a.phi = phi;
   a.out = xphys;
   a.out2 = dxp;
   a.part = part_;
   a.beta = beta;
-  std::copy(w_, w_ + 13, a.wc);
-  a.nx = g_.nx;
-  a.ny = g_.ny;
-  a.nz = g_.nz;
   launch_filter(kPhys, R_, a, nblk_, s_);
   if (vol_dev) finalize(part_, nblk_, 1, vol_dev, s_);
 }
@@ -571,19 +693,13 @@ void FilterContext::oc_begin(const double* x, const double* dcx, const double* d
 }
 
 void FilterContext::oc_launch(int count) {
-  FilterArgs a{};
+  FilterArgs a = args();
   a.in = run_.x;
   a.dcx = r0_;
   a.dvx = nullptr;
-  a.flags = flags_;
-  a.invHs = invHs_;
   a.phi = run_.phi;
   a.beta = run_.beta;
   a.move = run_.move;
-  std::copy(w_, w_ + 13, a.wc);
-  a.nx = g_.nx;
-  a.ny = g_.ny;
-  a.nz = g_.nz;
   // Evaluation e reads the state in oc_[e % 2], first takes the step of
   // evaluation e - 1 from the partials in part_[(e - 1) % 2] (except the first
   // of a batch, whose predecessor the batch finisher took), evaluates and

@@ -1,6 +1,8 @@
 // Density-side kernels of the SIMP loop: hat filter, Heaviside projection,
 // sensitivities and the OC bisection, all FP64 on the full voxel grid.
 #pragma once
+#include <vector>
+
 #include "stencil.hpp"
 
 namespace anneal {
@@ -16,6 +18,8 @@ constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOc
               kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19, kOcLo0 = 20, kOcHi0 = 21, kOcEnds = 22,
               kOcWidth = 23;
 constexpr int kOcLen = 24;
+
+struct FilterArgs;
 
 class FilterContext {
  public:
@@ -62,11 +66,13 @@ class FilterContext {
  private:
   GridDims g_;
   int R_;
-  double w_[13];  // hat weight per squared voxel distance
+  std::vector<double> w_;  // hat weight per squared voxel distance 0 .. 3R^2
   hipStream_t s_;
   uint8_t* flags_ = nullptr;
   double *Hs_ = nullptr, *invHs_ = nullptr, *r0_ = nullptr, *part_ = nullptr, *oc_ = nullptr, *host_ = nullptr;
   double* host_dev_ = nullptr;  // device alias of host_ (pinned): state read-back and upload without copies
+  double* wt_ = nullptr;        // w_ in device memory
+  double* stg_ = nullptr;       // staged values of the gather path (radii above the ring kernels')
   int nblk_ = 0;
   // the bisection in flight (oc_begin .. oc_end)
   struct OcRun {
@@ -80,6 +86,7 @@ class FilterContext {
   } run_{};
   hipEvent_t oc_ready_ = nullptr;  // recorded after each state read-back
   void oc_launch(int count);
+  FilterArgs args() const;  // launch arguments common to every mode
 };
 
 // ce_r(e) = u_e^T ke u_e for E = 1 (anneal/fem.py:239-242), out [nrhs][nelem].

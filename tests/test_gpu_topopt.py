@@ -34,8 +34,11 @@ def _setup(shape, seed, r, inactive=0.2, frozen_frac=0.1):
 
 
 @pytest.mark.parametrize("shape,r,seed", [((9, 7, 5), 1.5, 0), ((40, 11, 13), 3.0, 1), ((33, 17, 10), 2.2, 2),
-                                          ((32, 8, 12), 1.6, 3)])
+                                          ((32, 8, 12), 1.6, 3), ((35, 13, 11), 4.0, 4), ((41, 19, 12), 5.5, 5),
+                                          ((30, 17, 14), 7.2, 6), ((6, 5, 4), 9.0, 7)])
 def test_filter_forward_backward(torch, shape, r, seed):
+    """Radii 1.5-6 run the ring-sum kernel (R = 1..5), 7.2 and 9 the gather kernels (R = 7, 8; the
+    second wider than the grid)."""
     print(f"filter parity shape={shape} r={r}", flush=True)
     from anneal_amd.fem import Grid
     from anneal_amd.optimize import Filter
@@ -48,11 +51,11 @@ def test_filter_forward_backward(torch, shape, r, seed):
         assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
 
 
-@pytest.mark.parametrize("beta", [0.0, 1.0, 4.0])
-def test_physical_projection(torch, beta):
+@pytest.mark.parametrize("beta,r", [(0.0, 1.5), (1.0, 1.5), (4.0, 1.5), (2.0, 4.0), (4.0, 5.5), (2.0, 7.2)])
+def test_physical_projection(torch, beta, r):
     from anneal_amd.optimize import DeviceFilter
     from anneal_amd.fem import Grid
-    shape, r = (20, 9, 7), 1.5
+    shape = (20, 9, 7)
     g, active, frozen, rng = _setup(shape, 5, r)
     ref = FilterRef(g, active, r)
     x_act = rng.random(ref.act.size)
@@ -77,13 +80,15 @@ def test_physical_projection(torch, beta):
     assert abs(vol.item() - (phi_act * xp).sum()) <= 1e-12 * (phi_act * xp).sum()
 
 
-@pytest.mark.parametrize("beta,target", [(0.0, "inside"), (2.0, "inside"), (0.0, "unreachable")])
-def test_oc_bisection(torch, beta, target):
+@pytest.mark.parametrize("beta,target,r", [(0.0, "inside", 1.5), (2.0, "inside", 1.5), (0.0, "unreachable", 1.5),
+                                           (2.0, "inside", 4.0), (0.0, "inside", 5.5), (2.0, "inside", 7.2),
+                                           (0.0, "unreachable", 7.2)])
+def test_oc_bisection(torch, beta, target, r):
     """Device OC update vs the oracle's 36-trial bisection (optimize.py:144-161): same trial
     count and the same last trial design, from fewer volume evaluations."""
     from anneal_amd.optimize import DeviceFilter
     from anneal_amd.fem import Grid
-    shape, r = (24, 12, 10), 1.5
+    shape = (24, 12, 10)
     g, active, frozen, rng = _setup(shape, 9, r)
     ref = FilterRef(g, active, r)
     act = ref.act
@@ -240,3 +245,24 @@ def test_v6_simp_beats_uniform(torch):
     r = benchmark_cantilever()
     print(f"V6 stiffer {r['stiffer_pct']:.1f}% vol_err {r['vol_err']:.1e}", flush=True)
     assert r["stiffer_pct"] > 20 and abs(r["vol_err"]) < 1e-3
+
+
+@pytest.mark.parametrize("r_mm", [4.0, 7.2])
+def test_simp_with_wide_filter_matches_oracle(torch, r_mm):
+    """simp() with filter radii the reference's run.py produces for thin-walled materials at fine voxel sizes
+    (run.py:197-199: r = max(minMember / 2, 1.5 h)): R = 3 (ring kernel) and R = 7 (gather kernels)."""
+    from oracle.optimize_ref import simp_ref
+    from anneal_amd.fem import Grid
+    from anneal_amd.optimize import simp
+    shape = (20, 10, 10)
+    g, fill, active, frozen, fixed, F = cantilever(shape)
+    kw = dict(iters=12, r_mm=r_mm, tol=0.0, emin=1e-9, log=lambda *a: None)
+    want = simp_ref(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, 0.3, rtol=1e-11, **kw)
+    print(f"simp with r_mm={r_mm} on {shape}", flush=True)
+    got = simp(Grid(shape, 1.0, (0, 0, 0)), fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, 0.3,
+               rtol=1e-10, **kw)
+    c_got = np.array([h["compliance"][0] for h in got["history"]])
+    c_want = np.array([h["compliance"][0] for h in want["history"]])
+    assert (np.abs(c_got - c_want) / c_want).max() <= 1e-6
+    assert np.abs(got["x"] - want["x"]).max() <= 1e-4
+    assert [h["ocSteps"] for h in got["history"]] == [h["ocSteps"] for h in want["history"]]
