@@ -52,6 +52,7 @@ SIGNATURES = {
     "anneal_design_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
                             c_void_p, c_void_p, c_void_p],
     "anneal_dot_f64": [c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p, c_void_p],
+    "anneal_dot_f64_noalloc": [c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p],
     "anneal_unet_create": [ctypes.POINTER(c_void_p), c_int, c_int, c_intp, c_int, c_void_p],
     "anneal_unet_destroy": [c_void_p],
     "anneal_unet_nparams": [c_void_p, c_i64p],

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cstring>
 #include <exception>
+#include <mutex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,17 @@ const char* get_error() { return g_err.c_str(); }
 }  // namespace anneal
 
 using namespace anneal;
+
+// Static scratch of anneal_dot_f64_noalloc: the per-block partial sums of one
+// call (kDotBlocks per right-hand side, at most kNoAllocMaxRhs right-hand
+// sides, 256 KiB), so that the call allocates nothing.  Each device holds its
+// own copy at its own address until the process ends; the host only ever uses
+// the address hipGetSymbolAddress reports for the current device (the
+// variable's name is a host-side stand-in).  Calls that overlap on one device
+// share the array and race.
+constexpr int kNoAllocMaxRhs = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocMaxRhs * kDotBlocks];
 
 namespace {
 template <typename F>
@@ -54,6 +67,26 @@ inline UNet* net_of(void* p) {
 inline FilterContext* filt_of(void* p) {
   if (!p) throw std::invalid_argument("null filter context");
   return static_cast<FilterContext*>(p);
+}
+
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The address of `scratch` on the current device, looked up on the first call
+// on that device (with it current) and kept.
+double* scratch_here() {
+  int dev = 0;
+  ANNEAL_HIP(hipGetDevice(&dev));
+  ANNEAL_CHECK(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
 }
 }  // namespace
 
@@ -429,6 +462,20 @@ int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int6
   return guarded([&] {
     ANNEAL_CHECK(a_dev && b_dev && out_dev && work_dev && nrhs >= 1, "null argument");
     dot(a_dev, b_dev, nnode, ldn, nrhs, work_dev, out_dev, (hipStream_t)stream);
+    return 0;
+  });
+}
+
+int anneal_dot_f64_noalloc(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs,
+                           double* out_dev, void* stream) {
+  return guarded([&] {
+    ANNEAL_CHECK(a_dev && b_dev && out_dev, "null argument");
+    ANNEAL_CHECK(nrhs >= 1 && nrhs <= kNoAllocMaxRhs,
+                 "anneal_dot_f64_noalloc: nrhs must be in [1, 64] (the size of its static scratch)");
+    ANNEAL_CHECK(nnode >= 0 && ldn >= nnode, "anneal_dot_f64_noalloc: need 0 <= nnode <= ldn");
+    // every partial the reduction reads is written by this call's first kernel, so
+    // the scratch needs no clearing
+    dot(a_dev, b_dev, nnode, ldn, nrhs, scratch_here(), out_dev, (hipStream_t)stream);
     return 0;
   });
 }

@@ -185,6 +185,15 @@ int anneal_design_stats(const double* xold_dev, const double* xnew_dev, const do
  * of a_r . b_r (compliance f.u, optimize.py:125).  work_dev: 512*nrhs doubles. */
 int anneal_dot_f64(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs, double* out_dev,
                    double* work_dev, void* stream);
+/* anneal_dot_f64 without the work_dev argument: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * partial sums live in a static 256 KiB device array of the library, so this
+ * call allocates nothing.  nrhs <= 64 (larger fails with status 1).  Calls
+ * that overlap on one device share that array and race: order them on one
+ * stream.  Same kernels and results as anneal_dot_f64. */
+int anneal_dot_f64_noalloc(const double* a_dev, const double* b_dev, int64_t nnode, int64_t ldn, int nrhs,
+                           double* out_dev, void* stream);
 
 /* FEM.stress / FEM.von_mises (fem.py:244-263) for nodal solutions U_dev
  * (FP64 [nrhs][3][ldn]): sig_dev[r][e][6] = E_e D B0 u_e at the element centre
