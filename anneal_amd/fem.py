@@ -76,7 +76,11 @@ def hex8_matrices(nu, h=1.0):
 
 
 class Grid:
-    """Regular voxel grid; element (i,j,k) spans origin + h*[i,i+1] x ... (fem.py:87-127)."""
+    """Box of nx x ny x nz cubic voxels of side h whose lowest corner sits at ``origin``.
+
+    Voxels and lattice nodes are numbered with x varying fastest, then y, then z;
+    the public methods and that numbering are those of fem.py:87-127.
+    """
 
     def __init__(self, shape, h, origin):
         self.shape = tuple(int(s) for s in shape)
@@ -85,33 +89,36 @@ class Grid:
         self.origin = np.asarray(origin, dtype=float)
         self.nelem = self.nx * self.ny * self.nz
         self.nnode = (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
-        self._noff = LOCAL[:, 0] + (self.nx + 1) * (LOCAL[:, 1] + (self.ny + 1) * LOCAL[:, 2])
+        # node-index step of one lattice unit along x, y, z; the 8 corners' offsets from corner (0,0,0)
+        self._nstep = np.array([1, self.nx + 1, (self.nx + 1) * (self.ny + 1)])
+        self._noff = LOCAL @ self._nstep
 
     def elem_ijk(self, e):
-        e = np.asarray(e)
-        return e % self.nx, (e // self.nx) % self.ny, e // (self.nx * self.ny)
+        rest, i = np.divmod(np.asarray(e), self.nx)
+        k, j = np.divmod(rest, self.ny)
+        return i, j, k
 
     def element_nodes(self, e):
-        i, j, k = self.elem_ijk(e)
-        return (i + (self.nx + 1) * (j + (self.ny + 1) * k))[:, None] + self._noff[None, :]
+        """Node ids of each voxel's 8 corners, in the LOCAL corner order: shape (len(e), 8)."""
+        ijk = np.stack(self.elem_ijk(e), axis=-1)
+        return (ijk @ self._nstep)[:, None] + self._noff[None, :]
 
     def node_coords(self, n):
-        n = np.asarray(n)
-        sx, sy = self.nx + 1, self.ny + 1
-        return self.origin + self.h * np.stack([n % sx, (n // sx) % sy, n // (sx * sy)], axis=-1)
+        rest, i = np.divmod(np.asarray(n), self.nx + 1)
+        k, j = np.divmod(rest, self.ny + 1)
+        return self.origin + self.h * np.stack([i, j, k], axis=-1)
 
     def elem_centers(self, e=None):
-        if e is None:
-            e = np.arange(self.nelem)
-        i, j, k = self.elem_ijk(e)
-        return self.origin + self.h * (np.stack([i, j, k], axis=-1) + 0.5)
+        ijk = np.stack(self.elem_ijk(np.arange(self.nelem) if e is None else e), axis=-1)
+        return self.origin + self.h * (ijk + 0.5)
 
     def to3d(self, flat):
-        """Flat element array (x fastest) -> array indexed [i, j, k]."""
-        return np.asarray(flat).reshape(self.nz, self.ny, self.nx).transpose(2, 1, 0)
+        """Per-voxel values in voxel order -> a 3-D array indexed [i, j, k] (a view where possible)."""
+        return np.asarray(flat).reshape(self.nz, self.ny, self.nx).T
 
     def to1d(self, arr3):
-        return np.asarray(arr3).transpose(2, 1, 0).ravel()
+        """Inverse of to3d: a 3-D array indexed [i, j, k] -> values in voxel order."""
+        return np.asarray(arr3).T.ravel()
 
 
 def _torch():
@@ -402,11 +409,12 @@ class FEM:
 
     @staticmethod
     def normal_stress(sig, d):
-        """Normal stress on planes across direction d (fem.py:250-256)."""
-        d = np.asarray(d, float) / np.linalg.norm(d)
-        x, y, z = d
-        return (sig[:, 0] * x * x + sig[:, 1] * y * y + sig[:, 2] * z * z
-                + 2 * (sig[:, 3] * x * y + sig[:, 4] * y * z + sig[:, 5] * z * x))
+        """Normal stress n.S.n on the planes whose normal is the direction d (fem.py:250-256), from Voigt
+        stresses (xx yy zz xy yz zx) per element."""
+        n = np.asarray(d, float)
+        n = n / np.linalg.norm(n)
+        tensor = np.asarray(sig)[:, [[0, 3, 5], [3, 1, 4], [5, 4, 2]]]      # (m, 3, 3) symmetric stress tensors
+        return np.einsum("i,mij,j->m", n, tensor, n)
 
     def von_mises(self, u_full, E_material):
         """Element-centre von Mises stress per active element (fem.py:258-263), computed on the GPU."""

@@ -470,26 +470,30 @@ def simp(grid, fill, active, frozen, fixed, loads, weights, E0, nu, vf,
 
 
 def benchmark_cantilever(log=lambda *a: None):
-    """V6 (optimize.py:183-210): the optimised cantilever must beat a uniform design."""
-    g = Grid((32, 8, 12), 1.0, (0, 0, 0))
-    fill = np.ones(g.nelem)
-    active = np.ones(g.nelem, bool)
-    frozen = np.zeros(g.nelem, bool)
-    n = np.arange(g.nnode)
-    pxyz = g.node_coords(n)
-    clamp = n[pxyz[:, 0] < 1e-9]
-    fixed = (3 * clamp[:, None] + np.arange(3)).ravel()
-    tip = n[(abs(pxyz[:, 0] - 32) < 1e-9) & (abs(pxyz[:, 2] - 6) < 1e-9)]
-    F = np.zeros(3 * g.nnode)
-    F[3 * tip + 2] = -1.0 / tip.size
-    vf = 0.3
-    r = simp(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1000.0, 0.3, vf, iters=40, r_mm=1.6, log=log)
-    fem = FEM(g, active, fixed, 0.3)
-    Fr = fem.reduce_load(F)
+    """V6 of the reference's validation (optimize.py:183-210).
 
-    def comp(xp):
-        u = fem.solve(fem.assemble(1000.0 * (EMIN + xp ** 3 * (1 - EMIN))), [Fr], rebuild=True)[0]
-        return float(Fr @ u)
-    c_opt = comp(r["x"])
-    c_uni = comp(np.full(g.nelem, vf))
-    return {"stiffer_pct": 100 * (1 - c_opt / c_uni), "vol_err": r["history"][-1]["volumeFraction"] - vf}
+    A 32 x 8 x 12 voxel cantilever, clamped on its x = 0 face and pulled down by
+    a unit load spread over the tip line (x = 32, z = 6), is optimised at 30 %
+    volume (E0 = 1000, 40 iterations, 1.6 mm filter).  Its compliance at
+    penalty 3 is compared with that of a uniform design of the same mass.
+    """
+    g = Grid((32, 8, 12), 1.0, (0, 0, 0))
+    xyz = g.node_coords(np.arange(g.nnode))
+    wall = np.flatnonzero(xyz[:, 0] < 1e-9)
+    tip_line = np.flatnonzero((np.abs(xyz[:, 0] - 32) < 1e-9) & (np.abs(xyz[:, 2] - 6) < 1e-9))
+    fixed = np.repeat(3 * wall, 3) + np.tile(np.arange(3), wall.size)
+    load = np.zeros(3 * g.nnode)
+    load[3 * tip_line + 2] = -1.0 / tip_line.size
+    solid = np.ones(g.nelem, bool)
+    E0, vf = 1000.0, 0.3
+    res = simp(g, np.ones(g.nelem), solid, ~solid, fixed, {"c": load}, {"c": 1.0}, E0, 0.3, vf, iters=40, r_mm=1.6,
+               log=log)
+    fem = FEM(g, solid, fixed, 0.3)
+    f = fem.reduce_load(load)
+
+    def compliance(density):
+        K = fem.assemble(E0 * (EMIN + density ** 3 * (1 - EMIN)))
+        return float(f @ fem.solve(K, [f], rebuild=True)[0])
+
+    c_design, c_uniform = compliance(res["x"]), compliance(np.full(g.nelem, vf))
+    return {"stiffer_pct": 100 * (1 - c_design / c_uniform), "vol_err": res["history"][-1]["volumeFraction"] - vf}

@@ -25,7 +25,8 @@ def test_large_part_two_iterations():
     fem = run.fem
     y = torch.empty_like(run.U)
     _lib.call("anneal_fem_apply_f64", fem._ctx, _lib.ptr(run.U), _lib.ptr(y), run.ncase)
-    res = (run.F - y).reshape(run.ncase, -1).norm(dim=1)
+    nn = g.nnode                    # the apply leaves the padding of y (nodes nnode .. ldn) unwritten
+    res = (run.F - y)[:, :, :nn].reshape(run.ncase, -1).norm(dim=1)
     rel = (res / run.F.reshape(run.ncase, -1).norm(dim=1)).cpu().numpy()
     print("true relative residuals", rel, "cg", h["cgIters"], flush=True)
     assert (rel <= 1.0e-6).all()

@@ -51,8 +51,10 @@ def test_grid_transfer_matches_oracle(fine):
     assert (yc.cpu().float() - rc).abs().max() <= 2 ** -7 * rc.abs().max()
 
 
-@pytest.mark.parametrize("dims,B", [((17, 9, 9), 2), ((12, 7, 6), 1)])
+@pytest.mark.parametrize("dims,B", [((17, 9, 9), 2), ((12, 7, 6), 1), ((65, 33, 33), 1)])
 def test_forward_and_gradients_match_oracle(dims, B):
+    """17x9x9 and 12x7x6 run every convolution with the K dimension split over workgroups; 65x33x33 (level 1 of
+    128x64x64, the learned-preconditioner size) runs the finest levels' convolutions unsplit."""
     from anneal_amd.learned import UNet
     rng = np.random.default_rng(dims[0])
     flat = U.init_params(CIN, COUT, WIDTHS, seed=5, out_scale=1.0)
