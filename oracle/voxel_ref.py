@@ -310,20 +310,54 @@ def path_graph(prob, f, need):
     return act, pos, W
 
 
-def shortest_paths(prob, groups, f, need):
+def shortest_paths(prob, groups, f, need, pred_rule="heap"):
     """export.py:81-83: scipy dijkstra, min_only from the fixed groups' voxels.
-    Returns (act, pos, dist, pred) with dist/pred over the active voxels."""
+    Returns (act, pos, dist, pred) with dist/pred over the active voxels.
+
+    pred_rule "heap": scipy's predecessors (the reference's).  Where several
+    neighbours realise a voxel's distance, scipy keeps the one its Fibonacci
+    heap finalised first, which depends on the order the sources are listed in
+    (tests/test_voxel_oracle.py shows the same graph and source set giving a
+    different tree when the sources are reversed), not on the geometry.
+    pred_rule "smallest": the defined rule the GPU path uses, same distances,
+    predecessor = the smallest voxel index among the active face neighbours v
+    with dist[v] + cost[u] == dist[u] (FP64, left to right), -1 for sources.
+    Both rules give the same predecessor wherever it is unique."""
     from scipy.sparse.csgraph import dijkstra
     act, pos, W = path_graph(prob, f, need)
     fixed_e = [pos[e] for n, ro, e in groups if ro == "fixed"]
     sources = np.unique(np.concatenate(fixed_e)) if fixed_e else np.zeros(0, int)
     sources = sources[sources >= 0]
     dist, pred, _ = dijkstra(W, indices=sources, min_only=True, return_predecessors=True)
+    if pred_rule == "smallest":
+        pred = smallest_index_predecessors(prob.grid, act, pos, dist, f, need, sources)
+    elif pred_rule != "heap":
+        raise ValueError(pred_rule)
     return act, pos, dist, pred
 
 
-def keep_load_paths(prob, f, level, log=print, margin=0.1, max_drop=0.15, repair=True):
-    """export.py:31-118."""
+def smallest_index_predecessors(g, act, pos, dist, f, need, sources):
+    """Predecessor (position in act, -1 if none) = the smallest-voxel-index active face neighbour realising the
+    distance; sources and unreachable voxels get -1."""
+    cost = np.maximum(0.0, need - f[act]) + 1e-3
+    pred = np.full(act.size, -1, dtype=np.int64)
+    i, j, k = g.elem_ijk(act)
+    best = np.full(act.size, np.iinfo(np.int64).max)
+    for ok, step in ((i > 0, -1), (i < g.nx - 1, 1), (j > 0, -g.nx), (j < g.ny - 1, g.nx),
+                     (k > 0, -g.nx * g.ny), (k < g.nz - 1, g.nx * g.ny)):
+        e = act + step
+        q = np.where(ok, pos[np.clip(e, 0, g.nelem - 1)], -1)
+        hit = (q >= 0) & np.isfinite(dist)
+        hit[hit] = dist[q[hit]] + cost[hit] == dist[hit]
+        take = hit & (e < best)
+        best[take] = e[take]
+        pred[take] = q[take]
+    pred[sources] = -1
+    return pred
+
+
+def keep_load_paths(prob, f, level, log=print, margin=0.1, max_drop=0.15, repair=True, pred_rule="heap"):
+    """export.py:31-118 (pred_rule: see shortest_paths)."""
     g = prob.grid
     groups = anchor_groups(prob)
     f = f.copy()
@@ -345,7 +379,7 @@ def keep_load_paths(prob, f, level, log=print, margin=0.1, max_drop=0.15, repair
             f"Turn repair on, or raise volumeFraction or iterations.")
         return f, level
     need = state["need"]
-    act, pos, dist, pred = shortest_paths(prob, groups, f, need)
+    act, pos, dist, pred = shortest_paths(prob, groups, f, need, pred_rule)
     added = 0
     nb = [1, -1, g.nx, -g.nx, g.nx * g.ny, -g.nx * g.ny]
     for name, e in cut:

@@ -239,6 +239,41 @@ def test_keep_load_paths_distances_with_ties():
     assert not any("still thinly connected" in s for s in logs)
 
 
+def test_keep_load_paths_on_uniform_voids_matches_defined_rule():
+    """Uniform cost in the voids (ties almost everywhere): predecessors, repaired field and log lines are
+    identical to the oracle's repair with the defined tie rule (smallest voxel index among the neighbours
+    that realise the distance).  The reference's own tie choice is scipy's heap order, which changes with the
+    order of the source list (tests/test_voxel_oracle.py), so ties are pinned to this rule instead."""
+    from anneal_amd.export import keep_load_paths, path_arrays
+    nx, ny, nz = 40, 16, 12
+    g = Grid((nx, ny, nz), 1.0, (0, 0, 0))
+    anchors = [V.Anchor("corner", "fixed", V.Box([-1, -1, -1], [0.5, 3.0, 3.0])),
+               V.Anchor("far", "load", V.Box([nx - 0.5, ny - 3.0, nz - 3.0], [nx + 1, ny + 1, nz + 1]),
+                        loads={"d": [0, 0, -1]}),
+               V.Anchor("mid", "load", V.Box([nx / 2 - 1, ny - 0.5, -1], [nx / 2 + 1, ny + 1, 2.0]),
+                        loads={"d": [0, -1, 0]})]
+    rng = np.random.default_rng(4)
+    active = rng.random(g.nelem) < 0.95
+    prob = V.Prob(g, anchors, np.ones(g.nelem), active=active)
+    f = np.zeros(g.nelem)
+    f[rng.random(g.nelem) < 0.02] = 1.0        # scattered solid voxels, voids uniform
+    need = 0.6
+    dist, pred, _, _ = path_arrays(prob, f, need)
+    groups = V.anchor_groups(prob)
+    act, pos, dref, pref = V.shortest_paths(prob, groups, f, need, pred_rule="smallest")
+    d, p = dist.cpu().numpy(), pred.cpu().numpy()
+    assert np.array_equal(d[act], dref)
+    want = np.where(pref >= 0, act[np.maximum(pref, 0)], -1)
+    assert np.array_equal(p[act], want)
+    logs, logs_ref = [], []
+    got, lvl = keep_load_paths(prob, f, 0.5, log=logs.append)
+    ref, lvl_ref = V.keep_load_paths(prob, f, 0.5, log=logs_ref.append, pred_rule="smallest")
+    print("\n".join(logs), flush=True)
+    assert lvl == lvl_ref and logs == logs_ref
+    assert any("reconnected anchor" in s for s in logs)
+    assert np.array_equal(got, ref)
+
+
 def test_keep_load_paths_lowering_and_no_repair():
     from anneal_amd.export import keep_load_paths
     prob, f = _path_problem((24, 8, 8), 5)

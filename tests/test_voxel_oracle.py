@@ -102,6 +102,75 @@ def test_keep_load_paths_repair_reconnects():
     assert V.broken(prob, groups, f2, min(level + 0.1, 0.95)) == []
 
 
+def _tie_problem(shape=(14, 8, 6), seed=0):
+    """Uniform cost in voids and small corner anchors: most voxels have several equal-cost predecessors."""
+    nx, ny, nz = shape
+    g = Grid(shape, 1.0, (0, 0, 0))
+    anchors = [V.Anchor("corner", "fixed", V.Box([-1, -1, -1], [0.5, 2.0, 2.0])),
+               V.Anchor("far", "load", V.Box([nx - 0.5, ny - 2.0, nz - 2.0], [nx + 1, ny + 1, nz + 1]),
+                        loads={"d": [0, 0, -1]})]
+    rng = np.random.default_rng(seed)
+    active = rng.random(g.nelem) < 0.95
+    prob = V.Prob(g, anchors, np.ones(g.nelem), active=active)
+    return prob, np.zeros(g.nelem)
+
+
+def test_scipy_tie_break_depends_on_source_order():
+    """The reference's predecessor among equal-cost paths is an artifact of scipy's heap order: the same graph
+    and the same source set, listed in reverse, give identical distances but a different predecessor tree."""
+    from scipy.sparse.csgraph import dijkstra
+    prob, f = _tie_problem()
+    act, pos, W = V.path_graph(prob, f, 0.6)
+    groups = V.anchor_groups(prob)
+    src = np.unique(np.concatenate([pos[e] for n, ro, e in groups if ro == "fixed"]))
+    src = src[src >= 0]
+    assert src.size > 1
+    d1, p1, _ = dijkstra(W, indices=src, min_only=True, return_predecessors=True)
+    d2, p2, _ = dijkstra(W, indices=src[::-1], min_only=True, return_predecessors=True)
+    assert np.array_equal(d1, d2)
+    assert (p1 != p2).sum() > 0.05 * act.size
+
+
+def test_smallest_index_predecessor_rule_bruteforce():
+    """The defined tie rule (pred_rule='smallest') against a loop over each voxel's face neighbours; where the
+    realising neighbour is unique it equals scipy's predecessor."""
+    prob, f = _tie_problem(seed=1)
+    f[:: 7] = 0.5                               # some non-uniform costs too
+    need = 0.6
+    groups = V.anchor_groups(prob)
+    act, pos, dist, pred = V.shortest_paths(prob, groups, f, need, pred_rule="smallest")
+    _, _, dist_h, pred_h = V.shortest_paths(prob, groups, f, need, pred_rule="heap")
+    assert np.array_equal(dist, dist_h)
+    g = prob.grid
+    cost = np.maximum(0.0, need - f) + 1e-3
+    src = set(np.unique(np.concatenate([pos[e] for n, ro, e in groups if ro == "fixed"])).tolist()) - {-1}
+    ties = unique = 0
+    for q, e in enumerate(act):
+        i, j, k = (int(v) for v in g.elem_ijk(e))
+        cands = []
+        for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            a, b_, c = i + di, j + dj, k + dk
+            if 0 <= a < g.nx and 0 <= b_ < g.ny and 0 <= c < g.nz:
+                v = a + g.nx * (b_ + g.ny * c)
+                if pos[v] >= 0 and np.isfinite(dist[q]) and dist[pos[v]] + cost[e] == dist[q]:
+                    cands.append(v)
+        want = -1 if (q in src or not cands) else pos[min(cands)]
+        assert pred[q] == want
+        if q not in src and len(cands) == 1:
+            unique += 1
+            assert pred_h[q] == pred[q]
+        ties += len(cands) > 1
+    assert ties > 0.1 * act.size and unique > 0
+
+
+def test_keep_load_paths_with_defined_rule_reconnects():
+    prob, f = _tie_problem(shape=(20, 10, 8), seed=2)
+    logs = []
+    f2, level = V.keep_load_paths(prob, f, 0.5, log=logs.append, pred_rule="smallest")
+    assert any("reconnected anchor 'far'" in s for s in logs)
+    assert V.broken(prob, V.anchor_groups(prob), f2, min(level + 0.1, 0.95)) == []
+
+
 # ---- iso level -----------------------------------------------------------------
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
 def test_choose_level_brackets_the_sorted_field(seed):
