@@ -196,6 +196,123 @@ __global__ void __launch_bounds__(256) k_conv_finish(ConvK a) {
   if (a.yf && co < a.cf) a.yf[v * a.cf + co] = val;
 }
 
+// 3x3x3 convolution with the input brick staged in LDS (grids with enough
+// bricks to fill the machine).  A workgroup owns a brick of 8 (x) x 8 (y) x MB
+// (z) output voxels of one sample: MFMA row blocks of 16 voxels (8 x by 2 y);
+// wave w takes the y pair w of every z layer (MB row blocks), times NB blocks
+// of 16 output channels.  The input brick with its one-voxel halo (10 x 10 x
+// (MB + 2) voxels) is staged for S input channels at a time (S = 32, 16 or 8,
+// a divisor of cin), so each activation is read from global memory once per
+// workgroup and the 27 taps read it from LDS.  Within a slice the K index runs
+// over (tap, channel): chunk c, lane group g covers kk = 32 c + 8 g -> tap
+// kk / S, channels kk % S .. +7; the chunk loop is unrolled, so the tap
+// offsets are constants (taps past 26 pad the last chunk with zero weights).
+// The summation order is fixed (slices, then chunks), so results are
+// reproducible.
+constexpr int kBX = 8, kBY = 8, kHX = kBX + 2, kHY = kBY + 2;
+
+__host__ __device__ constexpr int brick_tap_offset(int tap) {
+  return (tap % 3 - 1) + kHX * ((tap / 3) % 3 - 1) + kHX * kHY * (tap / 9 - 1);
+}
+
+template <int NB, int S, int MB>
+__global__ void __launch_bounds__(256) k_conv_brick(ConvK a) {
+  constexpr int PITCH = S + 8;  // bf16 per staged voxel: 16-byte aligned rows, spread over the banks
+  constexpr int HV = kHX * kHY * (MB + 2);
+  constexpr int NCH = (27 * S + 31) / 32;  // chunks per slice
+  __shared__ __attribute__((aligned(16))) uint16_t xs[HV * PITCH];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, r = lane & 15;
+  const int nbx = (a.nx + kBX - 1) / kBX, nby = (a.ny + kBY - 1) / kBY, nbz = (a.nz + MB - 1) / MB;
+  int bid = blockIdx.x;
+  const int bx = bid % nbx;
+  bid /= nbx;
+  const int by = bid % nby;
+  bid /= nby;
+  const int bz = bid % nbz;
+  const int b = bid / nbz;
+  const int X0 = bx * kBX, Y0 = by * kBY, Z0 = bz * MB;
+  const int64_t nxy = (int64_t)a.nx * a.ny, nvs = nxy * a.nz, vs0 = (int64_t)b * nvs;
+  // LDS voxel of this lane's output in row block mb (z layer mb, y pair wv), at tap offset 0
+  const int hv0 = ((r & 7) + 1) + kHX * ((2 * wv + (r >> 3)) + 1) + kHX * kHY * 1;
+  f32x4_t acc[MB][NB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int n0 = blockIdx.y * 16 * NB;
+  const uint16_t* wrow[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) wrow[nb] = a.w + (int64_t)(n0 + 16 * nb + r) * a.kp;
+  for (int s0 = 0; s0 < a.cin; s0 += S) {
+    // stage the halo brick for channels s0 .. s0 + S - 1 (zeros outside the grid)
+    __syncthreads();  // the previous slice's reads are done
+    for (int q = t; q < HV * (S / 8); q += 256) {
+      const int hv = q / (S / 8), cg = q % (S / 8);
+      const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+      const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+      u32x4_t u = {0u, 0u, 0u, 0u};
+      if (unsigned(gx) < unsigned(a.nx) && unsigned(gy) < unsigned(a.ny) && unsigned(gz) < unsigned(a.nz))
+        u = ld16(a.x + (vs0 + gx + (int64_t)a.nx * gy + nxy * gz) * a.xpitch + s0 + 8 * cg);
+      *reinterpret_cast<u32x4_t*>(&xs[hv * PITCH + 8 * cg]) = u;
+    }
+    __syncthreads();
+    // chunk c: tap = (32 c + 8 g) / S (lane-group dependent unless S = 32), channels cc = (8 g) % S
+    auto frag = [&](int c, bf16x8_t (&A)[MB], bf16x8_t (&Bv)[NB]) {
+      const int kk = 32 * c + 8 * g;
+      const int tap = kk / S, cc = kk % S;
+      const bool kv = tap < 27;
+      const int ta = kv ? tap : 26;
+      const int off = (hv0 + brick_tap_offset(ta)) * PITCH + cc;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        A[mb] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&xs[off + mb * kHX * kHY * PITCH]));
+      const int kw = ta * a.cin + s0 + cc;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        u32x4_t u = ld16(wrow[nb] + kw);
+        if (!kv) u = u32x4_t{0u, 0u, 0u, 0u};
+        Bv[nb] = __builtin_bit_cast(bf16x8_t, u);
+      }
+    };
+    bf16x8_t A0[MB], B0[NB], A1[MB], B1[NB];
+    frag(0, A0, B0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      if (c + 1 < NCH) frag(c + 1, A1, B1);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0[mb], B0[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) A0[mb] = A1[mb];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) B0[nb] = B1[nb];
+    }
+  }
+  // epilogue: lane holds rows 4 g .. 4 g + 3 of its row block, column r of each N block
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * g + i;
+      const int x = X0 + (row & 7), y = Y0 + 2 * wv + (row >> 3), z = Z0 + mb;
+      if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
+      const int64_t v = vs0 + x + (int64_t)a.nx * y + nxy * z;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int co = n0 + 16 * nb + r;
+        float val = acc[mb][nb][i];
+        if (a.bias) val += a.bias[co];
+        if (a.accum) val += bf2f(a.y[v * a.ypitch + co]);
+        if (a.relu) val = fmaxf(val, 0.f);
+        if (a.mask && !(bf2f(a.mask[v * a.mpitch + co]) > 0.f)) val = 0.f;
+        if (a.y) a.y[v * a.ypitch + co] = f2bf(val);
+        if (a.yf && co < a.cf) a.yf[v * a.cf + co] = val;
+      }
+    }
+}
+
 struct WgradK {
   const uint16_t* x;
   int xpitch, cin, taps;
@@ -828,9 +945,37 @@ static void launch_conv(const ConvK& a, int n16, hipStream_t s) {
   hipLaunchKernelGGL((k_conv<NB, MB>), grid, dim3(256), 0, s, a);
 }
 
-// Grids smaller than the machine split the K loop over blockIdx.z (at least 4
-// chunks each) so enough waves overlap their load latency.
+constexpr int kBrickMB = 4;  // z layers per brick
+
+template <int S>
+static void launch_brick(const ConvK& a, int n16, int64_t bricks, hipStream_t s) {
+  // all output channels of a brick in one workgroup when they fit (up to 64), else 64 per workgroup
+  const int NB = n16 >= 4 ? 4 : n16;
+  dim3 grid(unsigned(bricks), unsigned(n16 / NB));
+  if (NB == 4) hipLaunchKernelGGL((k_conv_brick<4, S, kBrickMB>), grid, dim3(256), 0, s, a);
+  else if (NB == 2) hipLaunchKernelGGL((k_conv_brick<2, S, kBrickMB>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_conv_brick<1, S, kBrickMB>), grid, dim3(256), 0, s, a);
+}
+
+// 3x3x3 convolutions on grids of at least kBrickMin bricks run k_conv_brick
+// (input staged in LDS); the others, and the 1x1x1 output convolution, run
+// k_conv, whose K loop is split over blockIdx.z on grids smaller than the
+// machine (at least 4 chunks each) so enough waves overlap their load latency.
+constexpr int64_t kBrickMin = 128;
+
 void UNet::conv_dispatch(ConvK a, int n16) {
+  if (a.taps == 27 && (n16 == 1 || n16 == 2 || n16 % 4 == 0) && a.cin % 8 == 0 && a.kp >= 27 * a.cin) {
+    const int64_t nvs = (int64_t)a.nx * a.ny * a.nz;
+    const int64_t bricks = (a.nvox / nvs) * ((a.nx + kBX - 1) / kBX) * ((a.ny + kBY - 1) / kBY) *
+                           ((a.nz + kBrickMB - 1) / kBrickMB);
+    if (bricks >= kBrickMin) {
+      if (a.cin % 32 == 0) launch_brick<32>(a, n16, bricks, s_);
+      else if (a.cin % 16 == 0) launch_brick<16>(a, n16, bricks, s_);
+      else launch_brick<8>(a, n16, bricks, s_);
+      ANNEAL_HIP(hipGetLastError());
+      return;
+    }
+  }
   int NB, MB;
   pick_nb(n16, NB, MB);
   const int64_t blocks = ((a.nvox + 64 * MB - 1) / (64 * MB)) * (n16 / NB);

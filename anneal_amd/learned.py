@@ -169,11 +169,17 @@ def training_problem(seed, shape=TRAIN_SHAPE):
     return g, np.ones(g.nelem), np.ones(g.nelem, bool), np.zeros(g.nelem, bool), fixed, F, vf
 
 
-def generate_pairs(n_runs, iters=50, shape=TRAIN_SHAPE, seed=0, level=1, r_mm=1.5, log=None):
+def generate_pairs(n_runs, iters=50, shape=TRAIN_SHAPE, seed=0, level=1, r_mm=1.5, log=None, kind="residual"):
     """(X bf16 [N, V, 8], T FP32 [N, V, 3], node dims) from every solve of n_runs seeded SIMP runs.
 
-    Each SIMP iteration contributes the (density, load, displacement) triple of
-    its solve, mapped onto multigrid level ``level`` by FEM.learned_pair.
+    kind "residual" (what the solve hands the network): each solve contributes
+    (density, residual r0 = F - K u0 of its starting point u0, correction
+    u - u0), so the pairs follow the distribution the network sees inside the
+    solver, where the start is extrapolated from the previous solutions and the
+    residual is small and local; the first solve of a run starts from 0, so it
+    contributes (density, load, displacement).  kind "load": every solve
+    contributes (density, load, displacement).  Mapped onto multigrid level
+    ``level`` by FEM.learned_pair.
     """
     torch = _torch()
     from .optimize import SimpRun
@@ -181,14 +187,23 @@ def generate_pairs(n_runs, iters=50, shape=TRAIN_SHAPE, seed=0, level=1, r_mm=1.
 
     def capture(run):
         nonlocal dims
-        x, t, dims = run.fem.learned_pair(run.F, run.U, run.ncase, level=level, e_ref=run.E0)
+        if kind == "residual":
+            y = torch.empty_like(run.U_start)
+            _lib.call("anneal_fem_apply_f64", run.fem._ctx, _lib.ptr(run.U_start), _lib.ptr(y), run.ncase)
+            r0 = run.F - y          # the apply leaves padding nodes unwritten; learned_pair reads nodes only
+            x, t, dims = run.fem.learned_pair(r0, run.U - run.U_start, run.ncase, level=level, e_ref=run.E0)
+        else:
+            x, t, dims = run.fem.learned_pair(run.F, run.U, run.ncase, level=level, e_ref=run.E0)
         xs.append(x)
         ts.append(t)
 
+    if kind not in ("residual", "load"):
+        raise ValueError(kind)
     for r in range(n_runs):
         g, fill, active, frozen, fixed, F, vf = training_problem(seed + r, shape)
         run = SimpRun(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, vf, iters=iters, r_mm=r_mm,
                       tol=0.0, emin=1e-9, rtol=1e-6, log=lambda m: None, on_solve=capture)
+        run.keep_start = kind == "residual"
         for _ in range(iters):
             run.step()
         run.fem.close()

@@ -257,6 +257,8 @@ class SimpRun:
         self.oc_ratio_prev = 0.0
         self.oc_extrapolate = True
         self.oc_evals = 0
+        self.keep_start = False   # True: keep each solve's starting point in U_start (training pairs)
+        self.U_start = None
 
     def _tick(self):
         if self.timing:
@@ -302,6 +304,8 @@ class SimpRun:
         if self._rebuild:
             fem.rebuild()
         self._extrapolate()
+        if self.keep_start:
+            self.U_start = self.U.clone() if self.have_u else torch.zeros_like(self.U)
         self._tb = tb
 
     def _extrapolate(self):

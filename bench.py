@@ -531,26 +531,55 @@ def learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, pl
                          "validation_rel_sq_error": round(val, 4)}}
 
 
-def cpu_baseline(budget_s=20.0):
-    """Oracle SIMP iterations at 48x24x24 on one host core, scaled by element count to 192x96x96."""
-    import numpy as np
+def _host_cpu():
+    """Model name of the host CPU (from /proc/cpuinfo) and the number of cores the OS reports."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _oracle_iterations(shape, iters, r_mm):
+    """Setup seconds and per-iteration seconds of the FP64 oracle SIMP on the cantilever at `shape` (one thread)."""
     from threadpoolctl import threadpool_limits
     from oracle.optimize_ref import cantilever as cant_ref, simp_ref
-    shape = (48, 24, 24)
     g, fill, active, frozen, fixed, F = cant_ref(shape)
     with threadpool_limits(limits=1):
         t0 = time.time()
-        r = simp_ref(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, 0.3, iters=4, r_mm=1.5,
+        r = simp_ref(g, fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3, 0.3, iters=iters, r_mm=r_mm,
                      tol=0.0, emin=1e-9, rtol=1e-6, log=lambda *a: None)
         wall = time.time() - t0
-    secs = [h["seconds"] for h in r["history"][1:]]
-    s_it = float(np.mean(secs))
-    scale = (SHAPE[0] * SHAPE[1] * SHAPE[2]) / (shape[0] * shape[1] * shape[2])
-    return {"value": 1.0 / (s_it * scale), "unit": "SIMP iter/s", "cores": 1, "kind": "port",
+    secs = [h["seconds"] for h in r["history"]]
+    return wall - sum(secs), secs
+
+
+def cpu_baseline():
+    """The FP64 oracle's SIMP iterations on one host core: measured at 96x48x48 (iterations 1-2, setup apart)
+    with the benchmark's physics (rmin 3 voxels, Emin 1e-9, rtol 1e-6), and at 48x24x24 (iterations 1-4) for
+    the growth of the cost with size; the 96x48x48 rate is scaled by the element count (x8) to 192x96x96, whose
+    CSR assembly (about 40-50 GB, minutes of setup) does not fit a bench run."""
+    import numpy as np
+    setup48, s48 = _oracle_iterations((48, 24, 24), 4, 3.0)
+    setup96, s96 = _oracle_iterations((96, 48, 48), 2, 3.0)
+    it48, it96 = float(np.mean(s48)), float(np.mean(s96))
+    exponent = float(np.log(it96 / it48) / np.log(8.0))    # cost ~ elements^exponent between the two sizes
+    scale = (SHAPE[0] * SHAPE[1] * SHAPE[2]) / (96 * 48 * 48)
+    model, ncpu = _host_cpu()
+    return {"value": 1.0 / (it96 * scale), "unit": "SIMP iter/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle (CSR refill + scipy CG with its own Galerkin V-cycle, scipy.ndimage filter, "
-                      f"36-step OC) on the same cantilever at 48x24x24, iterations 2-4 (mean {s_it:.2f} s/iter, "
-                      f"{wall:.1f} s wall incl. setup), scaled x{scale:.0f} by element count to 192x96x96",
-            "s_per_iter_48x24x24": s_it}
+                      f"36-step OC), one thread, on the benchmark cantilever physics (rmin 3 voxels): 96x48x48 "
+                      f"iterations 1-2 measured at {it96:.2f} s/iter (setup {setup96:.1f} s apart), scaled x{scale:.0f} "
+                      f"by element count to 192x96x96; 48x24x24 iterations 1-4 at {it48:.2f} s/iter, so the cost grows "
+                      f"as elements^{exponent:.2f} between the measured sizes",
+            "s_per_iter_96x48x48": round(it96, 3), "s_per_iter_48x24x24": round(it48, 3),
+            "setup_s_96x48x48": round(setup96, 2), "size_exponent": round(exponent, 3),
+            "host_cpu": model, "host_cpu_count": ncpu}
 
 
 def main():
@@ -643,11 +672,18 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = 1e3 * dt / steps
+    cg_timed = int(sum(run.fem.cg_iters))
+    # the rest of the schedule, so that the whole continuation (p ramp, beta doubling) is measured too
+    t1 = time.perf_counter()
+    while run.it < iters:
+        run.step()
+    torch.cuda.synchronize()
+    full_s = dt + time.perf_counter() - t1
     cg = run.fem.cg_iters
     h = run.hist
-    print(f"bench: {steps} steps in {dt:.3f} s ({ms:.2f} ms/iter), setup {setup_s:.2f} s, "
-          f"PCG iterations total {sum(cg)} (min {min(cg)}, max {max(cg)}), final compliance "
-          f"{h[-1]['compliance']}", file=sys.stderr, flush=True)
+    print(f"bench: {steps} steps in {dt:.3f} s ({ms:.2f} ms/iter), setup {setup_s:.2f} s; whole {iters}-iteration "
+          f"schedule {1e3 * full_s / iters:.2f} ms/iter, PCG iterations total {sum(cg)} (min {min(cg)}, max {max(cg)}), "
+          f"final compliance {h[-1]['compliance']}", file=sys.stderr, flush=True)
     roof = apply_roofline(run.fem, reps=args.apply_reps)
     print(f"bench: apply {roof['us_per_launch']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
     lrn = None
@@ -665,7 +701,10 @@ def main():
         "vs_baseline": None, "dtype": "f32 operator/smoother, f64 dots/refinement/density",
         "data": "synthetic (no dataset)",
         "config": {"workload": workload, "solver": "plain MG-PCG", "iterations_timed": f"1-{steps}",
-                   "setup_s": round(setup_s, 3), "pcg_iters_total": int(sum(cg)), "pcg_iters_max": int(max(cg)),
+                   "setup_s": round(setup_s, 3), "pcg_iters_timed": cg_timed,
+                   "full_schedule": {"iterations": iters, "iter_per_s": round(iters / full_s, 4),
+                                     "ms_per_iter": round(1e3 * full_s / iters, 3)},
+                   "pcg_iters_total": int(sum(cg)), "pcg_iters_max": int(max(cg)),
                    "level_tiles": {"ids": [lv[3] for lv in run.fem.level_info()[:-1]],
                                    "shapes": [TILE_SHAPES[lv[3]] for lv in run.fem.level_info()[:-1]]},
                    "oc_volume_evals": int(run.oc_evals), "oc_trials": int(sum(x["ocSteps"] for x in h)),

@@ -13,6 +13,12 @@ Pinning (what checks this restatement independently of itself):
   * ``FEMRef.assemble``: equals a dense element-by-element sum on tiny grids.
   * ``FEMRef.solve``: dense ``numpy.linalg.solve`` on tiny grids, tension and
     Timoshenko windows (validate.py:75-98), determinism (validate.py:111-121).
+  * ``FEMRef.element_energy``: sum_e Ee_e ce_e = u^T K u against the CSR
+    assembly (per element and summed), = f.u at a solution, h^3 eps^T D eps for
+    constant-strain fields, and -ce = d(f.u)/dEe by central differences
+    (tests/test_oracle_sensitivity.py).
+  * ``FEMRef.stress`` / ``von_mises`` / ``normal_stress``: E D eps for
+    constant-strain fields; uniaxial, pure-shear and hydrostatic closed forms.
 The reference preconditions CG with pyamg (fem.py:212-215); pyamg is not
 installed, so this oracle preconditions scipy's CG with its own Galerkin
 geometric V-cycle.  The converged answer does not depend on that choice; the

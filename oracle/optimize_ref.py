@@ -12,7 +12,9 @@ against adjointness; the projection against its closed form and finite
 differences; the OC update against the bisection invariants (final volume
 within tolerance of the target, move limits, frozen = 1); the whole loop by
 V6 (validate.py:124-129: > 20 % stiffer than uniform, |vol err| < 1e-3) and
-the continuation schedule tables.
+the continuation schedule tables; ``element_sensitivity`` and ``combine_cases``
+by central differences of the objective through filter, projection, stiffness
+law and solve, for "sum" and the p-norm "worst" (tests/test_oracle_sensitivity.py).
 """
 import time
 

@@ -132,8 +132,8 @@ def test_trained_network_gain():
     """
     from anneal_amd import learned as Lm
     from anneal_amd.optimize import SimpRun
-    X, T, dims = Lm.generate_pairs(150, 40, seed=0)     # 6000 pairs: fewer overfit the 0.9 M parameters
-    Xv, Tv, _ = Lm.generate_pairs(6, 40, seed=5000)
+    X, T, dims = Lm.generate_pairs(150, 40, seed=0, kind="load")   # 6000 pairs: fewer overfit the 0.9 M parameters
+    Xv, Tv, _ = Lm.generate_pairs(6, 40, seed=5000, kind="load")
     trained = Lm.UNet(seed=0)
     untrained = Lm.UNet(seed=0)
     Lm.train(trained, X, T, dims, 1500, batch=32, lr=2e-3, seed=0)
