@@ -33,6 +33,7 @@ SIGNATURES = {
     "anneal_fem_set_level_tile": [c_void_p, c_int, c_int],
     "anneal_fem_set_option": [c_void_p, ctypes.c_char_p, c_double],
     "anneal_fem_vcycle": [c_void_p, c_void_p, c_void_p, c_int],
+    "anneal_fem_smooth_step_f32": [c_void_p, c_void_p, c_void_p, c_void_p, c_int],
     "anneal_fem_element_energy": [c_void_p, c_void_p, c_int, c_void_p],
     "anneal_fem_sensitivity": [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_double, c_double, c_double, c_void_p, c_void_p],

@@ -293,6 +293,18 @@ int anneal_fem_set_option(void* ctx, const char* name, double value) {
   });
 }
 
+int anneal_fem_smooth_step_f32(void* ctx, const float* u_dev, const float* b_dev, float* y_dev, int nrhs) {
+  return guarded([&] {
+    ANNEAL_CHECK(u_dev && b_dev && y_dev && (const float*)y_dev != u_dev && (const float*)y_dev != b_dev,
+                 "smoothing step needs u, b and a distinct in-place y");
+    FemHandle* h = fem_of(ctx);
+    ANNEAL_CHECK(h->flags, "smoothing steps run on the problems of a group");
+    ANNEAL_CHECK(nrhs >= 1 && nrhs <= 64, "nrhs out of range");
+    h->fem->smooth_step_f32(u_dev, b_dev, y_dev, nrhs);
+    return 0;
+  });
+}
+
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs) {
   return guarded([&] {
     ANNEAL_CHECK(r_dev && z_dev && r_dev != (const float*)z_dev, "vcycle needs distinct r and z");

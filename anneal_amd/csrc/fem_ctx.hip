@@ -593,6 +593,20 @@ void FemContext::apply_f32(const float* u, float* y, int nrhs) {
   a.y = y;
   launch_apply(a, kApply, nrhs, s_);
 }
+void FemContext::smooth_step_f32(const float* u, const float* b, float* y, int nrhs) {
+  set_rhs_count(nrhs);
+  const double lmax = cheb_lmax, lmin = lmax / cheb_ratio;
+  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sg = th / de;
+  const double rn = 1.0 / (2.0 * sg - 1.0 / sg);  // the recurrence's k = 1 coefficients (as in vcycle)
+  ApplyArgs<float> a = args<float>(0);
+  a.u = u;
+  a.b = b;
+  a.prev = y;
+  a.y = y;
+  a.c1 = float(rn / sg);
+  a.c2 = float(2.0 * rn / de);
+  launch_apply(a, kCheb, nrhs, s_);
+}
 void FemContext::apply_f64(const double* u, double* y, int nrhs) {
   set_rhs_count(nrhs);
   ApplyArgs<double> a = args<double>(0);

@@ -74,6 +74,10 @@ class FemContext {
   bool rebuild_each_solve = true;  // solve() refreshes a stale coarse inverse
   void apply_f32(const float* u, float* y, int nrhs);
   void apply_f64(const double* u, double* y, int nrhs);
+  // One fine-level Chebyshev smoothing step in place, y <- M (u + c1 (u - y) + c2 D^-1 (b - K u)), with
+  // the coefficients of the second step of the fine level's recurrence: the post-smoothing kernel
+  // of the V-cycle on its own, for measurement.
+  void smooth_step_f32(const float* u, const float* b, float* y, int nrhs);
   // F, U: device, [nrhs][3][ldn] FP64; U holds x0 when use_x0.  Returns 0 on
   // convergence, 1 when maxiter was exhausted, 2 on a non-finite result.
   int solve(const double* F, double* U, int nrhs, double rtol, int maxiter, bool use_x0, int* iters_out);

@@ -1,4 +1,4 @@
-"""SIMP iterations per second at 192x96x96 on one MI355X, plus the K·u apply roofline.
+"""SIMP iterations per second at 192x96x96 on one MI355X, plus the rooflines of its largest kernel and of K·u.
 
 Workload (one "step" = one SIMP iteration of the GPU path):
   cantilever 192x96x96 hex8 voxels (1.77 M elements, 5.4 M dofs), x=0 face clamped,
@@ -11,9 +11,12 @@ The default K = 100 is the whole schedule, so continuation (p ramp, beta
 doubling) and the harder late solves are all inside the timed region.
 
 Also measured:
-  * roofline: the FP32 matrix-free K·u apply at the same grid, timed with HIP
-    events on the stream it is launched on, over rotating buffers larger than
-    the 256 MB MALL so the bytes come from HBM;
+  * roofline: the SIMP step's largest kernel, the fine-level Chebyshev smoothing
+    step of the V-cycle (the FP32 matrix-free K·u apply with the smoothing
+    epilogue, 53 B/node), at the same grid, timed with HIP events on the
+    stream it is launched on, over rotating buffers larger than the 256 MB
+    MALL so the bytes come from HBM; roofline_ku: the same for the plain K·u
+    apply (24 B/node, the north-star K·u figure);
   * cpu_baseline: the FP64 CPU oracle (a restatement of the reference's
     CSR + CG + filter + OC loop) on a bounded sample.
 
@@ -213,13 +216,19 @@ def traffic_probe():
         for i in range(8):
             u, y = bufs[i]
             _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(u), _lib.ptr(y), 1)
+    # the fine level's Chebyshev step on the benchmark's tile shape: 8 launches on distinct (u, b, y) sets
+    fem.autotune(2)
+    bset = [torch.randn(1, 3, fem.ldn, device="cuda") for _ in range(8)]
+    for i in range(8):
+        u, y = bufs[i]
+        _lib.call("anneal_fem_smooth_step_f32", fem._ctx, _lib.ptr(u), _lib.ptr(bset[i]), _lib.ptr(y), 1)
     a = torch.empty(COPY_BYTES // 4, device="cuda")
     b = torch.empty_like(a)
     for _ in range(4):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(f"probe: {len(TILE_SHAPES)} tile shapes x 8 applies, 4 copies of {COPY_BYTES} bytes", file=sys.stderr,
-          flush=True)
+    print(f"probe: {len(TILE_SHAPES)} tile shapes x 8 applies, 8 Chebyshev steps (32x8x8), 4 copies of "
+          f"{COPY_BYTES} bytes", file=sys.stderr, flush=True)
 
 
 def unet_flops(dims, B, cin=8, cout=3, widths=(16, 32, 64, 64)):
@@ -492,6 +501,41 @@ def apply_roofline(fem, reps=20, nbuf=8):
             "dof_applies_per_s": 3 * g.nnode / (ms * 1e-3)}
 
 
+def chebyshev_roofline(fem, reps=20, nbuf=8):
+    """Average duration of one fine-level Chebyshev smoothing step (the SIMP step's dominant kernel) over
+    rotating (u, b, y) buffer sets, y read as the previous iterate and overwritten in place."""
+    import torch
+    from anneal_amd import _lib
+    g = fem.grid
+    tile = TILE_SHAPES[fem.tile()] + ",kCheb"
+    sets = [tuple(torch.randn(1, 3, fem.ldn, device="cuda", dtype=torch.float32) for _ in range(3))
+            for _ in range(nbuf)]
+    for u, b, y in sets:         # warm
+        _lib.call("anneal_fem_smooth_step_f32", fem._ctx, _lib.ptr(u), _lib.ptr(b), _lib.ptr(y), 1)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    n = 0
+    for _ in range(reps):
+        for u, b, y in sets:
+            _lib.call("anneal_fem_smooth_step_f32", fem._ctx, _lib.ptr(u), _lib.ptr(b), _lib.ptr(y), 1)
+            n += 1
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    # u, b, previous iterate y read (12 B/node each), D^-1 (4 B/node), free-dof mask (1 B/node), y written
+    # (12 B/node); Ee (4 B/elem)
+    nbytes = 53 * g.nnode + 4 * g.nelem
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    kernel = f"k_apply<float,{tile}>"
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, (g.nx, g.ny, g.nz)),
+            "kernel": kernel, "bytes_per_launch": nbytes, "us_per_launch": round(ms * 1e3, 2),
+            "bytes_formula": "53 B/node (u, b, y in; D^-1 4; mask 1; y out) + 4 B/element (Ee)",
+            "share_of_simp_kernel_time": "13 % (the largest kernel of the step; 3 launches per PCG iteration)"}
+
+
 def learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, plain_ms, runs=200, train_steps=2000,
                 mode=1, log=None):
     """Train the U-Net on the GPU (pairs from seeded 48x24x24 cantilever SIMP runs) and time the same SIMP
@@ -684,8 +728,10 @@ def main():
     print(f"bench: {steps} steps in {dt:.3f} s ({ms:.2f} ms/iter), setup {setup_s:.2f} s; whole {iters}-iteration "
           f"schedule {1e3 * full_s / iters:.2f} ms/iter, PCG iterations total {sum(cg)} (min {min(cg)}, max {max(cg)}), "
           f"final compliance {h[-1]['compliance']}", file=sys.stderr, flush=True)
-    roof = apply_roofline(run.fem, reps=args.apply_reps)
-    print(f"bench: apply {roof['us_per_launch']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
+    roof_ku = apply_roofline(run.fem, reps=args.apply_reps)
+    print(f"bench: apply {roof_ku['us_per_launch']} us, {roof_ku['achieved']} GB/s", file=sys.stderr, flush=True)
+    roof = chebyshev_roofline(run.fem, reps=args.apply_reps)
+    print(f"bench: Chebyshev step {roof['us_per_launch']} us, {roof['achieved']} GB/s", file=sys.stderr, flush=True)
     lrn = None
     if not args.no_learned and args.workload in ("cantilever", "cantilever128"):
         lrn = learned_leg(g, fill, active, frozen, fixed, loads, weights, iters, steps, ms, mode=args.learned_mode,
@@ -710,7 +756,7 @@ def main():
                    "oc_volume_evals": int(run.oc_evals), "oc_trials": int(sum(x["ocSteps"] for x in h)),
                    "final_compliance": h[-1]["compliance"], "final_volume_fraction": h[-1]["volumeFraction"],
                    "learned": lrn},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "roofline_ku": roof_ku, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

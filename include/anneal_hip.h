@@ -114,6 +114,11 @@ int anneal_fem_set_level_tile(void* ctx, int level, int variant);
 int anneal_fem_set_option(void* ctx, const char* name, double value);
 /* One V-cycle of the multigrid preconditioner on FP32 vectors (z = M r). */
 int anneal_fem_vcycle(void* ctx, const float* r_dev, float* z_dev, int nrhs);
+/* One fine-level Chebyshev smoothing step of that V-cycle on FP32 vectors, in
+ * place: y <- M (u + c1 (u - y) + c2 D^-1 (b - K u)) (the post-smoothing
+ * kernel, the largest share of the solve's kernel time; no reference
+ * counterpart: for measurement). */
+int anneal_fem_smooth_step_f32(void* ctx, const float* u_dev, const float* b_dev, float* y_dev, int nrhs);
 /* FEM.element_energy (fem.py:239-242): ce[r*nelem + e] = u_e^T ke u_e, E = 1. */
 int anneal_fem_element_energy(void* ctx, const double* U_dev, int nrhs, double* ce_dev);
 /* Sensitivities of optimize.py:118,125-142 fused: with

@@ -297,3 +297,22 @@ def test_stress_and_von_mises_match_oracle(scalar_E):
     d = rng.normal(size=3)
     assert np.allclose(FEM.normal_stress(sig, d), ref.normal_stress(sig_ref, d), rtol=1e-12, atol=0)
     fem.close()
+
+
+def test_chebyshev_step_fixed_point(torch):
+    """anneal_fem_smooth_step_f32 (the V-cycle's fine-level Chebyshev step, in place): with b = K u and the
+    previous iterate equal to u, y = M (u + c1 (u - u) + c2 D^-1 (b - K u)) returns u on the free dofs."""
+    from anneal_amd import _lib
+    g, active, fixed, Ee = random_problem((40, 20, 16), 12, 0.1)
+    fem = _product_fem(g, active, fixed)
+    fem.assemble(Ee[fem.act])
+    mask = torch.from_numpy(((fem.dof_mask[:, None] >> np.arange(3)) & 1).T.astype(np.float32)).cuda()
+    U = torch.zeros(1, 3, fem.ldn, dtype=torch.float32, device="cuda")
+    U[0, :, :g.nnode] = torch.randn(3, g.nnode, device="cuda") * mask
+    B = torch.zeros_like(U)
+    _lib.call("anneal_fem_apply_f32", fem._ctx, _lib.ptr(U), _lib.ptr(B), 1)
+    Y = U.clone()
+    _lib.call("anneal_fem_smooth_step_f32", fem._ctx, _lib.ptr(U), _lib.ptr(B), _lib.ptr(Y), 1)
+    torch.cuda.synchronize()
+    d = (Y - U)[0, :, :g.nnode].abs().max().item()
+    assert d <= 1e-5 * U.abs().max().item(), d
