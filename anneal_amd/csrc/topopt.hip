@@ -42,22 +42,44 @@ struct FilterArgs {
   double* stg;       // staged values of the gather path (nelem doubles)
   int R;
   double beta, move;
+  double pt0, pinv;  // projection constants tanh(beta eta) and 1 / (tanh(beta eta) + tanh(beta (1 - eta))) (set_beta)
   int nx, ny, nz;
 };
 
+// The projection's two element-independent tanh values are formed once on the
+// host, in the order optimize.py:85-87 uses.
+static void set_beta(FilterArgs& a, double beta) {
+  const double eta = 0.5;
+  a.beta = beta;
+  a.pt0 = beta > 0.0 ? std::tanh(beta * eta) : 0.0;
+  a.pinv = beta > 0.0 ? 1.0 / (a.pt0 + std::tanh(beta * (1.0 - eta))) : 1.0;
+}
+
 constexpr int FTX = 32, FTY = 8, FZC = 8;
 
-__device__ __forceinline__ void project(double xt, double beta, double& xp, double& dxp) {
-  if (beta <= 0.0) {
+// tanh(y) = sign(y) (1 - e) / (1 + e) with e = exp(-2|y|), written with
+// m = expm1(-2|y|) in (-1, 0] as -m / (2 + m) so that small |y| keeps its
+// relative accuracy: within 3 ulp of the C library's tanh over |y| <= 40 and
+// at 1e-3 scale, exact 0 and +-1 at the ends; one expm1 and one division
+// instead of the library tanh's longer evaluation (this is the per-element
+// cost of every projected filter launch).
+__device__ __forceinline__ double tanh_expm1(double y) {
+  const double m = expm1(-2.0 * fabs(y));
+  return copysign(-m / (2.0 + m), y);
+}
+
+// Heaviside projection of optimize.py:81-87 (eta = 1/2) and its derivative;
+// pt0 = tanh(beta eta) and pinv = 1 / (pt0 + tanh(beta (1 - eta))) come from
+// set_beta (a multiply by pinv instead of the division: within 1 ulp more).
+__device__ __forceinline__ void project(double xt, const FilterArgs& a, double& xp, double& dxp) {
+  if (a.beta <= 0.0) {
     xp = xt;
     dxp = 1.0;
     return;
   }
-  const double eta = 0.5;
-  const double t0 = tanh(beta * eta), t1 = tanh(beta * (1.0 - eta));
-  const double th = tanh(beta * (xt - eta));
-  xp = (t0 + th) / (t0 + t1);
-  dxp = beta * (1.0 - th * th) / (t0 + t1);
+  const double th = tanh_expm1(a.beta * (xt - 0.5));
+  xp = (a.pt0 + th) * a.pinv;
+  dxp = a.beta * (1.0 - th * th) * a.pinv;
 }
 
 // z-marching filter.  A workgroup owns an FTX x FTY tile of element columns
@@ -77,8 +99,12 @@ __device__ __forceinline__ void project(double xt, double beta, double& xp, doub
 // Sum the volume partials in a fixed order, record the evaluation in the
 // bracket, replay every bisection trial it decides, and choose the next
 // multiplier to evaluate.
-constexpr double kOcEps = 0.02;          // default half-width of the warm bracket around a hint
-constexpr double kOcSecantWidth = 0.25;  // log-width below which the root is interpolated
+constexpr double kOcTau = 1e-12;       // log-step past the root estimate that closes the bracket
+// Evaluations queued by oc_begin and by each oc_poll that finds the bisection
+// still running (launches after the stop only carry the state forward, ~4 us
+// each): the replay needs 5-6 evaluations on most SIMP iterations.
+constexpr int kOcFirstBatch = 7, kOcNextBatch = 4;
+constexpr double kOcGrowStep = 0.005;  // default log step from a one-sided first evaluation
 
 __device__ void oc_record(double* oc, double lam, double f) {
   if (f > 0.0) {
@@ -119,11 +145,63 @@ __device__ double oc_volume(const double* __restrict__ part, int nblk, double* s
   return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// The multiplier to evaluate next when the reference's trial lam lies inside
+// the bracket (or on the unknown side).  Only which multipliers are evaluated
+// depends on this choice, never a decision or the result.
+__device__ double oc_choose(const double* oc, double lo, double hi, double lam) {
+  const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
+  const double n = oc[kOcEvals];
+  const double t1 = oc[kOcT1], f1 = oc[kOcF1], t2 = oc[kOcT2], f2 = oc[kOcF2];
+  const double ta = hasA ? log(oc[kOcA]) : 0.0, tb = hasB ? log(oc[kOcB]) : 0.0;
+  const double tlo = log(lo), thi = log(hi);
+  // still one-sided after several evaluations (a target the bracket may not
+  // contain): the far end of the trial interval decides every remaining trial
+  // if the root is not inside it
+  if (n >= 6.0 && hasA != hasB) return hasA ? hi : lo;
+  double t = 0.0;
+  bool have = false;
+  if (n >= 2.0 && f1 != f2) {
+    t = t1 - f1 * (t1 - t2) / (f1 - f2);  // secant through the last two evaluations
+    have = true;
+  } else if (n == 1.0 && oc[kOcSlope] < 0.0) {
+    t = t1 - f1 / oc[kOcSlope];  // Newton step with the previous bisection's slope
+    have = true;
+  }
+  if (have && hasA && hasB) {
+    const double tau = fmin(kOcTau, 0.25 * (tb - ta));
+    // outside the bracket: the Illinois point of [A, B] instead
+    if (!(t > ta && t < tb)) t = ta - oc[kOcFA] * (tb - ta) / (oc[kOcFB] - oc[kOcFA]);
+    // an estimate next to the last evaluation: step just past it, across the root
+    if (fabs(t - t1) < tau) t = t1 + (f1 > 0.0 ? tau : -tau);
+    t = fmin(fmax(t, ta + tau), tb - tau);
+  } else if (have) {
+    if (hasA) t = fmax(t, ta);
+    if (hasB) t = fmin(t, tb);
+  }
+  auto inside = [&](double u) {
+    return isfinite(u) && u > tlo && u < thi && (!hasA || u > ta) && (!hasB || u < tb);
+  };
+  if (have && inside(t)) return exp(t);
+  if (n == 1.0 && hasA != hasB) {
+    // one side known and no usable slope: a fixed log step toward the root
+    const double u = t1 + (hasA ? oc[kOcGrow] : -oc[kOcGrow]);
+    if (inside(u)) return exp(u);
+  }
+  // one-sided and the secant leaves the interval: test its far end
+  if (n >= 2.0 && hasA != hasB) return hasA ? hi : lo;
+  return lam;
+}
+
 // One bisection step for the evaluation vol = vol(oc[kOcLam]) (one thread).
 __device__ void oc_step(double* oc, double vol) {
+  const double lamE = oc[kOcLam], f = vol - oc[kOcVt];
   oc[kOcVol] = vol;
   oc[kOcEvals] += 1.0;
-  oc_record(oc, oc[kOcLam], vol - oc[kOcVt]);
+  oc[kOcT2] = oc[kOcT1];
+  oc[kOcF2] = oc[kOcF1];
+  oc[kOcT1] = log(lamE);
+  oc[kOcF1] = f;
+  oc_record(oc, lamE, f);
   double lo = oc[kOcLo], hi = oc[kOcHi];
   for (int guard = 0; guard < 200; ++guard) {
     const double lam = sqrt(lo * hi);  // the reference's next trial
@@ -135,49 +213,22 @@ __device__ void oc_step(double* oc, double vol) {
       else hi = lam;
       oc[kOcLamLast] = lam;
       oc[kOcSteps] += 1.0;
-      oc[kOcTries] = 0.0;
       if (hi / lo < 1.0 + 1e-9 || oc[kOcSteps] >= 100.0) {
         oc[kOcDone] = 1.0;
         break;
       }
       continue;
     }
-    // undecided: pick the next multiplier to evaluate
-    double next = lam;
-    const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
-    const double stage = oc[kOcStage];
-    if (stage == 1.0) {
-      // second point of the warm bracket around the hint
-      const double c = oc[kOcHint] * (1.0 + oc[kOcWidth]);
-      if (c > lo && c < hi) next = c;
-      oc[kOcStage] = 2.0;
-    } else if (stage == 2.0 && (hasA != hasB)) {
-      // one side known: step outward geometrically from it
-      const double g = oc[kOcGrow];
-      const double c = hasA ? oc[kOcA] * g : oc[kOcB] / g;
-      if (c > lo && c < hi) next = c;
-      else oc[kOcStage] = 3.0;
-      oc[kOcGrow] = g * g;
-    } else if ((hasA != hasB) && oc[kOcSteps] >= 4.0 && oc[kOcEnds] == 0.0) {
-      // still one-sided after several trials: test the far end of the initial
-      // bracket; if it is on the same side, every remaining trial is decided
-      next = hasA ? oc[kOcHi0] : oc[kOcLo0];
-      oc[kOcEnds] = 1.0;
-    } else if (hasA && hasB && log(oc[kOcB] / oc[kOcA]) < kOcSecantWidth && oc[kOcTries] < 3.0) {
-      // narrow bracket: Illinois (regula falsi) step in log(lam)
-      const double ta = log(oc[kOcA]), tb = log(oc[kOcB]);
-      const double fa = oc[kOcFA], fb = oc[kOcFB];
-      double t = ta - fa * (tb - ta) / (fb - fa);
-      if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
-      const double c = exp(t);
-      if (c > oc[kOcA] && c < oc[kOcB]) next = c;
-      oc[kOcTries] += 1.0;
-    }
-    oc[kOcLam] = next;
+    oc[kOcLam] = oc_choose(oc, lo, hi, lam);
     break;
   }
   oc[kOcLo] = lo;
   oc[kOcHi] = hi;
+  if (oc[kOcDone] != 0.0) {
+    // the slope handed to the next bisection: from the last two evaluations
+    const double dt = oc[kOcT1] - oc[kOcT2], s = dt != 0.0 ? (oc[kOcF1] - oc[kOcF2]) / dt : 0.0;
+    oc[kOcSlopeNext] = (oc[kOcEvals] >= 2.0 && s < 0.0 && isfinite(s)) ? s : oc[kOcSlope];
+  }
 }
 
 // number of (dy', dx') offsets in [-R, R]^2 with dy'^2 + dx'^2 = m
@@ -361,7 +412,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
       } else {
         double xp = 0.0, dxp = 0.0;
         if (act) {
-          project(v * E.ih, a.beta, xp, dxp);
+          project(v * E.ih, a, xp, dxp);
           if (E.f & kFrozen) xp = 1.0;
           vol += E.ph * xp;
         }
@@ -468,7 +519,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter_gather(FilterArgs a) {
     } else {
       double xp = 0.0, dxp = 0.0;
       if (act) {
-        project(v * a.invHs[e], a.beta, xp, dxp);
+        project(v * a.invHs[e], a, xp, dxp);
         if (f & kFrozen) xp = 1.0;
         vol += a.phi[e] * xp;
       }
@@ -659,7 +710,7 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
   a.out = xphys;
   a.out2 = dxp;
   a.part = part_;
-  a.beta = beta;
+  set_beta(a, beta);
   launch_filter(kPhys, R_, a, nblk_, s_);
   if (vol_dev) finalize(part_, nblk_, 1, vol_dev, s_);
 }
@@ -667,29 +718,22 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
 void FilterContext::oc_begin(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
                              double move, double lo, double hi, double vtarget, double* xnew, double hint,
                              double width) {
-  const double eps = width > 0.0 ? std::min(width, 0.5) : kOcEps;
   // the initial state goes up from the pinned upper half of host_ (k_oc_prep
   // reads it, possibly after this call returns)
   double* st = host_ + kOcLen;
   std::fill(st, st + kOcLen, 0.0);
   st[kOcLo] = lo;
   st[kOcHi] = hi;
-  const bool warm = hint * (1.0 - eps) > lo && hint * (1.0 + eps) < hi;
-  st[kOcLam] = warm ? hint * (1.0 - eps) : std::sqrt(lo * hi);
-  st[kOcStage] = warm ? 1.0 : 3.0;
-  st[kOcHint] = hint;
-  st[kOcWidth] = eps;
-  st[kOcLo0] = lo;
-  st[kOcHi0] = hi;
+  st[kOcLam] = (hint > lo && hint < hi) ? hint : std::sqrt(lo * hi);
   st[kOcVt] = vtarget;
-  // outward steps after a miss start a few bracket widths away and square each time
-  st[kOcGrow] = 1.0 + std::min(0.25, std::max(0.02, 10.0 * eps));
+  st[kOcGrow] = width > 0.0 ? std::min(width, 0.5) : kOcGrowStep;
+  st[kOcSlope] = oc_slope_;
   const int64_t ne = g_.nelem();
   hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_,
                      host_dev_ + kOcLen, oc_);
   ANNEAL_HIP(hipGetLastError());
   run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true, oc_};
-  oc_launch(12);
+  oc_launch(kOcFirstBatch);
 }
 
 void FilterContext::oc_launch(int count) {
@@ -698,7 +742,7 @@ void FilterContext::oc_launch(int count) {
   a.dcx = r0_;
   a.dvx = nullptr;
   a.phi = run_.phi;
-  a.beta = run_.beta;
+  set_beta(a, run_.beta);
   a.move = run_.move;
   // Evaluation e reads the state in oc_[e % 2], first takes the step of
   // evaluation e - 1 from the partials in part_[(e - 1) % 2] (except the first
@@ -733,7 +777,7 @@ bool FilterContext::oc_poll() {
     run_.active = false;
     throw std::runtime_error("OC bisection did not terminate");
   }
-  oc_launch(8);
+  oc_launch(kOcNextBatch);
   return false;
 }
 
@@ -744,6 +788,7 @@ int FilterContext::oc_end(int* evals, double* lam_last) {
   hipLaunchKernelGGL(k_oc_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s_, run_.x, run_.dcx, run_.dvx,
                      flags_, run_.state, run_.beta, run_.move, n, run_.xnew);
   ANNEAL_HIP(hipGetLastError());
+  oc_slope_ = host_[kOcSlopeNext];
   if (evals) *evals = (int)host_[kOcEvals];
   if (lam_last) *lam_last = host_[kOcLamLast];
   return (int)host_[kOcSteps];

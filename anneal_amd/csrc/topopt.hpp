@@ -12,11 +12,14 @@ constexpr uint8_t kActive = 1, kFrozen = 2;
 
 // OC bisection state on the device (FP64).  kOcLam is the multiplier the next
 // trial-volume kernel evaluates; (lo, hi) is the reference's bisection state;
-// [A, B] is a certified bracket of the root of vol(lam) = vt.
+// [A, B] is a certified bracket of the root of vol(lam) = vt (FA, FB: vol - vt
+// there); (T1, F1), (T2, F2) are log(lam) and vol - vt of the last two
+// evaluations; Slope is d(vol)/d(log lam) carried over from the previous
+// bisection (0: none), SlopeNext the one this bisection hands on; Grow is the
+// log step taken from a one-sided first evaluation when no slope is known.
 constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7,
-              kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcTries = 15,
-              kOcEvals = 16, kOcGrow = 17, kOcStage = 18, kOcHint = 19, kOcLo0 = 20, kOcHi0 = 21, kOcEnds = 22,
-              kOcWidth = 23;
+              kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcEvals = 15,
+              kOcGrow = 16, kOcT1 = 17, kOcF1 = 18, kOcT2 = 19, kOcF2 = 20, kOcSlope = 21, kOcSlopeNext = 22;
 constexpr int kOcLen = 24;
 
 struct FilterArgs;
@@ -38,11 +41,15 @@ class FilterContext {
   // The geometric bisection of optimize.py:144-161 on the device, replayed
   // exactly (same trial multipliers, decisions and stop) from fewer volume
   // evaluations: vol(lam) is non-increasing, so a certified bracket [A, B] of
-  // its root decides every trial outside it; inside, Illinois steps in log(lam)
-  // narrow the bracket first.  hint > 0 is a guess of the root (e.g. the
-  // previous iteration's): the first two evaluations sit at hint (1 -+ width)
-  // (width <= 0: 0.02).  xnew receives the last trial design; returns the
-  // number of trials (the reference's count).
+  // its root decides every trial outside it.  The multipliers evaluated follow
+  // a safeguarded secant in log(lam): the first at hint > 0 (a guess of the
+  // root, e.g. the previous iteration's), the second a Newton step with the
+  // slope the previous bisection of this context measured (or a log step of
+  // width, <= 0: 0.005, toward the root), then secant steps through the last
+  // two evaluations kept inside [A, B]; an estimate within 1e-12 of the last
+  // evaluation is stepped 1e-12 past it, so the bracket closes on the root.
+  // xnew receives the last trial design; returns the number of trials (the
+  // reference's count).
   int oc_bisect(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
                 double lo, double hi, double vtarget, double* xnew, double hint = 0.0, double width = 0.0,
                 int* evals = nullptr, double* lam_last = nullptr);
@@ -84,6 +91,7 @@ class FilterContext {
     bool active;
     double* state;  // the state after the last batch (one of the two buffers in oc_)
   } run_{};
+  double oc_slope_ = 0.0;  // d(vol)/d(log lam) measured by the last bisection (0: none yet)
   hipEvent_t oc_ready_ = nullptr;  // recorded after each state read-back
   void oc_launch(int count);
   FilterArgs args() const;  // launch arguments common to every mode

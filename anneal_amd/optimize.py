@@ -82,7 +82,8 @@ class DeviceFilter:
 
     def oc_bisect(self, x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint=0.0, info=None, width=0.0):
         """Replayed OC bisection; returns the number of trials.  ``info`` (a dict) gets evals and lam_last.
-        ``hint`` / ``width``: guess of the root and relative half-width of the first bracket around it."""
+        ``hint``: guess of the root (the first multiplier evaluated); ``width``: log step of the second
+        evaluation when the context has no slope from an earlier bisection (<= 0: 0.005)."""
         steps, evals, lam = _lib.ctypes.c_int(), _lib.ctypes.c_int(), _lib.c_double()
         _lib.call("anneal_oc_bisect", self._ctx, _lib.ptr(x), _lib.ptr(dcx), _lib.ptr(dvx), _lib.ptr(phi),
                   float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), float(width),

@@ -147,10 +147,13 @@ int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const do
  * bracket [lo, hi].  The trial multipliers, the decisions vol > vtarget and
  * the stop hi/lo < 1+1e-9 are the reference's; since vol(lam) is
  * non-increasing, trials outside a certified bracket of its root are decided
- * without a volume evaluation.  hint (0 = none) is a guess of the root: the
- * first two evaluations sit at hint (1 -+ hint_width) (hint_width <= 0: 0.02);
- * it changes which multipliers are evaluated, never the trial decisions or the
- * result.  xnew_dev receives the last trial design, *steps_host the
+ * without a volume evaluation.  The multipliers evaluated follow a safeguarded
+ * secant in log(lam): the first at hint (0 = none: the reference's first
+ * trial), the second a Newton step with the slope d(vol)/d(log lam) that the
+ * context's previous bisection measured, or, without one, a log step of
+ * hint_width (<= 0: 0.005) toward the root.  These change which multipliers
+ * are evaluated, never the trial decisions or the result.  xnew_dev receives
+ * the last trial design, *steps_host the
  * number of trials, *evals_host the volume evaluations spent (may be NULL) and
  * *lam_last_host the last trial multiplier (may be NULL). */
 int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
