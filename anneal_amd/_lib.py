@@ -46,6 +46,8 @@ SIGNATURES = {
                          c_double, c_double, c_double, c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],
     "anneal_oc_begin": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double,
                         c_double, c_double, c_double, c_void_p],
+    "anneal_oc_begin_stats": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                              c_double, c_double, c_double, c_void_p],
     "anneal_oc_poll": [c_void_p, c_intp],
     "anneal_oc_end": [c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],
     "anneal_stiffness_law": [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_int64, c_void_p,

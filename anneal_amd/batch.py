@@ -229,9 +229,8 @@ class BatchSimp:
         # the launches are those of SimpRun.step
         for r, i in zip(runs, its):
             r._sensitivities(i)
-        self._sync(runs)
         for r in runs:
-            r._oc_begin()
+            r._oc_begin()          # bracketed on the device: no wait for the statistics
         pending = runs
         while pending:
             pending = [r for r in pending if not r.filt.oc_poll()]

@@ -202,8 +202,9 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
         st.hst[r * kSt + kDone] = st.done[r] ? 1.0 : 0.0;
         st.idone[r] = st.done[r];
       }
-      ANNEAL_HIP(hipMemcpyAsync(f.scale_, st.sc.data(), nr * sizeof(double), hipMemcpyHostToDevice, s_));
-      ANNEAL_HIP(hipMemcpyAsync(f.st_, st.hst.data(), st.hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+      std::copy(st.sc.begin(), st.sc.begin() + nr, f.host_up_);
+      std::copy(st.hst.begin(), st.hst.end(), f.host_up_ + 64);
+      upload(f.host_up_dev_, f.scale_, nr, f.host_up_dev_ + 64, f.st_, (int)st.hst.size(), s_);
       to_f32_scaled(f.r64_, f.pr_, f.scale_, n, ld, nr, s_);
       ANNEAL_HIP(hipMemsetAsync(f.px_, 0, (size_t)nr * 3 * ld * sizeof(float), s_));
       f.vcycle_pcg(nr);  // skips the rhs that already meet the tolerance
@@ -275,8 +276,8 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
         continue;
       }
       const GridDims& g0 = f.lv_[0].g;
-      for (int r = 0; r < nrhs[p]; ++r) st.sc[r] = st.done[r] ? 0.0 : st.rnorm[r];
-      ANNEAL_HIP(hipMemcpyAsync(f.scale_, st.sc.data(), nrhs[p] * sizeof(double), hipMemcpyHostToDevice, s_));
+      for (int r = 0; r < nrhs[p]; ++r) f.host_up_[r] = st.done[r] ? 0.0 : st.rnorm[r];
+      upload(f.host_up_dev_, f.scale_, nrhs[p], nullptr, nullptr, 0, s_);
       axpy_f32_to_f64(f.px_, U[p], f.scale_, g0.nnode(), g0.ldn(), nrhs[p], s_);
     }
   }

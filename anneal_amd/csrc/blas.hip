@@ -414,6 +414,16 @@ void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, 
   ANNEAL_HIP(hipGetLastError());
 }
 
+__global__ __launch_bounds__(256) void k_upload(const double* __restrict__ src1, double* __restrict__ dst1, int n1,
+                                                const double* __restrict__ src2, double* __restrict__ dst2, int n2) {
+  for (int i = threadIdx.x; i < n1; i += 256) dst1[i] = src1[i];
+  for (int i = threadIdx.x; i < n2; i += 256) dst2[i] = src2[i];
+}
+void upload(const double* src1, double* dst1, int n1, const double* src2, double* dst2, int n2, hipStream_t s) {
+  hipLaunchKernelGGL(k_upload, dim3(1), dim3(256), 0, s, src1, dst1, n1, src2, dst2, n2);
+  ANNEAL_HIP(hipGetLastError());
+}
+
 // Refinement glue: r32 = scale_r * r64 ; x64 += inv_scale_r * x32
 __global__ void k_to_f32_scaled(const double* __restrict__ a, float* __restrict__ b, const double* __restrict__ sc,
                                 int64_t n, int64_t ld) {
@@ -453,6 +463,19 @@ __global__ void k_cast_f64_f32(const double* __restrict__ a, float* __restrict__
 void cast_f64_f32(const double* a, float* b, int64_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_cast_f64_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+  ANNEAL_HIP(hipGetLastError());
+}
+
+__global__ void k_copy_cast_f64(const double* __restrict__ a, double* __restrict__ a64, float* __restrict__ b,
+                                int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = a[i];
+  a64[i] = v;
+  b[i] = (float)v;
+}
+void copy_cast_f64(const double* a, double* a64, float* b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_cast_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, a64, b, n);
   ANNEAL_HIP(hipGetLastError());
 }
 

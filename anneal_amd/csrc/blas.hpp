@@ -29,8 +29,19 @@ int apply_blocks(int nx, int ny, int nz, int variant);
 constexpr int kApplyVariant64 = 0;  // the one tile shape of the FP64 apply (register pressure)
 int apply_variants();
 
-void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
-                     hipStream_t s);
+// Jacobi diagonals of up to kMax multigrid levels in one launch (first[] is
+// filled by the launcher).
+struct DiagLevels {
+  static constexpr int kMax = 8;
+  const float* Ee[kMax];
+  const uint8_t* mask[kMax];
+  float* dinv[kMax];
+  int nx[kMax], ny[kMax], nz[kMax];
+  double kdiag[kMax];
+  int64_t first[kMax + 1];
+  int nlev;
+};
+void launch_inv_diag(DiagLevels d, hipStream_t s);
 void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, float q, hipStream_t s);
 // With ops set (a device table of the level the mask belongs to), rhs r uses
 // ops[r / rpo].mask instead of the mask argument.
@@ -64,7 +75,12 @@ void pcg_update(float* x, float* r, const float* p, const float* q, const double
                 double* out_host = nullptr);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
 void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
+// dst1[0, n1) = src1[0, n1) and dst2[0, n2) = src2[0, n2) in one small kernel;
+// src1 / src2 are device aliases of pinned host memory (n2 = 0: no second copy)
+void upload(const double* src1, double* dst1, int n1, const double* src2, double* dst2, int n2, hipStream_t s);
 void axpy_f32_to_f64(const float* x, double* y, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
 void cast_f64_f32(const double* a, float* b, int64_t n, hipStream_t s);
+// a64 = a and b = (float)a in one pass
+void copy_cast_f64(const double* a, double* a64, float* b, int64_t n, hipStream_t s);
 
 }  // namespace anneal

@@ -429,6 +429,17 @@ int anneal_oc_begin(void* ctx, const double* x_dev, const double* dcx_dev, const
   });
 }
 
+int anneal_oc_begin_stats(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                          const double* phi_dev, double beta, double move, const double* stats_dev,
+                          double hint_ratio, double vtarget, double hint_width, double* xnew_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(x_dev && dcx_dev && dvx_dev && phi_dev && stats_dev && xnew_dev, "null argument");
+    filt_of(ctx)->oc_begin_stats(x_dev, dcx_dev, dvx_dev, phi_dev, beta, move, stats_dev, hint_ratio, vtarget,
+                                 xnew_dev, hint_width);
+    return 0;
+  });
+}
+
 int anneal_oc_poll(void* ctx, int* done_host) {
   return guarded([&] {
     ANNEAL_CHECK(done_host, "null argument");

@@ -275,6 +275,8 @@ void FemContext::alloc_solver(int nrhs_max) {
   scale_ = (double*)alloc(64 * sizeof(double));
   ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_red_dev_, host_red_, 0));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_up_, (64 + 64 * kSt) * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_up_dev_, host_up_, 0));
   ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
   ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   vgraph_.assign(nrhs_max + 1, nullptr);
@@ -518,6 +520,7 @@ FemContext::~FemContext() {
   if (ev_) (void)hipEventDestroy(ev_);
   for (void* p : owned_) (void)hipFree(p);
   if (host_red_) (void)hipHostFree(host_red_);
+  if (host_up_) (void)hipHostFree(host_up_);
 }
 
 template <>
@@ -561,11 +564,23 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
 void FemContext::set_Ee(const double* Ee_dev) {
   ANNEAL_CHECK(!is_group(), "set the stiffness on the problems of a group, not on the group");
   const GridDims& g0 = lv_[0].g;
-  ANNEAL_HIP(hipMemcpyAsync(Ee64_, Ee_dev, g0.nelem() * sizeof(double), hipMemcpyDeviceToDevice, s_));
-  cast_f64_f32(Ee64_, lv_[0].Ee, g0.nelem(), s_);
+  copy_cast_f64(Ee_dev, Ee64_, lv_[0].Ee, g0.nelem(), s_);
   for (size_t l = 1; l < lv_.size(); ++l)
     launch_coarsen_Ee(lv_[l - 1].Ee, lv_[l].Ee, lv_[l - 1].g, lv_[l].g, (float)coarsen_power, s_);
-  for (size_t l = 0; l < lv_.size(); ++l) launch_inv_diag(lv_[l].Ee, lv_[l].mask, lv_[l].dinv, lv_[l].g, lv_[l].kdiag, s_);
+  DiagLevels d{};
+  d.nlev = (int)lv_.size();
+  ANNEAL_CHECK(d.nlev <= DiagLevels::kMax, "too many multigrid levels");
+  for (int l = 0; l < d.nlev; ++l) {
+    const Level& L = lv_[l];
+    d.Ee[l] = L.Ee;
+    d.mask[l] = L.mask;
+    d.dinv[l] = L.dinv;
+    d.nx[l] = L.g.nx;
+    d.ny[l] = L.g.ny;
+    d.nz[l] = L.g.nz;
+    d.kdiag[l] = L.kdiag;
+  }
+  launch_inv_diag(d, s_);
   if (!tuned_ && g0.nelem() >= 4096) {
     tuned_ = true;
     autotune(-1, nullptr);  // first operator on a large grid: pick the fastest tile shape
@@ -823,8 +838,9 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
       idone[r] = done[r];
     }
-    ANNEAL_HIP(hipMemcpyAsync(scale_, sc.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s_));
-    ANNEAL_HIP(hipMemcpyAsync(st_, hst.data(), hst.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    std::copy(sc.begin(), sc.end(), host_up_);
+    std::copy(hst.begin(), hst.end(), host_up_ + 64);
+    upload(host_up_dev_, scale_, nrhs, host_up_dev_ + 64, st_, (int)hst.size(), s_);
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
     if (net_ && net_mode_ == 1) learned_warm_start(nrhs);
@@ -896,8 +912,8 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       if (!speculate) next_direction();
     }
     if (bad) break;
-    for (int r = 0; r < nrhs; ++r) sc[r] = done[r] ? 0.0 : rnorm[r];
-    ANNEAL_HIP(hipMemcpyAsync(scale_, sc.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s_));
+    for (int r = 0; r < nrhs; ++r) host_up_[r] = done[r] ? 0.0 : rnorm[r];
+    upload(host_up_dev_, scale_, nrhs, nullptr, nullptr, 0, s_);
     axpy_f32_to_f64(px_, U, scale_, n, ld, nrhs, s_);
   }
   if (bad) status = 2;

@@ -200,6 +200,12 @@ class FemContext {
   double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
   double* host_red_ = nullptr;      // pinned
   double* host_red_dev_ = nullptr;  // its device alias: reductions are stored there directly
+  // pinned staging of the per-pass PCG set-up ([0, 64): scale per rhs, then the
+  // 64 x kSt scalar blocks), read by one small kernel (upload) instead of two
+  // copies from pageable memory; rewritten only after a host wait that follows
+  // the upload of its previous contents
+  double* host_up_ = nullptr;
+  double* host_up_dev_ = nullptr;
   bool tuned_ = false;
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs

@@ -343,10 +343,19 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
 
 // ---------------------------------------------------------------------------
 // Jacobi diagonal: diag(n) = kdiag * sum of Ee over the <= 8 elements at n.
-__global__ void k_inv_diag(const float* __restrict__ Ee, const uint8_t* __restrict__ mask, float* __restrict__ dinv,
-                           int nx, int ny, int nz, double kdiag) {
+// One launch serves every multigrid level: workgroups are assigned to levels
+// by the (workgroup-aligned) node offsets in first[].
+__global__ __launch_bounds__(256) void k_inv_diag(DiagLevels lv) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int l = 0;
+  while (l + 1 < lv.nlev && t >= lv.first[l + 1]) ++l;
+  const float* __restrict__ Ee = lv.Ee[l];
+  const uint8_t* __restrict__ mask = lv.mask[l];
+  float* __restrict__ dinv = lv.dinv[l];
+  const int nx = lv.nx[l], ny = lv.ny[l], nz = lv.nz[l];
+  const double kdiag = lv.kdiag[l];
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
-  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n = t - lv.first[l];
   if (n >= nn) return;
   const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
   double s = 0.0;
@@ -545,9 +554,15 @@ template void launch_apply<double>(const ApplyArgs<double>&, int, int, hipStream
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_inv_diag(const float* Ee, const uint8_t* mask, float* dinv, const GridDims& g, double kdiag,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_inv_diag, dim3(nblk(g.nnode(), 256)), dim3(256), 0, s, Ee, mask, dinv, g.nx, g.ny, g.nz, kdiag);
+void launch_inv_diag(DiagLevels d, hipStream_t s) {
+  ANNEAL_CHECK(d.nlev >= 1 && d.nlev <= DiagLevels::kMax, "bad level count for the Jacobi diagonals");
+  // level l's nodes start at a multiple of the workgroup size
+  d.first[0] = 0;
+  for (int l = 0; l < d.nlev; ++l) {
+    const int64_t nn = (int64_t)(d.nx[l] + 1) * (d.ny[l] + 1) * (d.nz[l] + 1);
+    d.first[l + 1] = d.first[l] + (nn + 255) / 256 * 256;
+  }
+  hipLaunchKernelGGL(k_inv_diag, dim3(nblk(d.first[d.nlev], 256)), dim3(256), 0, s, d);
   ANNEAL_HIP(hipGetLastError());
 }
 void launch_coarsen_Ee(const float* Ef, float* Ec, const GridDims& f, const GridDims& c, float q, hipStream_t s) {

@@ -557,9 +557,26 @@ __global__ __launch_bounds__(256) void k_oc_bisect(const double* __restrict__ pa
 // sqrt(max(-dcx, 0) / dvx) per element, so a trial needs one multiply by
 // lam^-1/2 (dvx = 0 with -dcx > 0 gives +inf, which the move limit clips).
 // Block 0 also brings the initial bisection state in from pinned host memory.
+// With stats (the design statistics of design_stats), block 0 also sets the
+// bracket and the first multiplier from them: the reference's bracket scale
+// (optimize.py:146-147) with lo = 1e-8 scale, hi = 1e8 scale, and the first
+// evaluation at scale * ratio when that lies inside (else at sqrt(lo hi)).
 __global__ void k_oc_prep(const double* __restrict__ dcx, const double* __restrict__ dvx, int64_t n,
-                          double* __restrict__ r0, const double* __restrict__ st_host, double* __restrict__ st) {
-  if (blockIdx.x == 0 && threadIdx.x < kOcLen) st[threadIdx.x] = st_host[threadIdx.x];
+                          double* __restrict__ r0, const double* __restrict__ st_host, double* __restrict__ st,
+                          const double* __restrict__ stats, double ratio) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double v[kOcLen];
+    for (int i = 0; i < kOcLen; ++i) v[i] = st_host[i];
+    if (stats) {
+      const double nd = stats[3];
+      const double scale = nd > 0.0 ? fmax((stats[4] / nd) / fmax(stats[5] / nd, 1e-30), 1e-300) : 1.0;
+      const double lo = scale * 1e-8, hi = scale * 1e8, hint = scale * ratio;
+      v[kOcLo] = lo;
+      v[kOcHi] = hi;
+      v[kOcLam] = (ratio > 0.0 && hint > lo && hint < hi) ? hint : sqrt(lo * hi);
+    }
+    for (int i = 0; i < kOcLen; ++i) st[i] = v[i];
+  }
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double c = fmax(-dcx[e], 0.0);
@@ -718,6 +735,24 @@ void FilterContext::physical(const double* x, double beta, const double* phi, do
 void FilterContext::oc_begin(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
                              double move, double lo, double hi, double vtarget, double* xnew, double hint,
                              double width) {
+  oc_start(x, dcx, dvx, phi, beta, move, lo, hi, vtarget, xnew, hint, width, nullptr, 0.0);
+}
+
+void FilterContext::oc_begin_stats(const double* x, const double* dcx, const double* dvx, const double* phi,
+                                   double beta, double move, const double* stats, double ratio, double vtarget,
+                                   double* xnew, double width) {
+  oc_start(x, dcx, dvx, phi, beta, move, 1.0, 1.0, vtarget, xnew, 0.0, width, stats, ratio);
+}
+
+void FilterContext::oc_start(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
+                             double move, double lo, double hi, double vtarget, double* xnew, double hint,
+                             double width, const double* stats, double ratio) {
+  if (run_.active) {
+    // an abandoned bisection (its caller stopped before oc_end): let its
+    // launches finish before the pinned state is rewritten
+    ANNEAL_HIP(hipStreamSynchronize(s_));
+    run_.active = false;
+  }
   // the initial state goes up from the pinned upper half of host_ (k_oc_prep
   // reads it, possibly after this call returns)
   double* st = host_ + kOcLen;
@@ -730,7 +765,7 @@ void FilterContext::oc_begin(const double* x, const double* dcx, const double* d
   st[kOcSlope] = oc_slope_;
   const int64_t ne = g_.nelem();
   hipLaunchKernelGGL(k_oc_prep, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s_, dcx, dvx, ne, r0_,
-                     host_dev_ + kOcLen, oc_);
+                     host_dev_ + kOcLen, oc_, stats, ratio);
   ANNEAL_HIP(hipGetLastError());
   run_ = OcRun{x, phi, xnew, dcx, dvx, beta, move, 0, true, oc_};
   oc_launch(kOcFirstBatch);

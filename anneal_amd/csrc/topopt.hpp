@@ -61,6 +61,13 @@ class FilterContext {
   // these calls in sequence (same launches).
   void oc_begin(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
                 double lo, double hi, double vtarget, double* xnew, double hint = 0.0, double width = 0.0);
+  // oc_begin with the bracket taken on the device from design statistics
+  // (design_stats: stats[3] design count, stats[4] sum -dcx, stats[5] sum dvx),
+  // as optimize.py:146-147 forms it, and the first evaluation at ratio times
+  // its scale (ratio <= 0: none): no host wait for the statistics.
+  void oc_begin_stats(const double* x, const double* dcx, const double* dvx, const double* phi, double beta,
+                      double move, const double* stats, double ratio, double vtarget, double* xnew,
+                      double width = 0.0);
   bool oc_poll();
   int oc_end(int* evals = nullptr, double* lam_last = nullptr);
 
@@ -93,6 +100,9 @@ class FilterContext {
   } run_{};
   double oc_slope_ = 0.0;  // d(vol)/d(log lam) measured by the last bisection (0: none yet)
   hipEvent_t oc_ready_ = nullptr;  // recorded after each state read-back
+  void oc_start(const double* x, const double* dcx, const double* dvx, const double* phi, double beta, double move,
+                double lo, double hi, double vtarget, double* xnew, double hint, double width, const double* stats,
+                double ratio);
   void oc_launch(int count);
   FilterArgs args() const;  // launch arguments common to every mode
 };

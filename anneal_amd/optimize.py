@@ -99,6 +99,13 @@ class DeviceFilter:
                   float(beta), float(move), float(lo), float(hi), float(vtarget), float(hint), float(width),
                   _lib.ptr(xnew))
 
+    def oc_begin_stats(self, x, dcx, dvx, phi, beta, move, stats, vtarget, xnew, ratio=0.0, width=0.0):
+        """:meth:`oc_begin` with the bracket formed on the device from the design statistics ``stats``
+        (optimize.py:146-148); the first evaluation at ``ratio`` times the bracket scale."""
+        _lib.call("anneal_oc_begin_stats", self._ctx, _lib.ptr(x), _lib.ptr(dcx), _lib.ptr(dvx), _lib.ptr(phi),
+                  float(beta), float(move), _lib.ptr(stats), float(ratio), float(vtarget), float(width),
+                  _lib.ptr(xnew))
+
     def oc_poll(self):
         """Wait for the last queued evaluations; True once the bisection has stopped (else queue more)."""
         done = _lib.ctypes.c_int()
@@ -250,6 +257,9 @@ class SimpRun:
         self.base_iters = None
         self.have_u = False
         self.filt.physical(self.x, self.beta, self.phi, self.xphys, self.dxp)
+        # (design buffer, beta) that xphys / dxp currently hold: an iteration whose beta did not change
+        # starts from the physical density the previous OC step already formed
+        self._phys_key = (self.x.data_ptr(), self.beta)
         self.beta_step = max(5, int(0.12 * iters))
         self.beta_since = 0
         self.it = 0
@@ -294,7 +304,9 @@ class SimpRun:
         beta = self.beta
         nel, flags, stream = self.nel, filt.flags, self.stream
         ta = self._tick()
-        filt.physical(self.x, beta, self.phi, self.xphys, self.dxp)
+        if self._phys_key != (self.x.data_ptr(), beta):
+            filt.physical(self.x, beta, self.phi, self.xphys, self.dxp)
+            self._phys_key = (self.x.data_ptr(), beta)
         _lib.call("anneal_stiffness_law", _lib.ptr(self.xphys), _lib.ptr(self.phi), _lib.ptr(flags), self.E0,
                   self.emin, float(p), nel, _lib.ptr(self.Ee), stream)
         fem.assemble_device(self.Ee)
@@ -330,7 +342,6 @@ class SimpRun:
     def _post_solve(self, its):
         """Compliance, sensitivities, OC update and the history entry (optimize.py:125-175)."""
         self._sensitivities(its)
-        self._sync()
         self._oc_begin()
         while not self.filt.oc_poll():
             pass
@@ -381,30 +392,33 @@ class SimpRun:
         self.stats_host.copy_(self.stats, non_blocking=True)
 
     def _oc_begin(self):
-        """Bracket scale and root guess from the statistics (read after a wait), then the first OC trials."""
-        st = self.stats_host.numpy()
-        ndes = self._ndes = st[3]
-        scale = self._scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
+        """The first OC trials, bracketed on the device from the statistics of :meth:`_sensitivities`
+        (no host wait): lo, hi = 1e-8, 1e8 times scale = mean(-dcx) / mean(dvx) over design voxels."""
         # where to look for the root: the previous roots relative to their scales, extrapolated
         # geometrically from the last two (the multiplier drifts smoothly); only the choice of
         # multipliers to evaluate depends on it, never the trial decisions or the result
         ratio = self.oc_ratio
         if self.oc_extrapolate and self.oc_ratio and self.oc_ratio_prev:
             ratio = self.oc_ratio * (self.oc_ratio / self.oc_ratio_prev)
-        hint = scale * ratio if ratio else 0.0
-        self.filt.oc_begin(self.x, self.dcx, self.dvx, self.phi, self.beta, self.move, scale * 1e-8, scale * 1e8,
-                           self.Vtarget, self.xnew, hint=hint)
+        self.filt.oc_begin_stats(self.x, self.dcx, self.dvx, self.phi, self.beta, self.move, self.stats,
+                                 self.Vtarget, self.xnew, ratio=ratio or 0.0)
 
     def _oc_end(self):
         """The OC design (optimize.py:144-163), its physical density and the statistics of the history entry."""
         filt, flags, nel, stream = self.filt, self.filt.flags, self.nel, self.stream
         info = self._oc_info = {}
         self._oc_steps = filt.oc_end(info)
+        # the statistics that scaled the bracket have reached the host before the bisection's
+        # state read-back (same stream); the next read-back below overwrites them
+        st = self.stats_host.numpy()
+        ndes = self._ndes = st[3]
+        self._scale = max(float((st[4] / ndes) / max(st[5] / ndes, 1e-30)), 1e-300) if ndes > 0 else 1.0
         self.oc_ratio_prev, self.oc_ratio = self.oc_ratio, info["lam_last"] / self._scale
         self.oc_evals += info["evals"]
         self._te = self._tick()
         self.T["oc"] += self._te - self._td
         filt.physical(self.xnew, self.beta, self.phi, self.xphys, self.dxp)
+        self._phys_key = (self.xnew.data_ptr(), self.beta)
         _lib.call("anneal_design_stats", _lib.ptr(self.x), _lib.ptr(self.xnew), _lib.ptr(self.xphys),
                   _lib.ptr(self.phi), None, None, _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work),
                   stream)

@@ -173,6 +173,17 @@ int anneal_oc_bisect(void* ctx, const double* x_dev, const double* dcx_dev, cons
 int anneal_oc_begin(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
                     const double* phi_dev, double beta, double move, double lo, double hi, double vtarget, double hint,
                     double hint_width, double* xnew_dev);
+/* anneal_oc_begin with the bracket formed on the device from the design
+ * statistics that anneal_design_stats writes, where stats_dev[3] is the design
+ * count, [4] the sum of -dcx and [5] the sum of dvx over design voxels: scale = max(mean(-dcx) /
+ * max(mean(dvx), 1e-30), 1e-300) (1 without design voxels), lo = 1e-8 scale,
+ * hi = 1e8 scale (optimize.py:146-148), and the first evaluation at
+ * hint_ratio * scale (<= 0: none), so the caller needs no wait for the
+ * statistics before the bisection starts.  Continue with anneal_oc_poll and
+ * anneal_oc_end. */
+int anneal_oc_begin_stats(void* ctx, const double* x_dev, const double* dcx_dev, const double* dvx_dev,
+                          const double* phi_dev, double beta, double move, const double* stats_dev,
+                          double hint_ratio, double vtarget, double hint_width, double* xnew_dev);
 int anneal_oc_poll(void* ctx, int* done_host);
 int anneal_oc_end(void* ctx, int* steps_host, int* evals_host, double* lam_last_host);
 
