@@ -280,6 +280,8 @@ void FemContext::alloc_solver(int nrhs_max) {
   ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
   ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   vgraph_.assign(nrhs_max + 1, nullptr);
+  sgraph_.assign(nrhs_max + 1, nullptr);
+  dgraph_.assign(nrhs_max + 1, nullptr);
 }
 
 FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hipStream_t stream)
@@ -370,14 +372,39 @@ void FemContext::set_rhs_count(int nrhs) {
 
 void FemContext::drop_graphs() {
   ++graph_gen_;  // graphs recorded elsewhere from this context's kernels are stale too
-  for (auto& g : vgraph_)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto* set : {&vgraph_, &sgraph_, &dgraph_})
+    for (auto& g : *set)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
   for (auto& kv : vgraph_bx_)
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
   vgraph_bx_.clear();
+}
+
+template <typename F>
+hipGraphExec_t FemContext::capture(F&& issue) {
+  hipStream_t saved = s_;
+  s_ = cap_s_;
+  hipGraph_t g = nullptr;
+  try {
+    ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
+    issue();
+    ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
+  } catch (...) {
+    s_ = saved;
+    hipGraph_t dummy = nullptr;
+    (void)hipStreamEndCapture(cap_s_, &dummy);
+    if (dummy) (void)hipGraphDestroy(dummy);
+    throw;
+  }
+  s_ = saved;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  ANNEAL_HIP(e);
+  return exec;
 }
 
 void FemContext::vcycle_graph(const float* b, float* x, int nrhs) {
@@ -387,27 +414,7 @@ void FemContext::vcycle_graph(const float* b, float* x, int nrhs) {
   }
   const auto key = std::make_tuple(nrhs, (const void*)b, (void*)x);
   auto it = vgraph_bx_.find(key);
-  if (it == vgraph_bx_.end()) {
-    hipStream_t saved = s_;
-    s_ = cap_s_;
-    hipGraph_t g = nullptr;
-    try {
-      ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
-      vcycle(0, b, x, nrhs);
-      ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
-    } catch (...) {
-      s_ = saved;
-      hipGraph_t dummy = nullptr;
-      (void)hipStreamEndCapture(cap_s_, &dummy);
-      if (dummy) (void)hipGraphDestroy(dummy);
-      throw;
-    }
-    s_ = saved;
-    hipGraphExec_t exec = nullptr;
-    ANNEAL_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    it = vgraph_bx_.emplace(key, exec).first;
-  }
+  if (it == vgraph_bx_.end()) it = vgraph_bx_.emplace(key, capture([&] { vcycle(0, b, x, nrhs); })).first;
   ANNEAL_HIP(hipGraphLaunch(it->second, s_));
 }
 
@@ -485,32 +492,51 @@ void FemContext::learned_pair(const double* F, const double* U, int nrhs, int le
 
 
 void FemContext::vcycle_pcg(int nrhs) {
+  auto issue = [&] { vcycle(0, pr_, pz_, nrhs, part_, st_); };
   if (!use_graphs) {
-    vcycle(0, pr_, pz_, nrhs, part_, st_);
+    issue();
     return;
   }
-  if (!vgraph_[nrhs]) {
-    hipStream_t saved = s_;
-    s_ = cap_s_;
-    hipGraph_t g = nullptr;
-    try {
-      ANNEAL_HIP(hipStreamBeginCapture(cap_s_, hipStreamCaptureModeThreadLocal));
-      vcycle(0, pr_, pz_, nrhs, part_, st_);
-      ANNEAL_HIP(hipStreamEndCapture(cap_s_, &g));
-    } catch (...) {
-      s_ = saved;
-      hipGraph_t dummy = nullptr;
-      (void)hipStreamEndCapture(cap_s_, &dummy);
-      if (dummy) (void)hipGraphDestroy(dummy);
-      throw;
-    }
-    s_ = saved;
-    hipGraphExec_t exec = nullptr;
-    ANNEAL_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    vgraph_[nrhs] = exec;
-  }
+  if (!vgraph_[nrhs]) vgraph_[nrhs] = capture(issue);
   ANNEAL_HIP(hipGraphLaunch(vgraph_[nrhs], s_));
+}
+
+void FemContext::pcg_step(int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  auto issue = [&] {
+    ApplyArgs<float> af = args<float>(0);
+    af.u = pp_;
+    af.y = pq_;
+    af.dot_part = part_;
+    af.skip = st_;
+    launch_apply(af, kApply, nrhs, s_);
+    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
+    pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
+  };
+  if (!use_graphs) {
+    issue();
+    return;
+  }
+  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture(issue);
+  ANNEAL_HIP(hipGraphLaunch(sgraph_[nrhs], s_));
+}
+
+void FemContext::pcg_direction(int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
+  auto issue = [&] {
+    vcycle(0, pr_, pz_, nrhs, part_, st_);
+    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
+    pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+  };
+  if (!use_graphs) {
+    issue();
+    return;
+  }
+  if (!dgraph_[nrhs]) dgraph_[nrhs] = capture(issue);
+  ANNEAL_HIP(hipGraphLaunch(dgraph_[nrhs], s_));
 }
 
 FemContext::~FemContext() {
@@ -857,14 +883,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     // queued before the host waits for rr, so the device never idles on it.
     std::vector<double> rr1(nrhs, 0.0), rr2(nrhs, 0.0);
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
-      ApplyArgs<float> af = args<float>(0);
-      af.u = pp_;
-      af.y = pq_;
-      af.dot_part = part_;
-      af.skip = st_;
-      launch_apply(af, kApply, nrhs, s_);
-      finalize_st(part_, nblk_apply, nrhs, st_, kPQ, 1, s_);  // pq, alpha
-      pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
+      pcg_step(nrhs);
       ANNEAL_HIP(hipEventRecord(ev_, s_));
       bool speculate = it >= 2;
       for (int r = 0; r < nrhs && speculate; ++r) {
@@ -874,6 +893,10 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       }
       const bool flexible = net_ && net_mode_ == 2;
       auto next_direction = [&]() {
+        if (!flexible && lv_.size() > 1) {
+          pcg_direction(nrhs);  // the plain multigrid preconditioner: one graph
+          return;
+        }
         const bool fused = precondition(nrhs);
         if (!fused) dot_partial(pr_, pz_, n, ld, nrhs, part_, s_, st_);
         const int nblk_rz = fused ? nblk_apply : kDotBlocks;

@@ -212,6 +212,15 @@ class FemContext {
   hipEvent_t ev_ = nullptr;                 // marks the residual-norm copy of a PCG step
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
+  // the two halves of a plain PCG iteration, per rhs count: step (K p with
+  // <p, K p>, alpha, x and r update with |r|^2 to the host) and direction
+  // (V-cycle with <r, z>, beta, p = z + beta p)
+  std::vector<hipGraphExec_t> sgraph_, dgraph_;
+  void pcg_step(int nrhs);
+  void pcg_direction(int nrhs);
+  // records what issue() launches on s_ (redirected to the capture stream)
+  template <typename F>
+  hipGraphExec_t capture(F&& issue);
   void drop_graphs();
   uint64_t graph_gen_ = 0;
   // pz_ = M pr_; returns true when part_ already holds the apply-grid partials of <pr_, pz_>
