@@ -501,44 +501,6 @@ void FemContext::vcycle_pcg(int nrhs) {
   ANNEAL_HIP(hipGraphLaunch(vgraph_[nrhs], s_));
 }
 
-void FemContext::pcg_step(int nrhs) {
-  const GridDims& g0 = lv_[0].g;
-  const int64_t n = g0.nnode(), ld = g0.ldn();
-  auto issue = [&] {
-    ApplyArgs<float> af = args<float>(0);
-    af.u = pp_;
-    af.y = pq_;
-    af.dot_part = part_;
-    af.skip = st_;
-    launch_apply(af, kApply, nrhs, s_);
-    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
-    pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
-  };
-  if (!use_graphs) {
-    issue();
-    return;
-  }
-  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture(issue);
-  ANNEAL_HIP(hipGraphLaunch(sgraph_[nrhs], s_));
-}
-
-void FemContext::pcg_direction(int nrhs) {
-  const GridDims& g0 = lv_[0].g;
-  const int64_t n = g0.nnode(), ld = g0.ldn();
-  // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
-  auto issue = [&] {
-    vcycle(0, pr_, pz_, nrhs, part_, st_);
-    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
-    pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
-  };
-  if (!use_graphs) {
-    issue();
-    return;
-  }
-  if (!dgraph_[nrhs]) dgraph_[nrhs] = capture(issue);
-  ANNEAL_HIP(hipGraphLaunch(dgraph_[nrhs], s_));
-}
-
 FemContext::~FemContext() {
   (void)hipStreamSynchronize(s_);
   drop_graphs();
@@ -585,6 +547,44 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
   a.rpo = rpo_;
   (void)l;
   return a;
+}
+
+void FemContext::pcg_step(int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  auto issue = [&] {
+    ApplyArgs<float> af = args<float>(0);
+    af.u = pp_;
+    af.y = pq_;
+    af.dot_part = part_;
+    af.skip = st_;
+    launch_apply(af, kApply, nrhs, s_);
+    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
+    pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
+  };
+  if (!use_graphs) {
+    issue();
+    return;
+  }
+  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture(issue);
+  ANNEAL_HIP(hipGraphLaunch(sgraph_[nrhs], s_));
+}
+
+void FemContext::pcg_direction(int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
+  auto issue = [&] {
+    vcycle(0, pr_, pz_, nrhs, part_, st_);
+    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
+    pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+  };
+  if (!use_graphs) {
+    issue();
+    return;
+  }
+  if (!dgraph_[nrhs]) dgraph_[nrhs] = capture(issue);
+  ANNEAL_HIP(hipGraphLaunch(dgraph_[nrhs], s_));
 }
 
 void FemContext::set_Ee(const double* Ee_dev) {
