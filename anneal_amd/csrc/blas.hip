@@ -419,6 +419,22 @@ __global__ __launch_bounds__(256) void k_upload(const double* __restrict__ src1,
   for (int i = threadIdx.x; i < n1; i += 256) dst1[i] = src1[i];
   for (int i = threadIdx.x; i < n2; i += 256) dst2[i] = src2[i];
 }
+__global__ __launch_bounds__(64) void k_pass_setup(const double* __restrict__ rr, const double* __restrict__ ff,
+                                                   double rtol, double* __restrict__ scale, double* __restrict__ st,
+                                                   int nrhs) {
+  const int r = threadIdx.x;
+  if (r >= nrhs) return;
+  const double rn = sqrt(rr[r]), fn = sqrt(ff[r]);
+  const bool done = rn <= rtol * fn;
+  scale[r] = done ? 0.0 : 1.0 / rn;
+  for (int k = 0; k < kSt; ++k) st[r * kSt + k] = 0.0;
+  st[r * kSt + kDone] = done ? 1.0 : 0.0;
+}
+void pass_setup(const double* rr, const double* ff, double rtol, double* scale, double* st, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(64), 0, s, rr, ff, rtol, scale, st, nrhs);
+  ANNEAL_HIP(hipGetLastError());
+}
+
 void upload(const double* src1, double* dst1, int n1, const double* src2, double* dst2, int n2, hipStream_t s) {
   hipLaunchKernelGGL(k_upload, dim3(1), dim3(256), 0, s, src1, dst1, n1, src2, dst2, n2);
   ANNEAL_HIP(hipGetLastError());

@@ -75,6 +75,11 @@ void pcg_update(float* x, float* r, const float* p, const float* q, const double
                 double* out_host = nullptr);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
 void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
+// PCG pass set-up on the device from |r|^2 (rr) and |F|^2 (ff) per rhs: a rhs
+// with |r| <= rtol |F| is done (scale 0, done flag set), otherwise scale 1/|r|;
+// the scalar blocks start from zero.  The host forms the same decisions from
+// the same norms after its next wait.
+void pass_setup(const double* rr, const double* ff, double rtol, double* scale, double* st, int nrhs, hipStream_t s);
 // dst1[0, n1) = src1[0, n1) and dst2[0, n2) = src2[0, n2) in one small kernel;
 // src1 / src2 are device aliases of pinned host memory (n2 = 0: no second copy)
 void upload(const double* src1, double* dst1, int n1, const double* src2, double* dst2, int n2, hipStream_t s);

@@ -838,35 +838,60 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     a.dot_part = part_;  // |r|^2 partials from the residual kernel itself
     launch_apply(a, kResid, nrhs, s_);
     finalize(part_, apply_blocks(g0.nx, g0.ny, g0.nz, kApplyVariant64), nrhs, red_, s_, host_red_dev_);
-    ANNEAL_HIP(hipStreamSynchronize(s_));
-    if (outer == 0)
-      for (int r = 0; r < nrhs; ++r) fnorm[r] = std::sqrt(host_red_[128 + r]);
+    // The host decisions of a pass, from |r|^2 and |F|^2 in host_red_ (after a wait).
     bool all = true;
-    for (int r = 0; r < nrhs; ++r) {
-      rnorm[r] = std::sqrt(host_red_[r]);
-      if (!std::isfinite(rnorm[r])) bad = true;
-      done[r] = (rnorm[r] <= rtol * fnorm[r]) ? 1 : 0;
-      all = all && done[r];
+    auto decide = [&]() {
+      if (outer == 0)
+        for (int r = 0; r < nrhs; ++r) fnorm[r] = std::sqrt(host_red_[128 + r]);
+      all = true;
+      for (int r = 0; r < nrhs; ++r) {
+        rnorm[r] = std::sqrt(host_red_[r]);
+        if (!std::isfinite(rnorm[r])) bad = true;
+        done[r] = (rnorm[r] <= rtol * fnorm[r]) ? 1 : 0;
+        all = all && done[r];
+        const double want = done[r] ? 1.0 : inner_tol_factor * rtol * fnorm[r] / rnorm[r];
+        const double t = std::max(want, inner_tol_floor);
+        tol2[r] = t * t;
+      }
+    };
+    // The first pass (a fresh start almost never meets the tolerance) is set up
+    // on the device from the same norms and the host decides after its first
+    // PCG step's wait; a pass after that usually ends the solve, so the host
+    // waits for its residual first.
+    const bool optimistic = outer == 0;
+    bool pending = optimistic;
+    if (optimistic) {
+      pass_setup(red_, red_ + 128, rtol, scale_, st_, nrhs, s_);
+      for (int r = 0; r < nrhs; ++r) idone[r] = 0;
+    } else {
+      ANNEAL_HIP(hipStreamSynchronize(s_));
+      decide();
+      if (bad) break;
+      if (all) {
+        status = 0;
+        break;
+      }
+      if (total >= maxiter) break;
+      // inner FP32 PCG on the residual scaled to unit norm
+      for (int r = 0; r < nrhs; ++r) {
+        sc[r] = done[r] ? 0.0 : 1.0 / rnorm[r];
+        for (int k = 0; k < kSt; ++k) hst[r * kSt + k] = 0.0;
+        hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
+        idone[r] = done[r];
+      }
+      std::copy(sc.begin(), sc.end(), host_up_);
+      std::copy(hst.begin(), hst.end(), host_up_ + 64);
+      upload(host_up_dev_, scale_, nrhs, host_up_dev_ + 64, st_, (int)hst.size(), s_);
     }
-    if (bad) break;
-    if (all) {
-      status = 0;
-      break;
-    }
-    if (total >= maxiter) break;
-    // inner FP32 PCG on the residual scaled to unit norm
-    for (int r = 0; r < nrhs; ++r) {
-      sc[r] = done[r] ? 0.0 : 1.0 / rnorm[r];
-      const double want = done[r] ? 1.0 : inner_tol_factor * rtol * fnorm[r] / rnorm[r];
-      const double t = std::max(want, inner_tol_floor);
-      tol2[r] = t * t;
-      for (int k = 0; k < kSt; ++k) hst[r * kSt + k] = 0.0;
-      hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
-      idone[r] = done[r];
-    }
-    std::copy(sc.begin(), sc.end(), host_up_);
-    std::copy(hst.begin(), hst.end(), host_up_ + 64);
-    upload(host_up_dev_, scale_, nrhs, host_up_dev_ + 64, st_, (int)hst.size(), s_);
+    // the pass's decisions once its residual norms have reached the host
+    auto resolve = [&]() {
+      pending = false;
+      decide();
+      for (int r = 0; r < nrhs; ++r) {
+        idone[r] = done[r];
+        hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
+      }
+    };
     to_f32_scaled(r64_, pr_, scale_, n, ld, nrhs, s_);
     ANNEAL_HIP(hipMemsetAsync(px_, 0, (size_t)nrhs * 3 * ld * sizeof(float), s_));
     if (net_ && net_mode_ == 1) learned_warm_start(nrhs);
@@ -892,8 +917,11 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
       pcg_step(nrhs);
       ANNEAL_HIP(hipEventRecord(ev_, s_));
+      // the first step of an optimistic pass is followed by another unless the
+      // start already met the tolerance (the first pass took 2-14 steps on every
+      // solve of the cantilever schedule): queue it before the tolerances are known
       bool speculate = true;
-      for (int r = 0; r < nrhs && speculate; ++r) {
+      for (int r = 0; r < nrhs && speculate && !pending; ++r) {
         if (idone[r]) continue;
         const double rate = rr1[r] / rr2[r];
         if (rr1[r] * rate <= 2.0 * tol2[r]) speculate = false;  // may converge now: do not queue work
@@ -918,6 +946,10 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       };
       if (speculate) next_direction();
       ANNEAL_HIP(hipEventSynchronize(ev_));
+      if (pending) {
+        resolve();
+        if (bad || all) break;  // the device skipped every rhs already within the tolerance
+      }
       ++total;
       bool iall = true, changed = false;
       for (int r = 0; r < nrhs; ++r) {
@@ -941,7 +973,15 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       }
       if (!speculate) next_direction();
     }
+    if (pending) {  // no PCG step ran (maxiter reached): decide from the residual alone
+      ANNEAL_HIP(hipStreamSynchronize(s_));
+      resolve();
+    }
     if (bad) break;
+    if (all) {
+      status = 0;
+      break;
+    }
     for (int r = 0; r < nrhs; ++r) host_up_[r] = done[r] ? 0.0 : rnorm[r];
     upload(host_up_dev_, scale_, nrhs, nullptr, nullptr, 0, s_);
     axpy_f32_to_f64(px_, U, scale_, n, ld, nrhs, s_);
