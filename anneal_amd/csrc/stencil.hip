@@ -117,6 +117,16 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const uint32_t nvo = ncol ? uint32_t(ni + sx * nj) : kOob;   // element index (scaled per array below)
   const uint32_t pbytes = uint32_t(sxy) * uint32_t(sizeof(T));
 
+  // Plane addressing: one buffer resource per array and plane, based at the
+  // plane (a 32-bit byte offset: the launcher keeps one component of a node
+  // array below 2 GiB), with the plane's byte count as its range.  Planes
+  // outside [0, nz] are clamped: their values only meet element layers outside
+  // the grid, whose stiffness is 0, and no output is stored for them.
+  auto plane_off = [&](int p) -> uint32_t { return uint32_t(min(max(p, 0), nz)) * pbytes; };
+  auto at = [](const void* base, uint32_t off) -> const void* { return (const char*)base + off; };
+  // components of a node array lie ldn entries apart (the launcher keeps 3 ldn entries below 4 GiB)
+  const uint32_t cstep = uint32_t(ldn) * uint32_t(sizeof(T));
+
   // Node planes are prefetched two steps ahead (two register slots), so the
   // load latency of a plane is covered by two steps of compute.
   // (the fused first smoothing steps stage mask, D^-1 and b per node, so they
@@ -124,13 +134,12 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   constexpr int kDepth = (MODE == kCheb0) ? 1 : 2;
   T pre[kDepth][NLD][3];
   auto fetch = [&](int p, int slot) {
-    const bool pin = (p >= 0) && (p <= nz);
-    const int64_t pb = sxy * (pin ? p : 0);
-    const uint32_t nb = pin ? pbytes : 0u;
+    const uint32_t o = plane_off(p);
     if (MODE == kCheb0) {
       // first smoothing step from a zero guess: u = M omega D^-1 b
-      const __amdgpu_buffer_rsrc_t rm = rsrc(opmask + pb, pin ? uint32_t(sxy) : 0u);
-      const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv + pb, pin ? uint32_t(sxy) * 4u : 0u);
+      const uint32_t on = o / uint32_t(sizeof(T));  // the plane's first node
+      const __amdgpu_buffer_rsrc_t rm = rsrc(opmask + on, uint32_t(sxy));
+      const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv + on, uint32_t(sxy) * 4u);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
         const uint32_t node = lvo[k] == kOob ? kOob : lvo[k] / uint32_t(sizeof(T));
@@ -138,13 +147,13 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
         const T d = a.omega * T(bload<float>(rd, node == kOob ? kOob : node * 4u));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T bv = bload<T>(rsrc(bb + c * ldn + pb, nb), lvo[k]);
+          const T bv = bload<T>(rsrc(at(bb, o + c * cstep), pbytes), lvo[k]);
           pre[slot][k][c] = ((m >> c) & 1) ? d * bv : T(0);
         }
       }
     } else {
-      const __amdgpu_buffer_rsrc_t r0 = rsrc(u + pb, nb), r1 = rsrc(u + ldn + pb, nb),
-                                   r2 = rsrc(u + 2 * ldn + pb, nb);
+      const __amdgpu_buffer_rsrc_t r0 = rsrc(at(u, o), pbytes), r1 = rsrc(at(u, o + cstep), pbytes),
+                                   r2 = rsrc(at(u, o + 2 * cstep), pbytes);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
         pre[slot][k][0] = bload<T>(r0, lvo[k]);
@@ -153,25 +162,26 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
       }
     }
   };
+  const uint32_t ebytes = uint32_t(nx) * uint32_t(ny) * uint32_t(sizeof(T));  // one element layer
   auto fetch_e = [&](int L) -> T {
-    const bool lin = L >= 0 && L < nz;
-    const __amdgpu_buffer_rsrc_t re = rsrc(opEe + exy * (lin ? L : 0), lin ? uint32_t(exy) * uint32_t(sizeof(T)) : 0u);
+    const bool lin = L >= 0 && L < nz;  // layers outside the grid read as 0
+    const __amdgpu_buffer_rsrc_t re = rsrc(at(opEe, uint32_t(min(max(L, 0), nz - 1)) * ebytes), lin ? ebytes : 0u);
     return bload<T>(re, evo);
   };
 
   // Operands of the output node in plane L (mask, b, D^-1, previous iterate),
-  // also fetched one step before they are used.
+  // also fetched one step before they are used (planes outside the chunk are
+  // clamped and their operands unused).
   struct Epi {
     uint8_t mk;
     float dv;
     T bv[3], pv[3];
   };
   const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
+  const uint32_t dvo = nvo == kOob ? kOob : nvo * 4u;
   auto fetch_epi = [&](int L, Epi& E) {
-    const bool ok = L >= Z0 && L <= nz && L < Z0 + ZC;
-    const int64_t lb = sxy * (ok ? L : 0);
-    const uint32_t nb_out = ok ? pbytes : 0u;
-    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(opmask + lb, ok ? uint32_t(sxy) : 0u), nvo, 0, 0);
+    const uint32_t o = plane_off(L), on = o / uint32_t(sizeof(T));
+    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(opmask + on, uint32_t(sxy)), nvo, 0, 0);
     E.dv = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -180,13 +190,13 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     }
     if (MODE != kApply) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rsrc(bb + c * ldn + lb, nb_out), ovo);
+      for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rsrc(at(bb, o + c * cstep), pbytes), ovo);
     }
     if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
-      E.dv = bload<float>(rsrc(opdinv + lb, ok ? uint32_t(sxy) * 4u : 0u), nvo == kOob ? kOob : nvo * 4u);
+      E.dv = bload<float>(rsrc(opdinv + on, uint32_t(sxy) * 4u), dvo);
     if (MODE == kCheb && pr) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(pr + c * ldn + lb, nb_out), ovo);
+      for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(at(pr, o + c * cstep), pbytes), ovo);
     } else if (MODE == kCheb && a.prev_j0) {
       // previous iterate x1 = omega0 D^-1 b of a smoothing that started from zero
 #pragma unroll
@@ -245,8 +255,8 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
         fetch_epi(L, E);
       }
     }
-    const int64_t lb = sxy * (outp ? L : 0);  // uniform base of the output plane
-    const uint32_t nb_out = outp ? pbytes : 0u;
+    const uint32_t lb = plane_off(L);           // byte offset of the output plane
+    const uint32_t nb_out = outp ? pbytes : 0u;  // stores dropped without an output plane
 
     __syncthreads();
     T Fcur[3][4];
@@ -305,7 +315,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
             else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(E.dv) * (E.bv[c] - v);
             else out = uc + a.c1 * (uc - E.pv[c]) + a.c2 * T(E.dv) * (E.bv[c] - v);
             if (!((E.mk >> c) & 1)) out = T(0);
-            bstore<T>(out, rsrc(y + c * ldn + lb, nb_out), ovo);  // dropped for lanes without a node
+            bstore<T>(out, rsrc(at(y, lb + c * cstep), nb_out), ovo);  // dropped for lanes without a node
             if (MODE == kApply) {
               if (a.dot_part && ncol) dot += (double)uc * (double)out;
             } else if (MODE == kCheb || MODE == kJacobi) {
@@ -515,6 +525,8 @@ int apply_blocks(int nx, int ny, int nz, int variant) {
 template <typename T, int MODE, int TX, int TY, int ZC>
 static void launch_tile(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
   ApplyArgs<T> a = a0;
+  ANNEAL_CHECK(3 * a.ldn * (int64_t)sizeof(T) < (int64_t(1) << 32),
+               "grid too large for the 32-bit node-array offsets of the apply kernel");
   a.ntx = (a.nx + 1 + TX - 2) / (TX - 1);
   a.nty = (a.ny + 1 + TY - 2) / (TY - 1);
   a.nzc = (a.nz + 1 + ZC - 1) / ZC;
