@@ -180,7 +180,12 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
   const uint32_t dvo = nvo == kOob ? kOob : nvo * 4u;
   auto fetch_epi = [&](int L, Epi& E) {
-    const uint32_t o = plane_off(L), on = o / uint32_t(sizeof(T));
+    uint32_t o = plane_off(L);
+    // the fused first steps staged plane L two steps earlier from the same
+    // arrays: an opaque offset keeps the compiler from holding those plane
+    // resources in scalar registers until now (which spilled them)
+    if constexpr (MODE == kCheb0) asm volatile("" : "+s"(o));
+    const uint32_t on = o / uint32_t(sizeof(T));
     E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(opmask + on, uint32_t(sxy)), nvo, 0, 0);
     E.dv = 0.f;
 #pragma unroll
