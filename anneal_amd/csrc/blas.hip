@@ -17,11 +17,25 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(const TA* __restric
   const TA* pa = a + (int64_t)3 * r * ld;
   const TB* pb = b + (int64_t)3 * r * ld;
   double s = 0.0;
+  // four grid-stride entries per pass with their loads issued first (clamped
+  // addresses); the products are summed in the entry order of a plain loop
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * kRedThreads;
   for (int c = 0; c < 3; ++c) {
     const TA* qa = pa + c * ld;
     const TB* qb = pb + c * ld;
-    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedThreads)
-      s += (double)qa[i] * (double)qb[i];
+    for (int64_t i0 = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i0 < n; i0 += U * stride) {
+      double va[U], vb[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * stride < n ? i0 + u * stride : n - 1;
+        va[u] = (double)qa[i];
+        vb[u] = (double)qb[i];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * stride < n) s += va[u] * vb[u];
+    }
   }
   __shared__ double sh[kRedThreads / 64];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);

@@ -372,15 +372,22 @@ __global__ __launch_bounds__(256) void k_inv_diag(DiagLevels lv) {
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   const int64_t n = t - lv.first[l];
   if (n >= nn) return;
-  const int i = n % (nx + 1), j = (n / (nx + 1)) % (ny + 1), k = n / ((int64_t)(nx + 1) * (ny + 1));
+  // 32-bit index arithmetic (a level has fewer than 2^31 nodes)
+  const uint32_t n32 = uint32_t(n), sx = uint32_t(nx + 1), q = n32 / sx;
+  const int i = int(n32 - q * sx), j = int(q % uint32_t(ny + 1)), k = int(q / uint32_t(ny + 1));
+  // the 8 elements at the node, loads first (clamped in-grid addresses), summed in (dk, dj, di) order
+  float ev[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1), ek = k - 1 + (c >> 2);
+    const bool in = ei >= 0 && ei < nx && ej >= 0 && ej < ny && ek >= 0 && ek < nz;
+    const uint32_t e = uint32_t(min(max(ei, 0), nx - 1)) +
+                       uint32_t(nx) * (uint32_t(min(max(ej, 0), ny - 1)) + uint32_t(ny) * uint32_t(min(max(ek, 0), nz - 1)));
+    ev[c] = in ? Ee[e] : 0.f;
+  }
   double s = 0.0;
-  for (int dk = -1; dk <= 0; ++dk)
-    for (int dj = -1; dj <= 0; ++dj)
-      for (int di = -1; di <= 0; ++di) {
-        const int ei = i + di, ej = j + dj, ek = k + dk;
-        if (ei >= 0 && ei < nx && ej >= 0 && ej < ny && ek >= 0 && ek < nz)
-          s += Ee[ei + (int64_t)nx * (ej + (int64_t)ny * ek)];
-      }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s += ev[c];
   const double d = s * kdiag;
   dinv[n] = (mask[n] != 0 && d > 0.0) ? float(1.0 / d) : 0.f;
 }
@@ -395,19 +402,27 @@ __global__ void k_coarsen_Ee(const float* __restrict__ Ef, float* __restrict__ E
   const int64_t nc = (int64_t)cx * cy * cz;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= nc) return;
-  const int I = e % cx, J = (e / cx) % cy, K = e / ((int64_t)cx * cy);
+  const uint32_t e32 = uint32_t(e), qx = e32 / uint32_t(cx);
+  const int I = int(e32 - qx * uint32_t(cx)), J = int(qx % uint32_t(cy)), K = int(qx / uint32_t(cy));
+  // the 8 children, loads first (clamped in-grid addresses), summed in (dk, dj, di) order
+  float v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = 2 * I + (c & 1), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c >> 2);
+    const bool in = i < nx && j < ny && k < nz;
+    v[c] = Ef[uint32_t(min(i, nx - 1)) + uint32_t(nx) * (uint32_t(min(j, ny - 1)) + uint32_t(ny) * uint32_t(min(k, nz - 1)))];
+    if (!in) v[c] = -1.f;  // missing child
+  }
   double s = 0.0;
-  for (int dk = 0; dk < 2; ++dk)
-    for (int dj = 0; dj < 2; ++dj)
-      for (int di = 0; di < 2; ++di) {
-        const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
-        if (i < nx && j < ny && k < nz) {
-          const double v = Ef[i + (int64_t)nx * (j + (int64_t)ny * k)];
-          s += q == 1.0f ? v : pow(fmax(v, 0.0), (double)q);
-        }
-      }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (v[c] < 0.f) continue;
+    const double d = v[c];
+    // q = 1/2 (the default) as sqrt and a square: the same power mean without pow
+    s += q == 1.0f ? d : (q == 0.5f ? sqrt(d) : pow(d, (double)q));
+  }
   const double m = s * 0.125;
-  Ec[e] = float(q == 1.0f ? m : pow(m, 1.0 / (double)q));
+  Ec[e] = float(q == 1.0f ? m : (q == 0.5f ? m * m : pow(m, 1.0 / (double)q)));
 }
 
 // 1-D stencil of the trilinear prolongation seen from coarse index I: fine

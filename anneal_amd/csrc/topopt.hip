@@ -929,19 +929,36 @@ __global__ __launch_bounds__(256) void k_design_stats(const double* __restrict__
                                                       const uint8_t* __restrict__ flags, int64_t n,
                                                       double* __restrict__ part) {
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-    const uint8_t f = flags[e];
-    if (!(f & kActive)) continue;
-    if (xphys) s[0] += phi[e] * xphys[e];
-    if (f & kFrozen) continue;
-    s[3] += 1.0;
-    if (xphys) {
-      const double v = xphys[e];
-      if (v > 0.1 && v < 0.9) s[1] += 1.0;
+  // four grid-stride elements per pass, all loads issued first (clamped
+  // addresses); the accumulation order is the element order of a plain loop
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t e0 = blockIdx.x * 256 + threadIdx.x; e0 < n; e0 += U * stride) {
+    uint8_t f[U];
+    double vp[U], vx[U], vo[U], vn[U], vc[U], vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = e0 + u * stride;
+      const int64_t c = e < n ? e : n - 1;
+      f[u] = e < n ? flags[c] : uint8_t(0);
+      vp[u] = xphys ? phi[c] : 0.0;
+      vx[u] = xphys ? xphys[c] : 0.0;
+      vo[u] = (xold && xnew) ? xold[c] : 0.0;
+      vn[u] = (xold && xnew) ? xnew[c] : 0.0;
+      vc[u] = dcx ? dcx[c] : 0.0;
+      vv[u] = dvx ? dvx[c] : 0.0;
     }
-    if (xold && xnew) s[2] = fmax(s[2], fabs(xnew[e] - xold[e]));
-    if (dcx) s[4] += -dcx[e];
-    if (dvx) s[5] += dvx[e];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!(f[u] & kActive)) continue;
+      if (xphys) s[0] += vp[u] * vx[u];
+      if (f[u] & kFrozen) continue;
+      s[3] += 1.0;
+      if (xphys && vx[u] > 0.1 && vx[u] < 0.9) s[1] += 1.0;
+      if (xold && xnew) s[2] = fmax(s[2], fabs(vn[u] - vo[u]));
+      if (dcx) s[4] += -vc[u];
+      if (dvx) s[5] += vv[u];
+    }
   }
   __shared__ double sh[6][4];
   for (int q = 0; q < 6; ++q) {
