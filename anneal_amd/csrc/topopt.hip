@@ -106,10 +106,11 @@ constexpr double kOcTau = 1e-12;       // log-step past the root estimate that c
 constexpr int kOcFirstBatch = 7, kOcNextBatch = 4;
 constexpr double kOcGrowStep = 0.005;  // default log step from a one-sided first evaluation
 
-__device__ void oc_record(double* oc, double lam, double f) {
+__device__ void oc_record(double* oc, double lam, double t, double f) {
   if (f > 0.0) {
     if (oc[kOcHasA] == 0.0 || lam > oc[kOcA]) {
       oc[kOcA] = lam;
+      oc[kOcTA] = t;
       oc[kOcFA] = f;
       oc[kOcHasA] = 1.0;
       if (oc[kOcSide] > 0.0 && oc[kOcHasB] != 0.0) oc[kOcFB] *= 0.5;  // Illinois
@@ -118,6 +119,7 @@ __device__ void oc_record(double* oc, double lam, double f) {
   } else {
     if (oc[kOcHasB] == 0.0 || lam < oc[kOcB]) {
       oc[kOcB] = lam;
+      oc[kOcTB] = t;
       oc[kOcFB] = f;
       oc[kOcHasB] = 1.0;
       if (oc[kOcSide] < 0.0 && oc[kOcHasA] != 0.0) oc[kOcFA] *= 0.5;
@@ -147,13 +149,14 @@ __device__ double oc_volume(const double* __restrict__ part, int nblk, double* s
 
 // The multiplier to evaluate next when the reference's trial lam lies inside
 // the bracket (or on the unknown side).  Only which multipliers are evaluated
-// depends on this choice, never a decision or the result.
+// depends on this choice, never a decision or the result.  One thread of
+// every workgroup runs it before the trial starts, so it takes no logarithm
+// (those of the evaluated multipliers are kept in the state) and one exp.
 __device__ double oc_choose(const double* oc, double lo, double hi, double lam) {
   const bool hasA = oc[kOcHasA] != 0.0, hasB = oc[kOcHasB] != 0.0;
   const double n = oc[kOcEvals];
   const double t1 = oc[kOcT1], f1 = oc[kOcF1], t2 = oc[kOcT2], f2 = oc[kOcF2];
-  const double ta = hasA ? log(oc[kOcA]) : 0.0, tb = hasB ? log(oc[kOcB]) : 0.0;
-  const double tlo = log(lo), thi = log(hi);
+  const double ta = oc[kOcTA], tb = oc[kOcTB];
   // still one-sided after several evaluations (a target the bracket may not
   // contain): the far end of the trial interval decides every remaining trial
   // if the root is not inside it
@@ -178,14 +181,21 @@ __device__ double oc_choose(const double* oc, double lo, double hi, double lam) 
     if (hasA) t = fmax(t, ta);
     if (hasB) t = fmin(t, tb);
   }
-  auto inside = [&](double u) {
-    return isfinite(u) && u > tlo && u < thi && (!hasA || u > ta) && (!hasB || u < tb);
+  // a candidate log multiplier, as a multiplier strictly inside the trial
+  // interval and the bracket (0 if it is not)
+  auto inside = [&](double u) -> double {
+    if (!isfinite(u) || (hasA && !(u > ta)) || (hasB && !(u < tb))) return 0.0;
+    const double c = exp(u);
+    return (c > lo && c < hi && (!hasA || c > oc[kOcA]) && (!hasB || c < oc[kOcB])) ? c : 0.0;
   };
-  if (have && inside(t)) return exp(t);
+  if (have) {
+    const double c = inside(t);
+    if (c > 0.0) return c;
+  }
   if (n == 1.0 && hasA != hasB) {
     // one side known and no usable slope: a fixed log step toward the root
-    const double u = t1 + (hasA ? oc[kOcGrow] : -oc[kOcGrow]);
-    if (inside(u)) return exp(u);
+    const double c = inside(t1 + (hasA ? oc[kOcGrow] : -oc[kOcGrow]));
+    if (c > 0.0) return c;
   }
   // one-sided and the secant leaves the interval: test its far end
   if (n >= 2.0 && hasA != hasB) return hasA ? hi : lo;
@@ -201,7 +211,7 @@ __device__ void oc_step(double* oc, double vol) {
   oc[kOcF2] = oc[kOcF1];
   oc[kOcT1] = log(lamE);
   oc[kOcF1] = f;
-  oc_record(oc, lamE, f);
+  oc_record(oc, lamE, oc[kOcT1], f);
   double lo = oc[kOcLo], hi = oc[kOcHi];
   for (int guard = 0; guard < 200; ++guard) {
     const double lam = sqrt(lo * hi);  // the reference's next trial

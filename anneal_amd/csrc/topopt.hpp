@@ -13,14 +13,15 @@ constexpr uint8_t kActive = 1, kFrozen = 2;
 // OC bisection state on the device (FP64).  kOcLam is the multiplier the next
 // trial-volume kernel evaluates; (lo, hi) is the reference's bisection state;
 // [A, B] is a certified bracket of the root of vol(lam) = vt (FA, FB: vol - vt
-// there); (T1, F1), (T2, F2) are log(lam) and vol - vt of the last two
+// there, TA, TB: their logs); (T1, F1), (T2, F2) are log(lam) and vol - vt of the last two
 // evaluations; Slope is d(vol)/d(log lam) carried over from the previous
 // bisection (0: none), SlopeNext the one this bisection hands on; Grow is the
 // log step taken from a one-sided first evaluation when no slope is known.
 constexpr int kOcLo = 0, kOcHi = 1, kOcLam = 2, kOcLamLast = 3, kOcDone = 4, kOcSteps = 5, kOcVt = 6, kOcVol = 7,
               kOcA = 8, kOcFA = 9, kOcB = 10, kOcFB = 11, kOcHasA = 12, kOcHasB = 13, kOcSide = 14, kOcEvals = 15,
-              kOcGrow = 16, kOcT1 = 17, kOcF1 = 18, kOcT2 = 19, kOcF2 = 20, kOcSlope = 21, kOcSlopeNext = 22;
-constexpr int kOcLen = 24;
+              kOcGrow = 16, kOcT1 = 17, kOcF1 = 18, kOcT2 = 19, kOcF2 = 20, kOcSlope = 21, kOcSlopeNext = 22,
+              kOcTA = 23, kOcTB = 24;  // log(A), log(B)
+constexpr int kOcLen = 26;
 
 struct FilterArgs;
 
