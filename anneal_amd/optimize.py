@@ -269,6 +269,8 @@ class SimpRun:
         self.oc_extrapolate = True
         self.oc_evals = 0
         self.keep_start = False   # True: keep each solve's starting point in U_start (training pairs)
+        self._queued = None       # ((it, p, beta, beta_since), t0) of an operator queued ahead
+        self._ahead = False
         self.U_start = None
 
     def _tick(self):
@@ -276,8 +278,13 @@ class SimpRun:
             self.torch.cuda.synchronize()
         return time.perf_counter()
 
-    def step(self):
-        """One SIMP iteration (optimize.py:105-175); returns its history entry."""
+    def step(self, ahead=False):
+        """One SIMP iteration (optimize.py:105-175); returns its history entry.
+
+        ``ahead``: queue the next iteration's operator (stiffness law and multigrid set-up) before this
+        iteration's last host wait when the next iteration is certain to follow; the FEM operator then
+        belongs to that next iteration when step returns (``run`` steps this way)."""
+        self._ahead = ahead
         self._pre_solve()
         its = []
         nb = self.fem.nrhs_max
@@ -287,29 +294,42 @@ class SimpRun:
                                          use_x0=self.have_u)
         return self._post_solve(its)
 
+    def _advance(self):
+        """The next iteration's number, penalty and projection sharpness (optimize.py:105-116), not applied."""
+        it = self.it + 1
+        p = _schedule_p(it, self.iters, self.p_final, self.warm)
+        beta, since = self.beta, self.beta_since
+        if p >= self.p_final and not self.warm and self.beta_max > 0:        # optimize.py:111-116
+            if beta == 0:
+                beta, since = 1.0, 0
+            elif since >= self.beta_step and beta < self.beta_max:
+                beta, since = min(self.beta_max, 2 * beta), 0
+            since += 1
+        return it, p, beta, since
+
+    def _operator(self, x, p, beta):
+        """Physical density (unless xphys already holds it), stiffness law and operator (optimize.py:117-118)."""
+        filt, fem = self.filt, self.fem
+        if self._phys_key != (x.data_ptr(), beta):
+            filt.physical(x, beta, self.phi, self.xphys, self.dxp)
+            self._phys_key = (x.data_ptr(), beta)
+        _lib.call("anneal_stiffness_law", _lib.ptr(self.xphys), _lib.ptr(self.phi), _lib.ptr(filt.flags), self.E0,
+                  self.emin, float(p), self.nel, _lib.ptr(self.Ee), self.stream)
+        fem.assemble_device(self.Ee)
+
     def _pre_solve(self):
         """Continuation, physical density, stiffness law and the rebuild decision (optimize.py:105-124)."""
-        torch, fem, filt, T = self.torch, self.fem, self.filt, self.T
-        self.it += 1
-        it = self.it
-        self._t0 = time.time()
-        p = _schedule_p(it, self.iters, self.p_final, self.warm)
-        if p >= self.p_final and not self.warm and self.beta_max > 0:        # optimize.py:111-116
-            if self.beta == 0:
-                self.beta, self.beta_since = 1.0, 0
-            elif self.beta_since >= self.beta_step and self.beta < self.beta_max:
-                self.beta, self.beta_since = min(self.beta_max, 2 * self.beta), 0
-            self.beta_since += 1
-        self._p = p
-        beta = self.beta
-        nel, flags, stream = self.nel, filt.flags, self.stream
+        torch, fem, T = self.torch, self.fem, self.T
+        queued, self._queued = self._queued, None
+        if queued is not None:      # the operator was queued before the previous iteration's last wait
+            (it, p, beta, since), self._t0 = queued
+        else:
+            it, p, beta, since = self._advance()
+            self._t0 = time.time()
+        self.it, self._p, self.beta, self.beta_since = it, p, beta, since
         ta = self._tick()
-        if self._phys_key != (self.x.data_ptr(), beta):
-            filt.physical(self.x, beta, self.phi, self.xphys, self.dxp)
-            self._phys_key = (self.x.data_ptr(), beta)
-        _lib.call("anneal_stiffness_law", _lib.ptr(self.xphys), _lib.ptr(self.phi), _lib.ptr(flags), self.E0,
-                  self.emin, float(p), nel, _lib.ptr(self.Ee), stream)
-        fem.assemble_device(self.Ee)
+        if queued is None:
+            self._operator(self.x, p, beta)
         tb = self._tick()
         T["physical"] += tb - ta
         last = fem.cg_iters[-1] if fem.cg_iters else None
@@ -346,8 +366,32 @@ class SimpRun:
         while not self.filt.oc_poll():
             pass
         self._oc_end()
-        self._sync()
+        nxt = self._next_operator_ahead()
+        if nxt is None:
+            self._sync()
+        else:
+            # the next iteration's operator (its design and sharpness are known now) runs on the device
+            # while the host waits for the statistics and writes the history entry
+            done = self.torch.cuda.Event()
+            done.record(self._torch_stream)
+            t0 = time.time()
+            self._operator(self.xnew, nxt[1], nxt[2])
+            self._queued = (nxt, t0)
+            done.synchronize()
         return self._record()
+
+    def _next_operator_ahead(self):
+        """The next iteration's schedule when its operator can be queued before this iteration's last
+        wait: a next iteration follows for certain (the schedule is not done and the convergence test
+        of optimize.py:173-175 cannot stop the loop here) and its projection sharpness is unchanged,
+        so the physical density is already in place (nothing that result() returns is touched)."""
+        if not self._ahead or self.timing or self.it >= self.iters:
+            return None
+        if (self.tol is not None and self.tol > 0 and self._p >= self.p_final and self.beta >= self.beta_max
+                and self.it > (5 if self.warm else 10)):
+            return None
+        nxt = self._advance()
+        return nxt if nxt[2] == self.beta else None
 
     # _post_solve in phases that only queue device work, separated by host waits (``_sync``,
     # ``filt.oc_poll``), so that a batch of runs can share each wait
@@ -465,7 +509,7 @@ class SimpRun:
 
     def run(self):
         while self.it < self.iters and not self.converged:
-            self.step()
+            self.step(ahead=True)
         return self.result()
 
 

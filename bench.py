@@ -712,7 +712,7 @@ def main():
     setup_s = time.perf_counter() - t_setup
     t0 = time.perf_counter()
     for _ in range(steps):
-        run.step()
+        run.step(ahead=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = 1e3 * dt / steps
@@ -720,7 +720,7 @@ def main():
     # the rest of the schedule, so that the whole continuation (p ramp, beta doubling) is measured too
     t1 = time.perf_counter()
     while run.it < iters:
-        run.step()
+        run.step(ahead=True)
     torch.cuda.synchronize()
     full_s = dt + time.perf_counter() - t1
     cg = run.fem.cg_iters

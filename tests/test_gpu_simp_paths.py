@@ -106,3 +106,32 @@ def test_more_load_cases_than_one_solve_block():
     assert comp.shape == (4, 70)
     assert (np.abs(comp - ref) / np.abs(ref)).max() <= 1e-6
     assert np.abs(got["x"] - want["x"]).max() <= 1e-4
+
+
+def test_operator_queued_ahead_changes_nothing():
+    """run() queues each next iteration's operator before the previous iteration's last wait; the
+    trajectory (every history value, the design) is bit-identical to stepping without it, through the
+    continuation's beta changes, and stepping ahead leaves the operator of the next iteration."""
+    from oracle.optimize_ref import cantilever
+    from anneal_amd.fem import Grid
+    from anneal_amd.optimize import SimpRun
+    shape = (24, 12, 12)
+    g, fill, active, frozen, fixed, F = cantilever(shape)
+    kw = dict(iters=30, r_mm=1.5, tol=0.0, emin=1e-9, rtol=1e-8, log=lambda *a: None)
+
+    def new():
+        return SimpRun(Grid(shape, 1.0, (0, 0, 0)), fill, active, frozen, fixed, {"c": F}, {"c": 1.0}, 1.0, 0.3,
+                       0.3, **kw)
+    print("cantilever 24x12x12, 30 iterations with and without the operator queued ahead", flush=True)
+    a, b = new(), new()
+    ra = a.run()
+    for _ in range(30):
+        b.step()
+    rb = b.result()
+    assert np.array_equal(ra["x"], rb["x"])
+    keys = ("objective", "compliance", "p", "beta", "volumeFraction", "change", "grey", "cgIters", "ocSteps")
+    for ha, hb in zip(ra["history"], rb["history"]):
+        assert all(ha[k] == hb[k] for k in keys)
+    assert {h["beta"] for h in ra["history"]} >= {0.0, 1.0, 2.0}
+    a.fem.close()
+    b.fem.close()
