@@ -41,6 +41,7 @@ SIGNATURES = {
     "anneal_filter_destroy": [c_void_p],
     "anneal_filter_forward": [c_void_p, c_void_p, c_void_p],
     "anneal_filter_backward": [c_void_p, c_void_p, c_void_p],
+    "anneal_filter_backward_pair": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "anneal_filter_physical": [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "anneal_oc_bisect": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double,
                          c_double, c_double, c_double, c_void_p, c_intp, c_intp, ctypes.POINTER(c_double)],

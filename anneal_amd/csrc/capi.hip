@@ -391,6 +391,18 @@ int anneal_filter_backward(void* ctx, const double* d_dev, double* y_dev) {
   });
 }
 
+int anneal_filter_backward_pair(void* ctx, const double* d1_dev, double* y1_dev, const double* d2_dev,
+                                double* y2_dev, double* stats_dev) {
+  return guarded([&] {
+    ANNEAL_CHECK(d1_dev && y1_dev && d2_dev && y2_dev, "null argument");
+    ANNEAL_CHECK(d1_dev != (const double*)y1_dev && d2_dev != (const double*)y2_dev && y1_dev != y2_dev &&
+                     d1_dev != (const double*)y2_dev && d2_dev != (const double*)y1_dev,
+                 "filter needs distinct inputs and outputs");
+    filt_of(ctx)->backward_pair(d1_dev, y1_dev, d2_dev, y2_dev, stats_dev);
+    return 0;
+  });
+}
+
 int anneal_filter_physical(void* ctx, const double* x_dev, double beta, const double* phi_dev, double* xphys_dev,
                            double* dxp_dev, double* vol_dev) {
   return guarded([&] {

@@ -15,7 +15,9 @@
 
 namespace anneal {
 
-enum FilterMode : int { kHs = 0, kFwd = 1, kBwd = 2, kPhys = 3, kOc = 4 };
+// kBwd2: two backward passes in one launch (in -> out, in2 -> out2) with the
+// design statistics of the pair (count, sum of -out, sum of out2) as partials.
+enum FilterMode : int { kHs = 0, kFwd = 1, kBwd = 2, kPhys = 3, kOc = 4, kBwd2 = 5 };
 
 // Radii R <= kRingMaxR run the z-marching ring-sum kernel (k_filter<R, MODE>);
 // larger ones stage the filtered quantity in a scratch grid and gather each
@@ -25,6 +27,7 @@ constexpr int kWc = 3 * kRingMaxR * kRingMaxR + 1;  // squared distances 0 .. 3R
 
 struct FilterArgs {
   const double* in;
+  const double* in2;  // kBwd2: the second input
   const double* dcx;
   const double* dvx;
   const uint8_t* flags;
@@ -279,7 +282,8 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   constexpr int K = 2 * R + 1, LX = FTX + 2 * R, LY = FTY + 2 * R, NP = LX * LY, NT = FTX * FTY;
   constexpr int NLD = (NP + NT - 1) / NT;  // staging loads per thread per layer
   constexpr int NS = FZC + 2 * R;          // input layers per chunk
-  __shared__ double buf[2][NP];
+  constexpr int C = MODE == kBwd2 ? 2 : 1; // channels filtered together
+  __shared__ double buf[2][C][NP];
   __shared__ double sred[NT / 64];
   const int tid = threadIdx.x;
   double rlam = 0.0, mv = 0.0;
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   // raw operands of one staged value (flags, input, second input)
   struct Raw {
     uint8_t f;
-    double x, y;
+    double x, y, x2;
   };
   Raw pre[2][NLD];
   auto fetch = [&](int q, int slot) {
@@ -316,20 +320,22 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
       v.f = 0;
       v.x = 0.0;
       v.y = 0.0;
+      v.x2 = 0.0;
       if (qin && soff[k] >= 0) {
         const int64_t e = soff[k] + exy * q;
         v.f = a.flags[e];
         if (MODE != kHs) v.x = a.in[e];
-        if (MODE == kBwd) v.y = a.invHs[e];
+        if (MODE == kBwd || MODE == kBwd2) v.y = a.invHs[e];
+        if (MODE == kBwd2) v.x2 = a.in2[e];
         if (MODE == kOc) v.y = a.dcx[e];
       }
     }
   };
-  auto staged = [&](const Raw& v) -> double {
+  auto staged = [&](const Raw& v, int ch) -> double {
     if (!(v.f & kActive)) return 0.0;
     if (MODE == kHs) return 1.0;
     if (MODE == kFwd || MODE == kPhys) return v.x;
-    if (MODE == kBwd) return v.x * v.y;
+    if (MODE == kBwd || MODE == kBwd2) return (ch == 0 ? v.x : v.x2) * v.y;
     // kOc: trial design of optimize.py:150-153, with sqrt(c/dv) precomputed
     if (v.f & kFrozen) return 1.0;
     return fmin(fmax(v.x * (v.y * rlam), fmax(0.0, v.x - mv)), fmin(1.0, v.x + mv));
@@ -356,11 +362,14 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
     }
   };
 
-  // win[i] accumulates output layer s - 2R + i while input layer s is processed
-  double win[K];
+  // win[c][i] accumulates output layer s - 2R + i of channel c while input layer s is processed
+  double win[C][K];
 #pragma unroll
-  for (int i = 0; i < K; ++i) win[i] = 0.0;
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int i = 0; i < K; ++i) win[c][i] = 0.0;
   double vol = 0.0;
+  double dst[3] = {0.0, 0.0, 0.0};  // kBwd2: design count, sum of -out, sum of out2
   fetch(Z0 - R, 0);
   fetch(Z0 - R + 1, 1);
   Epi ep[2] = {};
@@ -376,11 +385,13 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
   // one input layer; SL = s & 1 selects the LDS buffer and the prefetch slots
   auto step = [&](int s, auto slc) {
     constexpr int SL = decltype(slc)::value;
-    double* b = buf[SL];
 #pragma unroll
     for (int k = 0; k < NLD; ++k) {
       const int idx = tid + k * NT;
-      if (idx < NP) b[idx] = staged(pre[SL][k]);
+      if (idx < NP) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) buf[SL][c][idx] = staged(pre[SL][k], c);
+      }
     }
     if (s + 2 < NS) fetch(Z0 - R + s + 2, SL);
     // output layer jo = s - 2R is finished in this step (its operands were fetched
@@ -389,36 +400,49 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
     const Epi E = ep[SL];
     if (jo + 1 >= 0 && jo + 1 < FZC) fetch_epi(jo + 1, ep[SL ^ 1]);
     __syncthreads();
-    // ring sums of the (dy, dx) window by squared in-plane distance m = dy'^2 + dx'^2
-    double P[2 * R * R + 1];
 #pragma unroll
-    for (int m = 0; m <= 2 * R * R; ++m) P[m] = 0.0;
+    for (int c = 0; c < C; ++c) {
+      const double* b = buf[SL][c];
+      // ring sums of the (dy, dx) window by squared in-plane distance m = dy'^2 + dx'^2
+      double P[2 * R * R + 1];
 #pragma unroll
-    for (int dy = 0; dy < K; ++dy)
+      for (int m = 0; m <= 2 * R * R; ++m) P[m] = 0.0;
 #pragma unroll
-      for (int dx = 0; dx < K; ++dx) P[(dy - R) * (dy - R) + (dx - R) * (dx - R)] += b[(ty + dy) * LX + tx + dx];
-    // the layer reaches output layers s - R +- o (window slots R +- o) with weight
-    // w(o^2 + m) on ring m
+      for (int dy = 0; dy < K; ++dy)
 #pragma unroll
-    for (int o = 0; o <= R; ++o) {
-      double q = 0.0;
+        for (int dx = 0; dx < K; ++dx) P[(dy - R) * (dy - R) + (dx - R) * (dx - R)] += b[(ty + dy) * LX + tx + dx];
+      // the layer reaches output layers s - R +- o (window slots R +- o) with weight
+      // w(o^2 + m) on ring m
 #pragma unroll
-      for (int m = 0; m <= 2 * R * R; ++m)
-        if (ring_taps(R, m) > 0) q += a.wc[o * o + m] * P[m];
-      win[R + o] += q;
-      if (o > 0) win[R - o] += q;
+      for (int o = 0; o <= R; ++o) {
+        double q = 0.0;
+#pragma unroll
+        for (int m = 0; m <= 2 * R * R; ++m)
+          if (ring_taps(R, m) > 0) q += a.wc[o * o + m] * P[m];
+        win[c][R + o] += q;
+        if (o > 0) win[c][R - o] += q;
+      }
     }
     const int z = Z0 + jo;
     if (jo >= 0 && col && z < nz) {
       const int64_t e = coff + exy * z;
       const bool act = E.f & kActive;
-      const double v = win[0];
+      const double v = win[0][0];
       if (MODE == kHs) {
         a.out[e] = v;
       } else if (MODE == kFwd) {
         a.out[e] = act ? v * E.ih : 0.0;
       } else if (MODE == kBwd) {
         a.out[e] = act ? v : 0.0;
+      } else if (MODE == kBwd2) {
+        const double v2 = win[C - 1][0];
+        a.out[e] = act ? v : 0.0;
+        a.out2[e] = act ? v2 : 0.0;
+        if (act && !(E.f & kFrozen)) {
+          dst[0] += 1.0;
+          dst[1] += -v;
+          dst[2] += v2;
+        }
       } else {
         double xp = 0.0, dxp = 0.0;
         if (act) {
@@ -433,8 +457,11 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
       }
     }
 #pragma unroll
-    for (int i = 0; i + 1 < K; ++i) win[i] = win[i + 1];
-    win[K - 1] = 0.0;
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+      for (int i = 0; i + 1 < K; ++i) win[c][i] = win[c][i + 1];
+      win[c][K - 1] = 0.0;
+    }
   };
   static_assert(NS % 2 == 0, "the step loop handles input layers in pairs");
 #pragma unroll 1
@@ -450,6 +477,23 @@ __global__ __launch_bounds__(FTX* FTY) void k_filter(FilterArgs a) {
       double sm = 0.0;
       for (int w = 0; w < NT / 64; ++w) sm += sred[w];
       a.part[blockIdx.x] = sm;
+    }
+  }
+  if (MODE == kBwd2 && a.part) {
+    // partials in the layout of the design statistics: [q][block], q = 3 (count), 4, 5; 0-2 unused
+    __shared__ double sst[3][NT / 64];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double v = dst[q];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if ((tid & 63) == 0) sst[q][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 6) {
+      double sm = 0.0;
+      if (tid >= 3)
+        for (int w = 0; w < NT / 64; ++w) sm += sst[tid - 3][w];
+      a.part[tid * gridDim.x + blockIdx.x] = sm;
     }
   }
 }
@@ -670,7 +714,7 @@ FilterContext::FilterContext(int nx, int ny, int nz, double r_vox, const uint8_t
   ANNEAL_HIP(hipMalloc(&Hs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&invHs_, n * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&r0_, n * sizeof(double)));
-  ANNEAL_HIP(hipMalloc(&part_, std::max(2 * nblk_, 6 * 512) * sizeof(double)));
+  ANNEAL_HIP(hipMalloc(&part_, std::max(6 * nblk_, 6 * 512) * sizeof(double)));
   ANNEAL_HIP(hipMalloc(&oc_, 2 * kOcLen * sizeof(double)));
   ANNEAL_HIP(hipHostMalloc((void**)&host_, 2 * kOcLen * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_dev_, host_, 0));
@@ -1089,6 +1133,38 @@ void design_stats(const double* xold, const double* xnew, const double* xphys, c
   ANNEAL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_design_stats_final, dim3(1), dim3(384), 0, s, part, kStatBlocks, out_dev);
   ANNEAL_HIP(hipGetLastError());
+}
+
+// Both backward passes of the sensitivities (optimize.py:142-143) in one
+// launch, with the design statistics that scale the OC bracket (count, sum of
+// -y1, sum of y2 over design voxels; entries 0-2 zero) when stats_dev is set.
+// Radii above the ring kernels' take the two gather passes and design_stats.
+void FilterContext::backward_pair(const double* d1, double* y1, const double* d2, double* y2, double* stats_dev) {
+  if (R_ > kRingMaxR) {
+    backward(d1, y1);
+    backward(d2, y2);
+    if (stats_dev) design_stats(nullptr, nullptr, nullptr, nullptr, y1, y2, flags_, g_.nelem(), stats_dev, part_, s_);
+    return;
+  }
+  FilterArgs a = args();
+  a.in = d1;
+  a.in2 = d2;
+  a.out = y1;
+  a.out2 = y2;
+  a.part = stats_dev ? part_ : nullptr;
+  switch (R_) {
+    case 1: hipLaunchKernelGGL((k_filter<1, kBwd2>), dim3(nblk_), dim3(FTX * FTY), 0, s_, a); break;
+    case 2: hipLaunchKernelGGL((k_filter<2, kBwd2>), dim3(nblk_), dim3(FTX * FTY), 0, s_, a); break;
+    case 3: hipLaunchKernelGGL((k_filter<3, kBwd2>), dim3(nblk_), dim3(FTX * FTY), 0, s_, a); break;
+    case 4: hipLaunchKernelGGL((k_filter<4, kBwd2>), dim3(nblk_), dim3(FTX * FTY), 0, s_, a); break;
+    case 5: hipLaunchKernelGGL((k_filter<5, kBwd2>), dim3(nblk_), dim3(FTX * FTY), 0, s_, a); break;
+    default: throw std::logic_error("ring radius out of range");
+  }
+  ANNEAL_HIP(hipGetLastError());
+  if (stats_dev) {
+    hipLaunchKernelGGL(k_design_stats_final, dim3(1), dim3(384), 0, s_, part_, nblk_, stats_dev);
+    ANNEAL_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace anneal

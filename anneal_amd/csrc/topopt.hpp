@@ -36,6 +36,11 @@ class FilterContext {
 
   void forward(const double* x, double* y);   // optimize.py:41-43
   void backward(const double* d, double* y);  // optimize.py:45-47
+  // backward(d1, y1) and backward(d2, y2) in one pass over the grid; with
+  // stats_dev, also stats_dev[3] = design count, [4] = sum of -y1, [5] = sum of
+  // y2 over design voxels (active, not frozen) and [0..2] = 0, the layout of
+  // design_stats.
+  void backward_pair(const double* d1, double* y1, const double* d2, double* y2, double* stats_dev);
   // xPhys, dxp = project(forward(x)), frozen -> 1 (optimize.py:81-93).
   // Also returns sum(phi * xPhys) over active voxels in *vol_dev (device).
   void physical(const double* x, double beta, const double* phi, double* xphys, double* dxp, double* vol_dev);

@@ -76,6 +76,12 @@ class DeviceFilter:
         _lib.call("anneal_filter_backward", self._ctx, _lib.ptr(d), _lib.ptr(out))
         return out
 
+    def backward_pair(self, d1, out1, d2, out2, stats=None):
+        """backward(d1) and backward(d2) in one launch (same bits); ``stats`` (6 doubles on the device, may be
+        None) receives the design count, sum(-out1) and sum(out2) over design voxels at [3], [4], [5]."""
+        _lib.call("anneal_filter_backward_pair", self._ctx, _lib.ptr(d1), _lib.ptr(out1), _lib.ptr(d2),
+                  _lib.ptr(out2), _lib.ptr(stats) if stats is not None else None)
+
     def physical(self, x, beta, phi, xphys, dxp, vol=None):
         _lib.call("anneal_filter_physical", self._ctx, _lib.ptr(x), float(beta), _lib.ptr(phi), _lib.ptr(xphys),
                   _lib.ptr(dxp), _lib.ptr(vol))
@@ -427,12 +433,10 @@ class SimpRun:
         _lib.call("anneal_fem_sensitivity", fem._ctx, _lib.ptr(self.U), self.ncase, _lib.ptr(flags),
                   _lib.ptr(self.xphys), _lib.ptr(self.dxp), _lib.ptr(self.phi), _lib.ptr(self.coef_dev),
                   float(self._p), self.emin, self.E0, _lib.ptr(self.dcdx), _lib.ptr(self.dvdx))
-        filt.backward(self.dcdx, self.dcx)
-        filt.backward(self.dvdx, self.dvx)
+        # both backward passes and the statistics that scale the OC bracket in one launch
+        filt.backward_pair(self.dcdx, self.dcx, self.dvdx, self.dvx, self.stats)
         self._td = self._tick()
         T["sensitivity"] += self._td - tc
-        _lib.call("anneal_design_stats", None, None, None, _lib.ptr(self.phi), _lib.ptr(self.dcx), _lib.ptr(self.dvx),
-                  _lib.ptr(flags), nel, _lib.ptr(self.stats), _lib.ptr(self.work), stream)
         self.stats_host.copy_(self.stats, non_blocking=True)
 
     def _oc_begin(self):

@@ -138,6 +138,14 @@ int anneal_filter_destroy(void* ctx);
  * arrays; inactive voxels read as and are written as 0. */
 int anneal_filter_forward(void* ctx, const double* x_dev, double* y_dev);
 int anneal_filter_backward(void* ctx, const double* d_dev, double* y_dev);
+/* The two backward passes of the sensitivities (optimize.py:142-143: dc and
+ * dv) in one launch: y1 = backward(d1), y2 = backward(d2), the same bits as two
+ * anneal_filter_backward calls.  stats_dev (may be NULL) receives the design
+ * statistics the OC bracket is formed from, in the layout of the statistics
+ * array: [3] the number of design voxels, [4] the sum of -y1, [5] the sum of y2
+ * over them, [0..2] = 0. */
+int anneal_filter_backward_pair(void* ctx, const double* d1_dev, double* y1_dev, const double* d2_dev,
+                                double* y2_dev, double* stats_dev);
 /* physical(x, with_grad=True) (optimize.py:81-93): filter, Heaviside
  * projection with sharpness beta (identity for beta <= 0), frozen -> 1.
  * vol_dev (may be NULL) receives sum(phi * xPhys). */

@@ -51,6 +51,35 @@ def test_filter_forward_backward(torch, shape, r, seed):
         assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
 
 
+@pytest.mark.parametrize("shape,r,seed", [((9, 7, 5), 1.5, 0), ((40, 11, 13), 3.0, 1), ((35, 13, 11), 4.0, 4),
+                                          ((30, 17, 14), 7.2, 6)])
+def test_backward_pair_equals_two_backward_passes(torch, shape, r, seed):
+    """The paired backward pass of the sensitivities gives the bits of two single passes and the design
+    statistics of the pair (count, sum of -y1, sum of y2 over active non-frozen voxels) to FP64 rounding;
+    radii 1.5-4 run the two-channel ring kernel, 7.2 the gather passes."""
+    print(f"backward pair shape={shape} r={r}", flush=True)
+    from anneal_amd.fem import Grid
+    from anneal_amd.optimize import DeviceFilter
+    g, active, frozen, rng = _setup(shape, seed, r)
+    df = DeviceFilter(Grid(shape, 1.0, (0, 0, 0)), active, r, frozen)
+    n = g.nelem
+    d1 = torch.from_numpy(rng.standard_normal(n)).cuda()
+    d2 = torch.from_numpy(rng.random(n)).cuda()
+    y1, y2 = df.backward(d1), df.backward(d2)
+    p1, p2 = torch.empty_like(d1), torch.empty_like(d2)
+    st = torch.full((6,), 7.0, dtype=torch.float64, device="cuda")
+    df.backward_pair(d1, p1, d2, p2, st)
+    assert torch.equal(p1, y1) and torch.equal(p2, y2)
+    des = active & ~frozen
+    a1, a2, st = y1.cpu().numpy(), y2.cpu().numpy(), st.cpu().numpy()
+    assert st[0] == 0.0 and st[1] == 0.0 and st[2] == 0.0
+    assert st[3] == des.sum()
+    assert abs(st[4] - (-a1[des]).sum()) <= 1e-12 * np.abs(a1[des]).sum()
+    assert abs(st[5] - a2[des].sum()) <= 1e-12 * np.abs(a2[des]).sum()
+    df.backward_pair(d1, p1, d2, p2)                # without statistics
+    assert torch.equal(p1, y1) and torch.equal(p2, y2)
+
+
 @pytest.mark.parametrize("beta,r", [(0.0, 1.5), (1.0, 1.5), (4.0, 1.5), (2.0, 4.0), (4.0, 5.5), (2.0, 7.2)])
 def test_physical_projection(torch, beta, r):
     from anneal_amd.optimize import DeviceFilter
