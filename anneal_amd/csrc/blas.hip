@@ -409,6 +409,41 @@ void pcg_update(float* x, float* r, const float* p, const float* q, const double
   ANNEAL_HIP(hipGetLastError());
 }
 
+// |r|^2 of the step and the device-side convergence test (one block per rhs;
+// the summation order is k_finalize's).  A rhs skipped by the step (done
+// before it) left no partials and is not tested again.
+__global__ __launch_bounds__(kRedThreads) void k_finalize_converge(const double* __restrict__ part, int nblk,
+                                                                   double* __restrict__ st,
+                                                                   double* __restrict__ rr_host,
+                                                                   double* __restrict__ done_host) {
+  const int r = blockIdx.x;
+  double* q = st + r * kSt;
+  const bool live = q[kDone] == 0.0;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedThreads) s += part[(int64_t)r * nblk + i];
+  __shared__ double sh[kRedThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int w = 0; w < kRedThreads / 64; ++w) t += sh[w];
+  if (live) {
+    rr_host[r] = t;
+    if (t <= q[kTol2]) q[kDone] = 1.0;
+  }
+  done_host[r] = q[kDone];
+}
+void pcg_update_converge(float* x, float* r, const float* p, const float* q, double* st, int64_t n, int64_t ld,
+                         int nrhs, double* part, hipStream_t s, double* rr_host, double* done_host) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(kDotBlocks, nrhs), dim3(kRedThreads), 0, s, x, r, p, q, st, n, ld, part,
+                     (const double*)st);
+  ANNEAL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize_converge, dim3(nrhs), dim3(kRedThreads), 0, s, part, kDotBlocks, st, rr_host,
+                     done_host);
+  ANNEAL_HIP(hipGetLastError());
+}
+
 // p = z + beta p
 __global__ void k_pcg_dir(float* __restrict__ p, const float* __restrict__ z, const double* __restrict__ st, int64_t n,
                           int64_t ld, int first) {
@@ -434,8 +469,8 @@ __global__ __launch_bounds__(256) void k_upload(const double* __restrict__ src1,
   for (int i = threadIdx.x; i < n2; i += 256) dst2[i] = src2[i];
 }
 __global__ __launch_bounds__(64) void k_pass_setup(const double* __restrict__ rr, const double* __restrict__ ff,
-                                                   double rtol, double* __restrict__ scale, double* __restrict__ st,
-                                                   int nrhs) {
+                                                   double rtol, double factor, double floor_,
+                                                   double* __restrict__ scale, double* __restrict__ st, int nrhs) {
   const int r = threadIdx.x;
   if (r >= nrhs) return;
   const double rn = sqrt(rr[r]), fn = sqrt(ff[r]);
@@ -443,9 +478,12 @@ __global__ __launch_bounds__(64) void k_pass_setup(const double* __restrict__ rr
   scale[r] = done ? 0.0 : 1.0 / rn;
   for (int k = 0; k < kSt; ++k) st[r * kSt + k] = 0.0;
   st[r * kSt + kDone] = done ? 1.0 : 0.0;
+  const double t = fmax(done ? 1.0 : factor * rtol * fn / rn, floor_);
+  st[r * kSt + kTol2] = t * t;
 }
-void pass_setup(const double* rr, const double* ff, double rtol, double* scale, double* st, int nrhs, hipStream_t s) {
-  hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(64), 0, s, rr, ff, rtol, scale, st, nrhs);
+void pass_setup(const double* rr, const double* ff, double rtol, double factor, double floor_, double* scale,
+                double* st, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pass_setup, dim3(1), dim3(64), 0, s, rr, ff, rtol, factor, floor_, scale, st, nrhs);
   ANNEAL_HIP(hipGetLastError());
 }
 

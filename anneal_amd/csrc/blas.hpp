@@ -8,7 +8,8 @@ constexpr int kDotBlocks = 512;  // fixed reduction grid: results do not depend 
 
 // Per right-hand-side PCG scalars on the device (FP64).
 constexpr int kSt = 8;
-constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kRR = 3, kAlpha = 4, kBeta = 5, kDone = 6, kZQ = 7;
+// kTol2: the inner tolerance on |r|^2 of a rhs; the PCG step that reaches it sets kDone on the device
+constexpr int kRZ = 0, kRZnew = 1, kPQ = 2, kTol2 = 3, kAlpha = 4, kBeta = 5, kDone = 6, kZQ = 7;
 
 // Kernels of a PCG inner loop take an optional `skip` pointer (the per-rhs
 // scalar block above): the workgroups of a right-hand side whose done flag is
@@ -73,13 +74,20 @@ void dot_partial(const float* a, const float* b, int64_t n, int64_t ld, int nrhs
 void pcg_update(float* x, float* r, const float* p, const float* q, const double* st, int64_t n, int64_t ld, int nrhs,
                 double* part, double* out_rr, hipStream_t s, const double* skip = nullptr,
                 double* out_host = nullptr);
+// pcg_update with the convergence test on the device: rr = |r|^2 per rhs goes to
+// rr_host; a live rhs with rr <= st[kTol2] gets st[kDone] = 1, so the kernels
+// queued after it skip that rhs; done_host receives st[kDone] (both pinned).
+void pcg_update_converge(float* x, float* r, const float* p, const float* q, double* st, int64_t n, int64_t ld,
+                         int nrhs, double* part, hipStream_t s, double* rr_host, double* done_host);
 void pcg_dir(float* p, const float* z, const double* st, int64_t n, int64_t ld, int nrhs, int first, hipStream_t s);
 void to_f32_scaled(const double* a, float* b, const double* sc, int64_t n, int64_t ld, int nrhs, hipStream_t s);
 // PCG pass set-up on the device from |r|^2 (rr) and |F|^2 (ff) per rhs: a rhs
 // with |r| <= rtol |F| is done (scale 0, done flag set), otherwise scale 1/|r|;
-// the scalar blocks start from zero.  The host forms the same decisions from
-// the same norms after its next wait.
-void pass_setup(const double* rr, const double* ff, double rtol, double* scale, double* st, int nrhs, hipStream_t s);
+// the scalar blocks start from zero but for the inner tolerance kTol2 =
+// max(factor rtol |F| / |r|, floor)^2 (1 for a done rhs).  The host forms the
+// same pass decisions from the same norms after its next wait.
+void pass_setup(const double* rr, const double* ff, double rtol, double factor, double floor_, double* scale,
+                double* st, int nrhs, hipStream_t s);
 // dst1[0, n1) = src1[0, n1) and dst2[0, n2) = src2[0, n2) in one small kernel;
 // src1 / src2 are device aliases of pinned host memory (n2 = 0: no second copy)
 void upload(const double* src1, double* dst1, int n1, const double* src2, double* dst2, int n2, hipStream_t s);

@@ -279,9 +279,11 @@ void FemContext::alloc_solver(int nrhs_max) {
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_up_dev_, host_up_, 0));
   ANNEAL_HIP(hipStreamCreateWithFlags(&cap_s_, hipStreamNonBlocking));
   ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+  ANNEAL_HIP(hipEventCreateWithFlags(&ev2_, hipEventDisableTiming));
   vgraph_.assign(nrhs_max + 1, nullptr);
   sgraph_.assign(nrhs_max + 1, nullptr);
   dgraph_.assign(nrhs_max + 1, nullptr);
+  cgraph_.assign(nrhs_max + 1, nullptr);
 }
 
 FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hipStream_t stream)
@@ -372,7 +374,7 @@ void FemContext::set_rhs_count(int nrhs) {
 
 void FemContext::drop_graphs() {
   ++graph_gen_;  // graphs recorded elsewhere from this context's kernels are stale too
-  for (auto* set : {&vgraph_, &sgraph_, &dgraph_})
+  for (auto* set : {&vgraph_, &sgraph_, &dgraph_, &cgraph_})
     for (auto& g : *set)
       if (g) {
         (void)hipGraphExecDestroy(g);
@@ -506,6 +508,7 @@ FemContext::~FemContext() {
   drop_graphs();
   if (cap_s_) (void)hipStreamDestroy(cap_s_);
   if (ev_) (void)hipEventDestroy(ev_);
+  if (ev2_) (void)hipEventDestroy(ev2_);
   for (void* p : owned_) (void)hipFree(p);
   if (host_red_) (void)hipHostFree(host_red_);
   if (host_up_) (void)hipHostFree(host_up_);
@@ -549,42 +552,58 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
   return a;
 }
 
-void FemContext::pcg_step(int nrhs) {
+void FemContext::issue_step(int nrhs) {
   const GridDims& g0 = lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
-  auto issue = [&] {
-    ApplyArgs<float> af = args<float>(0);
-    af.u = pp_;
-    af.y = pq_;
-    af.dot_part = part_;
-    af.skip = st_;
-    launch_apply(af, kApply, nrhs, s_);
-    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
-    pcg_update(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, red_ + 64, s_, st_, host_red_dev_ + 64);
-  };
+  ApplyArgs<float> af = args<float>(0);
+  af.u = pp_;
+  af.y = pq_;
+  af.dot_part = part_;
+  af.skip = st_;
+  launch_apply(af, kApply, nrhs, s_);
+  finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
+  // |r|^2 to the host; a rhs that meets its inner tolerance is marked done on the device
+  pcg_update_converge(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, s_, host_red_dev_ + 64, host_red_dev_ + 192);
+}
+
+void FemContext::issue_direction(int nrhs) {
+  const GridDims& g0 = lv_[0].g;
+  const int64_t n = g0.nnode(), ld = g0.ldn();
+  // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
+  vcycle(0, pr_, pz_, nrhs, part_, st_);
+  finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
+  pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
+}
+
+void FemContext::pcg_step(int nrhs) {
   if (!use_graphs) {
-    issue();
+    issue_step(nrhs);
     return;
   }
-  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture(issue);
+  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture([&] { issue_step(nrhs); });
   ANNEAL_HIP(hipGraphLaunch(sgraph_[nrhs], s_));
 }
 
 void FemContext::pcg_direction(int nrhs) {
-  const GridDims& g0 = lv_[0].g;
-  const int64_t n = g0.nnode(), ld = g0.ldn();
-  // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
-  auto issue = [&] {
-    vcycle(0, pr_, pz_, nrhs, part_, st_);
-    finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
-    pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
-  };
   if (!use_graphs) {
-    issue();
+    issue_direction(nrhs);
     return;
   }
-  if (!dgraph_[nrhs]) dgraph_[nrhs] = capture(issue);
+  if (!dgraph_[nrhs]) dgraph_[nrhs] = capture([&] { issue_direction(nrhs); });
   ANNEAL_HIP(hipGraphLaunch(dgraph_[nrhs], s_));
+}
+
+void FemContext::pcg_direction_step(int nrhs) {
+  if (!use_graphs) {
+    issue_direction(nrhs);
+    issue_step(nrhs);
+    return;
+  }
+  if (!cgraph_[nrhs]) cgraph_[nrhs] = capture([&] {
+      issue_direction(nrhs);
+      issue_step(nrhs);
+    });
+  ANNEAL_HIP(hipGraphLaunch(cgraph_[nrhs], s_));
 }
 
 void FemContext::set_Ee(const double* Ee_dev) {
@@ -861,7 +880,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     const bool optimistic = outer == 0;
     bool pending = optimistic;
     if (optimistic) {
-      pass_setup(red_, red_ + 128, rtol, scale_, st_, nrhs, s_);
+      pass_setup(red_, red_ + 128, rtol, inner_tol_factor, inner_tol_floor, scale_, st_, nrhs, s_);
       for (int r = 0; r < nrhs; ++r) idone[r] = 0;
     } else {
       ANNEAL_HIP(hipStreamSynchronize(s_));
@@ -877,6 +896,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         sc[r] = done[r] ? 0.0 : 1.0 / rnorm[r];
         for (int k = 0; k < kSt; ++k) hst[r * kSt + k] = 0.0;
         hst[r * kSt + kDone] = done[r] ? 1.0 : 0.0;
+        hst[r * kSt + kTol2] = tol2[r];
         idone[r] = done[r];
       }
       std::copy(sc.begin(), sc.end(), host_up_);
@@ -914,9 +934,19 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     // rarely wasted.
     constexpr double kFastFirst = 0.003;
     std::vector<double> rr1(nrhs, 1.0), rr2(nrhs, 1.0 / kFastFirst);
+    // The plain multigrid iteration runs as one graph: the direction of step k
+    // followed by step k + 1 (one launch per iteration).  A rhs whose step meets
+    // its tolerance is marked done on the device, so a step queued before the
+    // host has seen that leaves it unchanged.
+    const bool flexible = net_ && net_mode_ == 2;
+    const bool merged = !flexible && lv_.size() > 1;
+    bool step_queued = false;  // step it was queued with the previous direction
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
-      pcg_step(nrhs);
-      ANNEAL_HIP(hipEventRecord(ev_, s_));
+      if (!step_queued) {
+        pcg_step(nrhs);
+        ANNEAL_HIP(hipEventRecord(ev_, s_));
+      }
+      step_queued = false;
       // the first step of an optimistic pass is followed by another unless the
       // start already met the tolerance (the first pass took 2-14 steps on every
       // solve of the cantilever schedule): queue it before the tolerances are known
@@ -926,10 +956,17 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         const double rate = rr1[r] / rr2[r];
         if (rr1[r] * rate <= 2.0 * tol2[r]) speculate = false;  // may converge now: do not queue work
       }
-      const bool flexible = net_ && net_mode_ == 2;
-      auto next_direction = [&]() {
-        if (!flexible && lv_.size() > 1) {
-          pcg_direction(nrhs);  // the plain multigrid preconditioner: one graph
+      // total_after: the step count once this step is counted
+      auto next_direction = [&](int total_after) {
+        if (merged) {
+          // with the next step when the loop would run it
+          if (it + 1 < inner_max && total_after < maxiter) {
+            pcg_direction_step(nrhs);
+            ANNEAL_HIP(hipEventRecord(ev2_, s_));
+            step_queued = true;
+          } else {
+            pcg_direction(nrhs);
+          }
           return;
         }
         const bool fused = precondition(nrhs);
@@ -944,14 +981,14 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         }
         pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
       };
-      if (speculate) next_direction();
+      if (speculate) next_direction(total + 1);
       ANNEAL_HIP(hipEventSynchronize(ev_));
       if (pending) {
         resolve();
         if (bad || all) break;  // the device skipped every rhs already within the tolerance
       }
       ++total;
-      bool iall = true, changed = false;
+      bool iall = true;
       for (int r = 0; r < nrhs; ++r) {
         if (idone[r]) continue;
         ++its[r];
@@ -959,19 +996,12 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         rr2[r] = rr1[r];
         rr1[r] = rr;
         if (!std::isfinite(rr)) bad = true;
-        if (rr <= tol2[r]) {
-          idone[r] = 1;
-          changed = true;
-        }
+        if (host_red_[192 + r] != 0.0) idone[r] = 1;  // the device's test |r|^2 <= tol2
         iall = iall && idone[r];
       }
       if (bad || iall) break;
-      if (changed) {
-        for (int r = 0; r < nrhs; ++r) hst[r * kSt + kDone] = idone[r] ? 1.0 : 0.0;
-        ANNEAL_HIP(hipMemcpy2DAsync(st_ + kDone, kSt * sizeof(double), hst.data() + kDone, kSt * sizeof(double),
-                                    sizeof(double), nrhs, hipMemcpyHostToDevice, s_));
-      }
-      if (!speculate) next_direction();
+      if (!speculate) next_direction(total);
+      if (step_queued) std::swap(ev_, ev2_);
     }
     if (pending) {  // no PCG step ran (maxiter reached): decide from the residual alone
       ANNEAL_HIP(hipStreamSynchronize(s_));

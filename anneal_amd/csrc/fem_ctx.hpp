@@ -209,15 +209,20 @@ class FemContext {
   bool tuned_ = false;
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
-  hipEvent_t ev_ = nullptr;                 // marks the residual-norm copy of a PCG step
+  hipEvent_t ev_ = nullptr, ev2_ = nullptr;  // mark the residual-norm copies of PCG steps
   std::vector<hipGraphExec_t> vgraph_;      // per rhs count
   std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
   // the two halves of a plain PCG iteration, per rhs count: step (K p with
   // <p, K p>, alpha, x and r update with |r|^2 to the host) and direction
   // (V-cycle with <r, z>, beta, p = z + beta p)
-  std::vector<hipGraphExec_t> sgraph_, dgraph_;
+  // and both in one graph (direction of step k, then step k + 1): one graph
+  // launch per iteration once the loop runs
+  std::vector<hipGraphExec_t> sgraph_, dgraph_, cgraph_;
   void pcg_step(int nrhs);
   void pcg_direction(int nrhs);
+  void pcg_direction_step(int nrhs);
+  void issue_step(int nrhs);
+  void issue_direction(int nrhs);
   // records what issue() launches on s_ (redirected to the capture stream)
   template <typename F>
   hipGraphExec_t capture(F&& issue);
