@@ -27,29 +27,32 @@ __device__ __forceinline__ KeCoef<T> scaled(const KeCoef<T>& k, T e) {
   return KeCoef<T>{k.nAmB * e, k.nB * e, k.g * e, k.pc * e, k.pd * e, k.tAmB * e, k.tB * e, k.w * e};
 }
 
-// Raw buffer access.  A resource covers one node plane (or one element
-// layer) of one array; lanes outside the tile's valid region use an
-// out-of-range offset, so their loads return 0 and their stores are dropped
-// by the hardware range check, with no per-lane selects or branches.
-constexpr uint32_t kOob = 0xFFFFFFF0u;
+// Raw buffer access.  A resource covers one whole array (the range is its
+// byte length, below 2 GiB) and each access adds a uniform byte offset (the
+// node plane or element layer, in a scalar register) to the lane's offset.
+// Lanes outside the tile's valid region use the offset kOob = 2^31, so their
+// loads return 0 and their stores are dropped by the hardware range check,
+// with no per-lane selects or branches (kOob plus any plane offset below 2^31
+// stays out of range, whether or not the check includes the scalar offset).
+constexpr uint32_t kOob = 0x80000000u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
 }
 template <typename T>
-__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
   } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
   }
 }
 template <typename T>
-__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   if constexpr (sizeof(T) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
   } else {
     using V2 = __attribute__((ext_vector_type(2))) uint32_t;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
   }
 }
 
@@ -69,11 +72,18 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int nb = a.ntx * a.nty * a.nzc;
-  const int lid = xcd_logical(blockIdx.x, nb);
-  const int bz = lid % a.nzc;
-  const int bxy = lid / a.nzc;
-  const int bx = bxy % a.ntx, by = bxy / a.ntx;
-  const int r = blockIdx.y;
+  // Workgroups reach the 8 XCDs round-robin in launch order; xcd_logical gives
+  // each XCD a contiguous run of (rhs, tile) ids, and tiles are numbered x
+  // fastest, then y, then z, so an XCD holds whole rows of the tile grid:
+  // their shared halo columns and rows, and the cache lines that two x
+  // neighbours both write in part, meet in one L2 (z fastest, the former
+  // order, was 3 % slower over the SIMP step; no remapping, 20 %).
+  const int gid = xcd_logical(blockIdx.x + nb * blockIdx.y, nb * gridDim.y);
+  const int r = gid / nb;
+  const int lid = gid - r * nb;
+  const int bx = lid % a.ntx;
+  const int byz = lid / a.ntx;
+  const int by = byz % a.nty, bz = byz / a.nty;
   if (rhs_skipped(a.skip, r)) return;
   const int X0 = bx * (TX - 1), Y0 = by * (TY - 1), Z0 = bz * ZC;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -96,9 +106,9 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     opdinv = o.dinv;
   }
 
-  // Addressing: every access goes through a buffer resource for one plane
-  // (uniform, scalar registers) plus a per-thread byte offset that is fixed
-  // along z; out-of-range lanes carry kOob.
+  // Addressing: every access goes through the buffer resource of its array
+  // (uniform, scalar registers), the plane's byte offset (scalar) and a
+  // per-thread byte offset that is fixed along z; out-of-range lanes carry kOob.
   uint32_t lvo[NLD];  // byte offsets of this thread's load slots in a node plane
 #pragma unroll
   for (int k = 0; k < NLD; ++k) {
@@ -117,15 +127,20 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const uint32_t nvo = ncol ? uint32_t(ni + sx * nj) : kOob;   // element index (scaled per array below)
   const uint32_t pbytes = uint32_t(sxy) * uint32_t(sizeof(T));
 
-  // Plane addressing: one buffer resource per array and plane, based at the
-  // plane (a 32-bit byte offset: the launcher keeps one component of a node
-  // array below 2 GiB), with the plane's byte count as its range.  Planes
-  // outside [0, nz] are clamped: their values only meet element layers outside
-  // the grid, whose stiffness is 0, and no output is stored for them.
+  // Plane addressing: a plane's byte offset in its array (32 bits: the
+  // launcher keeps the 3 ldn entries of one right-hand side below 2 GiB).
+  // Planes outside [0, nz] are clamped: their values only meet element layers
+  // outside the grid, whose stiffness is 0, and no output is stored for them.
   auto plane_off = [&](int p) -> uint32_t { return uint32_t(min(max(p, 0), nz)) * pbytes; };
-  auto at = [](const void* base, uint32_t off) -> const void* { return (const char*)base + off; };
-  // components of a node array lie ldn entries apart (the launcher keeps 3 ldn entries below 4 GiB)
+  // components of a node array lie ldn entries apart
   const uint32_t cstep = uint32_t(ldn) * uint32_t(sizeof(T));
+  const uint32_t vbytes = 3u * cstep;                                       // one rhs of a node array
+  const uint32_t nnode = uint32_t(sxy) * uint32_t(nz + 1);
+  const __amdgpu_buffer_rsrc_t ru = rsrc(u, vbytes), ry = rsrc(y, vbytes);
+  const __amdgpu_buffer_rsrc_t rb = rsrc(bb, MODE != kApply ? vbytes : 0u);
+  const __amdgpu_buffer_rsrc_t rp = rsrc(pr ? pr : u, pr ? vbytes : 0u);  // empty: the loads return 0
+  const __amdgpu_buffer_rsrc_t rm = rsrc(opmask, nnode);
+  const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv, (MODE == kJacobi || MODE == kCheb || MODE == kCheb0) ? nnode * 4u : 0u);
 
   // Node planes are prefetched two steps ahead (two register slots), so the
   // load latency of a plane is covered by two steps of compute.
@@ -138,35 +153,31 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     if (MODE == kCheb0) {
       // first smoothing step from a zero guess: u = M omega D^-1 b
       const uint32_t on = o / uint32_t(sizeof(T));  // the plane's first node
-      const __amdgpu_buffer_rsrc_t rm = rsrc(opmask + on, uint32_t(sxy));
-      const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv + on, uint32_t(sxy) * 4u);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
         const uint32_t node = lvo[k] == kOob ? kOob : lvo[k] / uint32_t(sizeof(T));
-        const uint8_t m = __builtin_amdgcn_raw_buffer_load_b8(rm, node, 0, 0);
-        const T d = a.omega * T(bload<float>(rd, node == kOob ? kOob : node * 4u));
+        const uint8_t m = __builtin_amdgcn_raw_buffer_load_b8(rm, node, on, 0);
+        const T d = a.omega * T(bload<float>(rd, node == kOob ? kOob : node * 4u, on * 4u));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T bv = bload<T>(rsrc(at(bb, o + c * cstep), pbytes), lvo[k]);
+          const T bv = bload<T>(rb, lvo[k], o + c * cstep);
           pre[slot][k][c] = ((m >> c) & 1) ? d * bv : T(0);
         }
       }
     } else {
-      const __amdgpu_buffer_rsrc_t r0 = rsrc(at(u, o), pbytes), r1 = rsrc(at(u, o + cstep), pbytes),
-                                   r2 = rsrc(at(u, o + 2 * cstep), pbytes);
 #pragma unroll
       for (int k = 0; k < NLD; ++k) {
-        pre[slot][k][0] = bload<T>(r0, lvo[k]);
-        pre[slot][k][1] = bload<T>(r1, lvo[k]);
-        pre[slot][k][2] = bload<T>(r2, lvo[k]);
+        pre[slot][k][0] = bload<T>(ru, lvo[k], o);
+        pre[slot][k][1] = bload<T>(ru, lvo[k], o + cstep);
+        pre[slot][k][2] = bload<T>(ru, lvo[k], o + 2 * cstep);
       }
     }
   };
   const uint32_t ebytes = uint32_t(nx) * uint32_t(ny) * uint32_t(sizeof(T));  // one element layer
+  const __amdgpu_buffer_rsrc_t re = rsrc(opEe, ebytes * uint32_t(nz));
   auto fetch_e = [&](int L) -> T {
     const bool lin = L >= 0 && L < nz;  // layers outside the grid read as 0
-    const __amdgpu_buffer_rsrc_t re = rsrc(at(opEe, uint32_t(min(max(L, 0), nz - 1)) * ebytes), lin ? ebytes : 0u);
-    return bload<T>(re, evo);
+    return bload<T>(re, lin ? evo : kOob, uint32_t(min(max(L, 0), nz - 1)) * ebytes);
   };
 
   // Operands of the output node in plane L (mask, b, D^-1, previous iterate),
@@ -180,13 +191,9 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const uint32_t ovo = nvo == kOob ? kOob : nvo * uint32_t(sizeof(T));
   const uint32_t dvo = nvo == kOob ? kOob : nvo * 4u;
   auto fetch_epi = [&](int L, Epi& E) {
-    uint32_t o = plane_off(L);
-    // the fused first steps staged plane L two steps earlier from the same
-    // arrays: an opaque offset keeps the compiler from holding those plane
-    // resources in scalar registers until now (which spilled them)
-    if constexpr (MODE == kCheb0) asm volatile("" : "+s"(o));
+    const uint32_t o = plane_off(L);
     const uint32_t on = o / uint32_t(sizeof(T));
-    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rsrc(opmask + on, uint32_t(sxy)), nvo, 0, 0);
+    E.mk = __builtin_amdgcn_raw_buffer_load_b8(rm, nvo, on, 0);
     E.dv = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -195,17 +202,15 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     }
     if (MODE != kApply) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rsrc(at(bb, o + c * cstep), pbytes), ovo);
+      for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rb, ovo, o + c * cstep);
     }
-    if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0)
-      E.dv = bload<float>(rsrc(opdinv + on, uint32_t(sxy) * 4u), dvo);
-    if (MODE == kCheb && pr) {
+    if (MODE == kJacobi || MODE == kCheb || MODE == kCheb0) E.dv = bload<float>(rd, dvo, on * 4u);
+    if (MODE == kCheb) {
+      // without a previous-iterate array the resource is empty and the loads
+      // return 0 (no branch: merging loaded and computed values would wait
+      // for the loads right here)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rsrc(at(pr, o + c * cstep), pbytes), ovo);
-    } else if (MODE == kCheb && a.prev_j0) {
-      // previous iterate x1 = omega0 D^-1 b of a smoothing that started from zero
-#pragma unroll
-      for (int c = 0; c < 3; ++c) E.pv[c] = a.omega0 * T(E.dv) * E.bv[c];
+      for (int c = 0; c < 3; ++c) E.pv[c] = bload<T>(rp, ovo, o + c * cstep);
     }
   };
 
@@ -260,8 +265,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
         fetch_epi(L, E);
       }
     }
-    const uint32_t lb = plane_off(L);           // byte offset of the output plane
-    const uint32_t nb_out = outp ? pbytes : 0u;  // stores dropped without an output plane
+    const uint32_t lb = plane_off(L);  // byte offset of the output plane
 
     __syncthreads();
     T Fcur[3][4];
@@ -318,9 +322,13 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
             else if (MODE == kResid) out = E.bv[c] - v;
             else if (MODE == kJacobi) out = uc + a.omega * T(E.dv) * (E.bv[c] - v);
             else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(E.dv) * (E.bv[c] - v);
-            else out = uc + a.c1 * (uc - E.pv[c]) + a.c2 * T(E.dv) * (E.bv[c] - v);
+            else {
+              // previous iterate: loaded, or x1 = omega0 D^-1 b of a smoothing that started from zero
+              const T pv = a.prev_j0 ? a.omega0 * T(E.dv) * E.bv[c] : E.pv[c];
+              out = uc + a.c1 * (uc - pv) + a.c2 * T(E.dv) * (E.bv[c] - v);
+            }
             if (!((E.mk >> c) & 1)) out = T(0);
-            bstore<T>(out, rsrc(at(y, lb + c * cstep), nb_out), ovo);  // dropped for lanes without a node
+            bstore<T>(out, ry, outp ? ovo : kOob, lb + c * cstep);  // dropped for lanes without a node
             if (MODE == kApply) {
               if (a.dot_part && ncol) dot += (double)uc * (double)out;
             } else if (MODE == kCheb || MODE == kJacobi) {
@@ -351,7 +359,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
     if (tid == 0) {
       double sum = 0.0;
       for (int w = 0; w < NT / 64; ++w) sum += sred[w];
-      a.dot_part[(int64_t)r * gridDim.x + blockIdx.x] = sum;
+      a.dot_part[(int64_t)r * nb + lid] = sum;
     }
   }
 }
@@ -545,8 +553,9 @@ int apply_blocks(int nx, int ny, int nz, int variant) {
 template <typename T, int MODE, int TX, int TY, int ZC>
 static void launch_tile(const ApplyArgs<T>& a0, int nrhs, hipStream_t s) {
   ApplyArgs<T> a = a0;
-  ANNEAL_CHECK(3 * a.ldn * (int64_t)sizeof(T) < (int64_t(1) << 32),
-               "grid too large for the 32-bit node-array offsets of the apply kernel");
+  ANNEAL_CHECK(3 * a.ldn * (int64_t)sizeof(T) < (int64_t(1) << 31) &&
+                   (int64_t)a.nx * a.ny * a.nz * (int64_t)sizeof(T) < (int64_t(1) << 31),
+               "grid too large for the 31-bit array offsets of the apply kernel");
   a.ntx = (a.nx + 1 + TX - 2) / (TX - 1);
   a.nty = (a.ny + 1 + TY - 2) / (TY - 1);
   a.nzc = (a.nz + 1 + ZC - 1) / ZC;
