@@ -476,7 +476,7 @@ struct XferK {
   int B, fx, fy, fz, cx, cy, cz;  // fine and coarse node dims per sample
 };
 
-__global__ void k_net_restrict(XferK a) {
+__global__ void __launch_bounds__(256) k_net_restrict(XferK a) {
   const int G = a.c >> 3;
   const int64_t ncv = (int64_t)a.cx * a.cy * a.cz, nfv = (int64_t)a.fx * a.fy * a.fz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,25 +488,33 @@ __global__ void k_net_restrict(XferK a) {
   const int K = int(loc / ((int64_t)a.cx * a.cy));
   const int64_t rem = loc - (int64_t)K * a.cx * a.cy;
   const int J = int(rem / a.cx), I = int(rem - (int64_t)J * a.cx);
+  // all 27 loads are issued together: out-of-grid taps read a clamped
+  // in-grid voxel with weight 0 (a loop that skipped them waited for each
+  // load in turn: 12 us per launch whatever the grid size)
+  u32x4_t raw[27];
+  float wt[27];
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int t9 = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        const int fk = 2 * K + dz, fj = 2 * J + dy, fi = 2 * I + dx;
+        const bool in = fk >= 0 && fk < a.fz && fj >= 0 && fj < a.fy && fi >= 0 && fi < a.fx;
+        const int ck = min(max(fk, 0), a.fz - 1), cj = min(max(fj, 0), a.fy - 1), ci = min(max(fi, 0), a.fx - 1);
+        wt[t9] = in ? (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f) : 0.f;
+        raw[t9] = ld16(a.x + ((int64_t)b * nfv + ci + (int64_t)a.fx * (cj + (int64_t)a.fy * ck)) * a.xpitch + 8 * q);
+      }
   float acc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0.f;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int fk = 2 * K + dz;
-    if (fk < 0 || fk >= a.fz) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int fj = 2 * J + dy;
-      if (fj < 0 || fj >= a.fy) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int fi = 2 * I + dx;
-        if (fi < 0 || fi >= a.fx) continue;
-        const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
-        float f[8];
-        unpack8(ld16(a.x + ((int64_t)b * nfv + fi + (int64_t)a.fx * (fj + (int64_t)a.fy * fk)) * a.xpitch + 8 * q), f);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[e] += w * f[e];
-      }
-    }
+  for (int t9 = 0; t9 < 27; ++t9) {
+    float f[8];
+    unpack8(raw[t9], f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] += wt[t9] * f[e];
   }
   uint16_t* yp = a.y + v * a.ypitch + 8 * q;
   float o[8];
@@ -523,7 +531,7 @@ __global__ void k_net_restrict(XferK a) {
   *reinterpret_cast<u32x4_t*>(yp) = pack8(o);
 }
 
-__global__ void k_net_prolong(XferK a) {
+__global__ void __launch_bounds__(256) k_net_prolong(XferK a) {
   const int G = a.c >> 3;
   const int64_t ncv = (int64_t)a.cx * a.cy * a.cz, nfv = (int64_t)a.fx * a.fy * a.fz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,20 +549,32 @@ __global__ void k_net_prolong(XferK a) {
   px[0] = i >> 1; px[1] = (i + 1) >> 1; nxp = (i & 1) ? 2 : 1; wx = (i & 1) ? 0.5f : 1.f;
   py[0] = j >> 1; py[1] = (j + 1) >> 1; nyp = (j & 1) ? 2 : 1; wy = (j & 1) ? 0.5f : 1.f;
   pz[0] = k >> 1; pz[1] = (k + 1) >> 1; nzp = (k & 1) ? 2 : 1; wz = (k & 1) ? 0.5f : 1.f;
+  // the 8 parent loads are issued together; an axis with one parent reads it
+  // twice, the second time with weight 0
+  const float w = wx * wy * wz;
+  u32x4_t raw[8];
+  float wt[8];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int aa = 0; aa < 2; ++aa) {
+        const int t8 = c * 4 + bb * 2 + aa;
+        wt[t8] = (c < nzp && bb < nyp && aa < nxp) ? w : 0.f;
+        raw[t8] = ld16(a.x + ((int64_t)b * ncv + px[aa] + (int64_t)a.cx * (py[bb] + (int64_t)a.cy * pz[c])) * a.xpitch +
+                       8 * q);
+      }
   float acc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0.f;
-  const float w = wx * wy * wz;
-  for (int c = 0; c < nzp; ++c)
-    for (int bb = 0; bb < nyp; ++bb)
-      for (int aa = 0; aa < nxp; ++aa) {
-        float f[8];
-        unpack8(ld16(a.x + ((int64_t)b * ncv + px[aa] + (int64_t)a.cx * (py[bb] + (int64_t)a.cy * pz[c])) * a.xpitch +
-                     8 * q),
-                f);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc[e] += w * f[e];
-      }
+  for (int t8 = 0; t8 < 8; ++t8) {
+    float f[8];
+    unpack8(raw[t8], f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] += wt[t8] * f[e];
+  }
   uint16_t* yp = a.y + v * a.ypitch + 8 * q;
   float o[8];
   if (a.accum) unpack8(ld16(yp), o);
