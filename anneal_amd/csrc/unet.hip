@@ -969,8 +969,12 @@ constexpr int kBrickMB = 4;  // z layers per brick
 
 template <int S>
 static void launch_brick(const ConvK& a, int n16, int64_t bricks, hipStream_t s) {
-  // all output channels of a brick in one workgroup when they fit (up to 64), else 64 per workgroup
-  const int NB = n16 >= 4 ? 4 : n16;
+  // all output channels of a brick in one workgroup when they fit (up to 64), else 64 per workgroup;
+  // on grids with few bricks the output channels are split further until there are at least two
+  // workgroups per CU (the 49x25x25 level has 196 bricks: one workgroup per brick left a quarter of
+  // the CUs idle and ran each brick's whole output width on one CU)
+  int NB = n16 >= 4 ? 4 : n16;
+  while (NB > 1 && bricks * (n16 / NB) < 512) NB /= 2;
   dim3 grid(unsigned(bricks), unsigned(n16 / NB));
   if (NB == 4) hipLaunchKernelGGL((k_conv_brick<4, S, kBrickMB>), grid, dim3(256), 0, s, a);
   else if (NB == 2) hipLaunchKernelGGL((k_conv_brick<2, S, kBrickMB>), grid, dim3(256), 0, s, a);
