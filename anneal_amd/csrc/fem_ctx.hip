@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace anneal {
 
@@ -243,6 +245,10 @@ FemContext::FemContext(int nx, int ny, int nz, double h, double nu, const uint8_
     }
     Ee64_ = (double*)alloc(lv_[0].g.nelem() * sizeof(double));
     alloc_solver(nrhs_max);
+    // stiffness scales start at 0: the tile sweep below (and any apply before
+    // the first operator is set) reads defined values
+    for (auto& L : lv_) ANNEAL_HIP(hipMemset(L.Ee, 0, L.g.nelem() * sizeof(float)));
+    if (lv_[0].g.nelem() >= 4096) pick_tiles();
   } catch (...) {
     for (void* p : owned_) (void)hipFree(p);
     owned_.clear();
@@ -626,10 +632,6 @@ void FemContext::set_Ee(const double* Ee_dev) {
     d.kdiag[l] = L.kdiag;
   }
   launch_inv_diag(d, s_);
-  if (!tuned_ && g0.nelem() >= 4096) {
-    tuned_ = true;
-    autotune(-1, nullptr);  // first operator on a large grid: pick the fastest tile shape
-  }
   coarse_stale_ = true;
 }
 
@@ -694,6 +696,42 @@ void FemContext::set_level_tile(int level, int variant) {
   tuned_ = true;  // a pinned shape is kept: no sweep when the first operator is set
 }
 
+// Tile shapes chosen by the timing sweep, per device and multigrid hierarchy
+// (the level dimensions), shared by every context of the process: the
+// apply's time does not depend on the stiffness values, so a context on a
+// grid that was tuned before (the next pass of a volume-fraction search, the
+// next run of a benchmark) takes the shapes without a sweep.
+namespace {
+std::mutex g_tiles_mu;
+std::map<std::vector<int>, std::vector<int>> g_tiles;
+}  // namespace
+
+std::vector<int> FemContext::tile_key() const {
+  int dev = 0;
+  ANNEAL_HIP(hipGetDevice(&dev));
+  std::vector<int> key{dev};
+  for (const auto& L : lv_) {
+    key.push_back(L.g.nx);
+    key.push_back(L.g.ny);
+    key.push_back(L.g.nz);
+  }
+  return key;
+}
+
+void FemContext::pick_tiles() {
+  const std::vector<int> key = tile_key();
+  {
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    auto it = g_tiles.find(key);
+    if (it != g_tiles.end()) {
+      for (size_t l = 0; l < lv_.size(); ++l) lv_[l].tile = it->second[l];
+      tuned_ = true;
+      return;
+    }
+  }
+  autotune(-1, nullptr);  // stores its choice
+}
+
 int FemContext::autotune(int variant, float* us_out) {
   ANNEAL_CHECK(!is_group(), "a group uses the tile shapes of its first problem");
   const int nv = apply_variants();
@@ -739,6 +777,13 @@ int FemContext::autotune(int variant, float* us_out) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  tuned_ = true;
+  std::vector<int> tiles;
+  for (const auto& L : lv_) tiles.push_back(L.tile);
+  {
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    g_tiles[tile_key()] = tiles;
+  }
   return lv_[0].tile;
 }
 

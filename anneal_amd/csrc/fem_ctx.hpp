@@ -207,6 +207,8 @@ class FemContext {
   double* host_up_ = nullptr;
   double* host_up_dev_ = nullptr;
   bool tuned_ = false;
+  std::vector<int> tile_key() const;  // device and level dimensions
+  void pick_tiles();                  // cached shapes for this hierarchy, else the timing sweep
   bool coarse_stale_ = true, coarse_valid_ = false;
   hipStream_t cap_s_ = nullptr;             // private stream used only to record graphs
   hipEvent_t ev_ = nullptr, ev2_ = nullptr;  // mark the residual-norm copies of PCG steps

@@ -221,7 +221,10 @@ class FEM:
         _lib.call("anneal_fem_set_option", self._ctx, name.encode(), float(value))
 
     def autotune(self, variant=-1):
-        """Pick the FP32 apply tile shape (variant < 0: time all on this grid); returns (chosen, us per shape)."""
+        """Pick the FP32 apply tile shape (variant < 0: time all on this grid); returns (chosen, us per shape).
+
+        A context picks its shapes when it is created (the process's earlier choice for the same grid
+        hierarchy, else this sweep); call this to re-time or to pin one shape on every level."""
         chosen, n = _lib.ctypes.c_int(), _lib.ctypes.c_int()
         us = (_lib.ctypes.c_float * 16)()
         _lib.call("anneal_fem_autotune", self._ctx, int(variant), _lib.ctypes.byref(chosen), us,

@@ -80,17 +80,19 @@ int anneal_fem_apply_f64(void* ctx, const double* u_dev, double* y_dev, int nrhs
 int anneal_fem_solve(void* ctx, const double* F_dev, double* U_dev, int nrhs, double rtol, int maxiter, int use_x0,
                      int* iters_host);
 /* Tile-shape autotuner of the FP32 apply (no reference counterpart: the
- * reference has no kernels).  variant >= 0 selects a shape; variant == -2
+ * reference has no kernels).  A context on a grid of at least 4096 elements
+ * picks its shapes when it is created: those the process already chose for
+ * the same device and multigrid hierarchy, else by this timing sweep.
+ * variant >= 0 selects a shape; variant == -2
  * only reports the shape in use; any other negative value times all shapes
- * on this grid and keeps the fastest.  *chosen_host receives the
+ * on this grid and keeps the fastest (for later contexts too).  *chosen_host receives the
  * shape in use; us_host (may be NULL; room for 16 floats) receives *n_host per-shape times in
  * microseconds, where *n_host is the number of shapes. */
 int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host, int* n_host);
 /* Pin the FP32 apply tile shape (an id as reported by anneal_fem_level_info)
  * of multigrid level `level` (no reference counterpart): reproduces a tile
  * choice of the autotuner, e.g. under a profiler whose per-dispatch overhead
- * hides the differences the autotuner measures.  A context with a pinned
- * shape skips the sweep it otherwise runs when its first operator is set. */
+ * hides the differences the autotuner measures. */
 int anneal_fem_set_level_tile(void* ctx, int level, int variant);
 /* Solver options (no reference counterpart; the reference's solver has none):
  * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch

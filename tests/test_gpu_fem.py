@@ -202,14 +202,14 @@ def test_autotune_shapes_agree(torch):
 
 
 def test_pinned_level_tiles(torch):
-    """Tile shapes pinned per level survive the first operator (no sweep) and solve to the same answer."""
+    """Tile shapes pinned per level survive the first operator and solve to the same answer."""
     from anneal_amd import _lib
     g, active, fixed, Ee = random_problem((40, 20, 20), 11, 0.05)
     fem = _product_fem(g, active, fixed)
     nlev = len(fem.level_info()) - 1  # the coarsest level is a dense solve
     pins = [7, 8, 10, 9][:nlev]
     fem.set_level_tiles(pins)
-    K = fem.assemble(Ee[fem.act])  # 16 000 elements: an unpinned context would sweep here
+    K = fem.assemble(Ee[fem.act])
     assert [lv[3] for lv in fem.level_info()[:nlev]] == pins
     f = np.random.default_rng(3).standard_normal(fem.nfree)
     u1 = fem.solve(K, [f], rtol=1e-10)[0]
