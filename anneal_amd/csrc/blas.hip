@@ -239,31 +239,50 @@ __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, 
     MI[idx] = (i < nr && jj < bk) ? mv[q] : 0.0;
   }
   __syncthreads();
-  // Gauss-Jordan on the pivot block, ping-ponging between P and Nt (free
-  // until N is formed) so that each pivot takes one barrier
-  double *Pc = P, *Pn = Nt;
+  // Gauss-Jordan on the pivot block.  Each thread keeps its kPer entries
+  // (column pj, rows pr0 + 8 q) in registers; before pivot k the entries of
+  // row k and of column k are published in LDS (in Nt, free until N is
+  // formed; double-buffered by pivot parity, so each pivot takes one barrier),
+  // and a thread reads the pivot, one row value and the column values of its
+  // rows: 6 LDS reads per pivot instead of 12, which the two workgroups of a
+  // CU contend for.
+  static_assert(kPer == 4 && 256 / kGJB == 8, "pivot-block entries: 4 rows of one column per thread");
+  const int pj = tid % kGJB, pr0 = tid / kGJB;
+  double e[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) e[q] = P[tid + q * 256];
+  auto publish = [&](int k, double* R) {
+    double* C = R + kGJB;
+    const int qk = k >> 3;
+    if (pr0 == (k & 7)) R[pj] = qk == 0 ? e[0] : qk == 1 ? e[1] : qk == 2 ? e[2] : e[3];
+    if (pj == k) {
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) C[pr0 + 8 * q] = e[q];
+    }
+  };
+  if (bk > 0) publish(0, Nt);
+  __syncthreads();
   for (int k = 0; k < bk; ++k) {
-    const double rp = 1.0 / Pc[k * kGJB + k];
+    const double* R = Nt + (k & 1) * 2 * kGJB;  // row k, then column k
+    const double* C = R + kGJB;
+    const double rp = 1.0 / R[k];
+    const double akj = R[pj] * rp;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
-      const int idx = tid + q * 256, pi = idx / kGJB, pj = idx % kGJB;
-      const double aik = Pc[pi * kGJB + k], akj = Pc[k * kGJB + pj] * rp;
+      const int pi = pr0 + 8 * q;
+      const double aik = C[pi];
       double v;
       if (pi == k) v = pj == k ? rp : akj;
       else if (pj == k) v = -aik * rp;
-      else v = Pc[idx] - aik * akj;
-      Pn[idx] = v;
+      else v = e[q] - aik * akj;
+      e[q] = v;
     }
+    if (k + 1 < bk) publish(k + 1, Nt + ((k + 1) & 1) * 2 * kGJB);
     __syncthreads();
-    double* t = Pc;
-    Pc = Pn;
-    Pn = t;
   }
-  if (Pc != P) {
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) P[tid + q * 256] = Pc[tid + q * 256];
-    __syncthreads();
-  }
+  for (int q = 0; q < kPer; ++q) P[tid + q * 256] = e[q];
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
     const int idx = tid + q * 256, ri = idx / kGJB, pj = idx % kGJB;
