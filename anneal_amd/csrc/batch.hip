@@ -2,14 +2,15 @@
 // different sizes, each with its own block of right-hand sides).
 //
 // One inner PCG iteration of a problem is a hipGraph recorded once (per rhs
-// count) and replayed on the problem's own stream; an iteration of the batch
-// launches the graphs of the problems still iterating, forked from and joined
-// back to the launch stream, so independent problems can run concurrently and
-// finished ones cost nothing, and the host waits once per iteration for the
-// whole batch instead of once per problem.  Within a problem, a right-hand
-// side that met its tolerance keeps its device "done" flag: the kernels of
-// the iteration skip its workgroups, so its x, r and p stay untouched and it
-// costs no bandwidth while the other right-hand sides keep iterating.  The FP64 true-residual refinement and its decisions are the
+// count and host slot) and replayed on the problem's own stream, forked from
+// the launch stream once per pass and joined back at its end, so independent
+// problems run concurrently and finished ones cost nothing.  Each problem's
+// next iteration is queued before the host waits for its current one, so the
+// device does not idle while the host reads the norms.  A right-hand side
+// whose step meets its inner tolerance is marked done on the device: the
+// kernels queued after it skip its workgroups, so its x, r and p stay
+// untouched and it costs no bandwidth while the other right-hand sides keep
+// iterating.  The FP64 true-residual refinement and its decisions are the
 // same as in FemContext::solve.
 #include "batch.hpp"
 
@@ -26,16 +27,17 @@ BatchSolver::BatchSolver(const std::vector<FemContext*>& fems) : f_(fems) {
     ANNEAL_CHECK(f->s_ == s_, "the contexts of a batch must share one stream");
   }
   br_.resize(f_.size());
-  graph_.assign(f_.size(), nullptr);
+  graph_.assign(2 * f_.size(), nullptr);
   graph_nrhs_.assign(f_.size(), 0);
   graph_gen_.assign(f_.size(), 0);
   join_.resize(f_.size());
+  ev_.resize(2 * f_.size());
   for (size_t i = 0; i < f_.size(); ++i) {
     ANNEAL_HIP(hipStreamCreateWithFlags(&br_[i], hipStreamNonBlocking));
     ANNEAL_HIP(hipEventCreateWithFlags(&join_[i], hipEventDisableTiming));
+    for (int k = 0; k < 2; ++k) ANNEAL_HIP(hipEventCreateWithFlags(&ev_[2 * i + k], hipEventDisableTiming));
   }
   ANNEAL_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
-  ANNEAL_HIP(hipEventCreateWithFlags(&done_ev_, hipEventDisableTiming));
 }
 
 BatchSolver::~BatchSolver() {
@@ -44,8 +46,9 @@ BatchSolver::~BatchSolver() {
   drop_graphs();
   for (auto s : br_) (void)hipStreamDestroy(s);
   for (auto e : join_) (void)hipEventDestroy(e);
+  for (auto e : ev_)
+    if (e) (void)hipEventDestroy(e);
   if (fork_) (void)hipEventDestroy(fork_);
-  if (done_ev_) (void)hipEventDestroy(done_ev_);
 }
 
 void BatchSolver::drop_graphs() {
@@ -56,9 +59,9 @@ void BatchSolver::drop_graphs() {
     }
 }
 
-// One inner iteration of problem p (the body of FemContext::solve's inner loop
-// without the host-side speculation), issued on p's current stream.
-void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
+// One inner iteration of problem p (FemContext::solve's step, then its
+// direction), issued on p's current stream.
+void BatchSolver::issue_iteration(FemContext& f, int nrhs, int slot) {
   const GridDims& g0 = f.lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
   const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, f.lv_[0].tile);
@@ -69,8 +72,9 @@ void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
   af.skip = f.st_;
   launch_apply(af, kApply, nrhs, f.s_);
   finalize_st(f.part_, nblk_apply, nrhs, f.st_, kPQ, 1, f.s_);
-  pcg_update(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.red_ + 64, f.s_, f.st_,
-             f.host_red_dev_ + 64);
+  // |r|^2 to the host; a rhs that meets its inner tolerance is marked done on the device
+  pcg_update_converge(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.s_,
+                      f.host_red_dev_ + FemContext::rr_slot(slot), f.host_red_dev_ + FemContext::done_slot(slot));
   // the cycle's last fine smoothing kernel forms the <r, z> partials (a
   // one-level hierarchy is a dense solve only: separate dot)
   const bool fused = f.lv_.size() > 1;
@@ -82,29 +86,33 @@ void BatchSolver::issue_iteration(FemContext& f, int nrhs) {
 
 void BatchSolver::record(int p, int nrhs) {
   FemContext& f = *f_[p];
-  if (graph_[p] && graph_nrhs_[p] == nrhs && graph_gen_[p] == f.graph_generation()) return;
-  if (graph_[p]) {
-    ANNEAL_HIP(hipStreamSynchronize(br_[p]));
-    (void)hipGraphExecDestroy(graph_[p]);
-    graph_[p] = nullptr;
+  if (graph_[2 * p] && graph_nrhs_[p] == nrhs && graph_gen_[p] == f.graph_generation()) return;
+  ANNEAL_HIP(hipStreamSynchronize(br_[p]));
+  for (int slot = 0; slot < 2; ++slot) {
+    hipGraphExec_t& ge = graph_[2 * p + slot];
+    if (ge) {
+      (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
+    hipStream_t saved = f.s_;
+    hipGraph_t g = nullptr;
+    try {
+      ANNEAL_HIP(hipStreamBeginCapture(br_[p], hipStreamCaptureModeThreadLocal));
+      f.s_ = br_[p];
+      issue_iteration(f, nrhs, slot);
+      f.s_ = saved;
+      ANNEAL_HIP(hipStreamEndCapture(br_[p], &g));
+    } catch (...) {
+      f.s_ = saved;
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(br_[p], &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    ANNEAL_HIP(e);
   }
-  hipStream_t saved = f.s_;
-  hipGraph_t g = nullptr;
-  try {
-    ANNEAL_HIP(hipStreamBeginCapture(br_[p], hipStreamCaptureModeThreadLocal));
-    f.s_ = br_[p];
-    issue_iteration(f, nrhs);
-    f.s_ = saved;
-    ANNEAL_HIP(hipStreamEndCapture(br_[p], &g));
-  } catch (...) {
-    f.s_ = saved;
-    hipGraph_t dummy = nullptr;
-    (void)hipStreamEndCapture(br_[p], &dummy);
-    if (dummy) (void)hipGraphDestroy(dummy);
-    throw;
-  }
-  ANNEAL_HIP(hipGraphInstantiate(&graph_[p], g, nullptr, nullptr, 0));
-  (void)hipGraphDestroy(g);
   graph_nrhs_[p] = nrhs;
   graph_gen_[p] = f.graph_generation();
 }
@@ -200,6 +208,7 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
         st.tol2[r] = t * t;
         for (int k = 0; k < kSt; ++k) st.hst[r * kSt + k] = 0.0;
         st.hst[r * kSt + kDone] = st.done[r] ? 1.0 : 0.0;
+        st.hst[r * kSt + kTol2] = st.tol2[r];
         st.idone[r] = st.done[r];
       }
       std::copy(st.sc.begin(), st.sc.begin() + nr, f.host_up_);
@@ -220,52 +229,54 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
     }
     if (active.empty()) break;
     for (int p : active) record(p, nrhs[p]);
-    int inner_max = 0;
-    for (int p : active) inner_max = std::max(inner_max, f_[p]->inner_max);
-    for (int it = 0; it < inner_max; ++it) {
+    // Every problem iterates on its own stream from the set-up above (one fork
+    // per pass), and its iteration k + 1 is launched before the host waits for
+    // its iteration k, so the device does not idle on the host's round trip.
+    // Iteration k stores its norms in host slot k % 2.  The step that meets a
+    // rhs's tolerance marks it done on the device, so the kernels of an
+    // iteration queued after it skip that rhs; an iteration queued for a
+    // problem whose every rhs is already done costs its launches only.  No
+    // iteration is launched past inner_max or maxiter.
+    ANNEAL_HIP(hipEventRecord(fork_, s_));
+    for (int p : active) ANNEAL_HIP(hipStreamWaitEvent(br_[p], fork_, 0));
+    std::vector<int> launched(np, 0), counted(np, 0);  // iterations of this pass, per problem
+    auto launch_next = [&](int p) {
+      St& st = S[p];
+      const int k = launched[p];
+      if (!st.inner || k >= f_[p]->inner_max || st.total + (k - counted[p]) >= maxiter) return;
+      ANNEAL_HIP(hipGraphLaunch(graph_[2 * p + (k & 1)], br_[p]));
+      ANNEAL_HIP(hipEventRecord(ev_[2 * p + (k & 1)], br_[p]));
+      ++launched[p];
+    };
+    for (int p : active) launch_next(p);
+    for (int k = 0;; ++k) {
+      for (int p : active) launch_next(p);  // iteration k + 1, queued ahead
       bool any = false;
-      ANNEAL_HIP(hipEventRecord(fork_, s_));
-      for (int p : active) {
-        if (!S[p].inner || S[p].total >= maxiter) continue;
-        any = true;
-        ANNEAL_HIP(hipStreamWaitEvent(br_[p], fork_, 0));
-        ANNEAL_HIP(hipGraphLaunch(graph_[p], br_[p]));
-        ANNEAL_HIP(hipEventRecord(join_[p], br_[p]));
-        ANNEAL_HIP(hipStreamWaitEvent(s_, join_[p], 0));
-      }
-      if (!any) break;
-      ANNEAL_HIP(hipEventRecord(done_ev_, s_));
-      ANNEAL_HIP(hipEventSynchronize(done_ev_));
       for (int p : active) {
         St& st = S[p];
-        if (!st.inner) continue;
+        if (!st.inner || launched[p] <= k) continue;  // no iteration k of this pass
+        any = true;
         FemContext& f = *f_[p];
+        ANNEAL_HIP(hipEventSynchronize(ev_[2 * p + (k & 1)]));
+        ++counted[p];
         ++st.total;
-        bool iall = true, changed = false;
+        bool iall = true;
         for (int r = 0; r < nrhs[p]; ++r) {
           if (st.idone[r]) continue;
           ++st.its[r];
-          const double rr = f.host_red_[64 + r];
+          const double rr = f.host_red_[FemContext::rr_slot(k & 1) + r];
           if (!std::isfinite(rr)) st.bad = true;
-          if (rr <= st.tol2[r]) {
-            st.idone[r] = 1;
-            changed = true;
-          }
+          if (f.host_red_[FemContext::done_slot(k & 1) + r] != 0.0) st.idone[r] = 1;  // |r|^2 <= tol2
           iall = iall && st.idone[r];
         }
-        if (st.bad || iall || st.total >= maxiter || it + 1 >= f.inner_max) {
-          // this problem's inner solve ends: its graph is no longer launched
-          for (int r = 0; r < nrhs[p]; ++r) st.hst[r * kSt + kDone] = 1.0;
-          changed = true;
-          st.inner = false;
-        }
-        if (changed) {
-          for (int r = 0; r < nrhs[p]; ++r)
-            if (st.idone[r]) st.hst[r * kSt + kDone] = 1.0;
-          ANNEAL_HIP(hipMemcpy2DAsync(f.st_ + kDone, kSt * sizeof(double), st.hst.data() + kDone, kSt * sizeof(double),
-                                      sizeof(double), nrhs[p], hipMemcpyHostToDevice, s_));
-        }
+        // this problem's inner solve ends: no further iteration is launched
+        if (st.bad || iall || st.total >= maxiter || k + 1 >= f.inner_max) st.inner = false;
       }
+      if (!any) break;
+    }
+    for (int p : active) {  // the launch stream waits for every iteration queued, counted or not
+      ANNEAL_HIP(hipEventRecord(join_[p], br_[p]));
+      ANNEAL_HIP(hipStreamWaitEvent(s_, join_[p], 0));
     }
     for (int p : active) {
       St& st = S[p];

@@ -22,16 +22,19 @@ class BatchSolver {
   int size() const { return (int)f_.size(); }
 
  private:
-  void issue_iteration(FemContext& f, int nrhs);
-  void record(int p, int nrhs);  // problem p's iteration graph (recorded once per shape)
+  // one inner iteration of a problem; its step stores |r|^2 and the done
+  // flags in the context's host slot `slot`
+  void issue_iteration(FemContext& f, int nrhs, int slot);
+  void record(int p, int nrhs);  // problem p's iteration graphs, one per host slot
   void drop_graphs();
 
   std::vector<FemContext*> f_;
   hipStream_t s_ = nullptr;
   std::vector<hipStream_t> br_;  // one stream per problem
   std::vector<hipEvent_t> join_;
-  hipEvent_t fork_ = nullptr, done_ev_ = nullptr;
-  std::vector<hipGraphExec_t> graph_;  // per problem
+  std::vector<hipEvent_t> ev_;  // [2 * p + slot]: end of problem p's last iteration in that slot
+  hipEvent_t fork_ = nullptr;
+  std::vector<hipGraphExec_t> graph_;  // [2 * p + slot]
   std::vector<int> graph_nrhs_;
   std::vector<uint64_t> graph_gen_;
 };

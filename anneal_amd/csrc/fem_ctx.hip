@@ -279,7 +279,7 @@ void FemContext::alloc_solver(int nrhs_max) {
   st_ = (double*)alloc(64 * kSt * sizeof(double));
   red_ = (double*)alloc(4 * 64 * sizeof(double));
   scale_ = (double*)alloc(64 * sizeof(double));
-  ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 4 * 64 * sizeof(double), hipHostMallocDefault));
+  ANNEAL_HIP(hipHostMalloc((void**)&host_red_, 6 * 64 * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_red_dev_, host_red_, 0));
   ANNEAL_HIP(hipHostMalloc((void**)&host_up_, (64 + 64 * kSt) * sizeof(double), hipHostMallocDefault));
   ANNEAL_HIP(hipHostGetDevicePointer((void**)&host_up_dev_, host_up_, 0));
@@ -287,9 +287,9 @@ void FemContext::alloc_solver(int nrhs_max) {
   ANNEAL_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   ANNEAL_HIP(hipEventCreateWithFlags(&ev2_, hipEventDisableTiming));
   vgraph_.assign(nrhs_max + 1, nullptr);
-  sgraph_.assign(nrhs_max + 1, nullptr);
+  sgraph_.assign(2 * (nrhs_max + 1), nullptr);
   dgraph_.assign(nrhs_max + 1, nullptr);
-  cgraph_.assign(nrhs_max + 1, nullptr);
+  cgraph_.assign(2 * (nrhs_max + 1), nullptr);
 }
 
 FemContext::FemContext(const std::vector<FemContext*>& members, int nrhs_max, hipStream_t stream)
@@ -558,7 +558,7 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
   return a;
 }
 
-void FemContext::issue_step(int nrhs) {
+void FemContext::issue_step(int nrhs, int slot) {
   const GridDims& g0 = lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
   ApplyArgs<float> af = args<float>(0);
@@ -569,7 +569,8 @@ void FemContext::issue_step(int nrhs) {
   launch_apply(af, kApply, nrhs, s_);
   finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
   // |r|^2 to the host; a rhs that meets its inner tolerance is marked done on the device
-  pcg_update_converge(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, s_, host_red_dev_ + 64, host_red_dev_ + 192);
+  pcg_update_converge(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, s_, host_red_dev_ + rr_slot(slot),
+                      host_red_dev_ + done_slot(slot));
 }
 
 void FemContext::issue_direction(int nrhs) {
@@ -581,13 +582,14 @@ void FemContext::issue_direction(int nrhs) {
   pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
 }
 
-void FemContext::pcg_step(int nrhs) {
+void FemContext::pcg_step(int nrhs, int slot) {
   if (!use_graphs) {
-    issue_step(nrhs);
+    issue_step(nrhs, slot);
     return;
   }
-  if (!sgraph_[nrhs]) sgraph_[nrhs] = capture([&] { issue_step(nrhs); });
-  ANNEAL_HIP(hipGraphLaunch(sgraph_[nrhs], s_));
+  hipGraphExec_t& g = sgraph_[2 * nrhs + slot];
+  if (!g) g = capture([&] { issue_step(nrhs, slot); });
+  ANNEAL_HIP(hipGraphLaunch(g, s_));
 }
 
 void FemContext::pcg_direction(int nrhs) {
@@ -599,17 +601,18 @@ void FemContext::pcg_direction(int nrhs) {
   ANNEAL_HIP(hipGraphLaunch(dgraph_[nrhs], s_));
 }
 
-void FemContext::pcg_direction_step(int nrhs) {
+void FemContext::pcg_direction_step(int nrhs, int slot) {
   if (!use_graphs) {
     issue_direction(nrhs);
-    issue_step(nrhs);
+    issue_step(nrhs, slot);
     return;
   }
-  if (!cgraph_[nrhs]) cgraph_[nrhs] = capture([&] {
+  hipGraphExec_t& g = cgraph_[2 * nrhs + slot];
+  if (!g) g = capture([&] {
       issue_direction(nrhs);
-      issue_step(nrhs);
+      issue_step(nrhs, slot);
     });
-  ANNEAL_HIP(hipGraphLaunch(cgraph_[nrhs], s_));
+  ANNEAL_HIP(hipGraphLaunch(g, s_));
 }
 
 void FemContext::set_Ee(const double* Ee_dev) {
@@ -982,13 +985,15 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
     // The plain multigrid iteration runs as one graph: the direction of step k
     // followed by step k + 1 (one launch per iteration).  A rhs whose step meets
     // its tolerance is marked done on the device, so a step queued before the
-    // host has seen that leaves it unchanged.
+    // host has seen that leaves it unchanged; the queued step stores its |r|^2
+    // and done flags in the other host slot, not over the ones the host reads.
     const bool flexible = net_ && net_mode_ == 2;
     const bool merged = !flexible && lv_.size() > 1;
     bool step_queued = false;  // step it was queued with the previous direction
+    int slot = 0;              // host slot of the step the host waits for next
     for (int it = 0; it < inner_max && total < maxiter; ++it) {
       if (!step_queued) {
-        pcg_step(nrhs);
+        pcg_step(nrhs, slot);
         ANNEAL_HIP(hipEventRecord(ev_, s_));
       }
       step_queued = false;
@@ -1006,7 +1011,7 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
         if (merged) {
           // with the next step when the loop would run it
           if (it + 1 < inner_max && total_after < maxiter) {
-            pcg_direction_step(nrhs);
+            pcg_direction_step(nrhs, slot ^ 1);
             ANNEAL_HIP(hipEventRecord(ev2_, s_));
             step_queued = true;
           } else {
@@ -1037,16 +1042,19 @@ int FemContext::solve(const double* F, double* U, int nrhs, double rtol, int max
       for (int r = 0; r < nrhs; ++r) {
         if (idone[r]) continue;
         ++its[r];
-        const double rr = host_red_[64 + r];
+        const double rr = host_red_[rr_slot(slot) + r];
         rr2[r] = rr1[r];
         rr1[r] = rr;
         if (!std::isfinite(rr)) bad = true;
-        if (host_red_[192 + r] != 0.0) idone[r] = 1;  // the device's test |r|^2 <= tol2
+        if (host_red_[done_slot(slot) + r] != 0.0) idone[r] = 1;  // the device's test |r|^2 <= tol2
         iall = iall && idone[r];
       }
       if (bad || iall) break;
       if (!speculate) next_direction(total);
-      if (step_queued) std::swap(ev_, ev2_);
+      if (step_queued) {
+        std::swap(ev_, ev2_);
+        slot ^= 1;
+      }
     }
     if (pending) {  // no PCG step ran (maxiter reached): decide from the residual alone
       ANNEAL_HIP(hipStreamSynchronize(s_));

@@ -198,8 +198,14 @@ class FemContext {
   float *px_ = nullptr, *pr_ = nullptr, *pz_ = nullptr, *pp_ = nullptr, *pq_ = nullptr;
   double *x64_ = nullptr, *r64_ = nullptr;
   double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
-  double* host_red_ = nullptr;      // pinned
+  // pinned, 6 x 64 doubles: [0, 64) |r|^2 of the FP64 residual, [128, 192) |F|^2,
+  // and the |r|^2 and done flag of a PCG step in one of two slots (rr_slot,
+  // done_slot) that alternate between consecutive steps, so a step queued
+  // before the host has read the previous step's values writes the other slot
+  double* host_red_ = nullptr;
   double* host_red_dev_ = nullptr;  // its device alias: reductions are stored there directly
+  static int rr_slot(int slot) { return slot ? 256 : 64; }
+  static int done_slot(int slot) { return slot ? 320 : 192; }
   // pinned staging of the per-pass PCG set-up ([0, 64): scale per rhs, then the
   // 64 x kSt scalar blocks), read by one small kernel (upload) instead of two
   // copies from pageable memory; rewritten only after a host wait that follows
@@ -218,12 +224,13 @@ class FemContext {
   // <p, K p>, alpha, x and r update with |r|^2 to the host) and direction
   // (V-cycle with <r, z>, beta, p = z + beta p)
   // and both in one graph (direction of step k, then step k + 1): one graph
-  // launch per iteration once the loop runs
+  // launch per iteration once the loop runs.  The graphs holding a step are
+  // recorded per host slot: [2 * nrhs + slot]
   std::vector<hipGraphExec_t> sgraph_, dgraph_, cgraph_;
-  void pcg_step(int nrhs);
+  void pcg_step(int nrhs, int slot);
   void pcg_direction(int nrhs);
-  void pcg_direction_step(int nrhs);
-  void issue_step(int nrhs);
+  void pcg_direction_step(int nrhs, int slot);
+  void issue_step(int nrhs, int slot);
   void issue_direction(int nrhs);
   // records what issue() launches on s_ (redirected to the capture stream)
   template <typename F>
