@@ -192,9 +192,10 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
 // pivot block K of up to kGJB rows, out of place between two buffers:
 //   new_KK = P = M_KK^-1, new_KJ = P M_KJ, new_IK = -M_IK P,
 //   new_IJ = M_IJ - (M_IK P) M_KJ.
-// A workgroup owns kGJB rows x kGJC columns.  It inverts the small pivot block
-// in LDS (redundantly in every workgroup), forms N = -M_IK P for its rows (N = P
-// on the pivot rows, whose tile starts from 0 instead of M_IJ), stages the pivot
+// A workgroup owns kGJB rows x kGJC columns.  Its first wave inverts the small
+// pivot block (redundantly in every workgroup); the workgroup then forms
+// N = -M_IK P for its rows (N = P on the pivot rows, whose tile starts from 0
+// instead of M_IJ), stages the pivot
 // rows M_KJ of its columns in LDS, and updates its tile as a register-blocked
 // product: each thread owns 4 rows x 8 columns and per pivot index l reads 4
 // values of N and 8 of M_KJ for 32 FMAs (a thread per column would read one
@@ -203,6 +204,12 @@ __global__ void k_dense_assemble(const float* __restrict__ Ee, const uint8_t* __
 // loops.  All global loads are issued first from clamped, in-bounds addresses
 // (padding selected afterwards), so they are in flight during the inversion.
 constexpr int kGJB = 32, kGJC = 256, kGJR = 4, kGJM = 8;
+// Orders the LDS accesses of one wave around this point (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 static_assert(kGJB * kGJC == 256 * kGJR * kGJM, "one 4x8 register block per thread");
 __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, double* __restrict__ B, int n, int k0,
                                                   int bk) {
@@ -239,49 +246,50 @@ __global__ __launch_bounds__(256) void k_bgj_step(const double* __restrict__ A, 
     MI[idx] = (i < nr && jj < bk) ? mv[q] : 0.0;
   }
   __syncthreads();
-  // Gauss-Jordan on the pivot block.  Each thread keeps its kPer entries
-  // (column pj, rows pr0 + 8 q) in registers; before pivot k the entries of
-  // row k and of column k are published in LDS (in Nt, free until N is
-  // formed; double-buffered by pivot parity, so each pivot takes one barrier),
-  // and a thread reads the pivot, one row value and the column values of its
-  // rows: 6 LDS reads per pivot instead of 12, which the two workgroups of a
-  // CU contend for.
-  static_assert(kPer == 4 && 256 / kGJB == 8, "pivot-block entries: 4 rows of one column per thread");
-  const int pj = tid % kGJB, pr0 = tid / kGJB;
-  double e[kPer];
+  // Gauss-Jordan on the pivot block by the first wave alone, so the 32
+  // serial pivots take no workgroup barrier (the other waves wait at the one
+  // after it).  Lane l keeps column pj = l % 32 of rows 16 h + q (h = l / 32,
+  // q < 16) in registers; before pivot k the lanes holding row k and column k
+  // publish them in LDS (Nt, free until N is formed; double-buffered by pivot
+  // parity), and the wave reads them back after a wave-level fence: the LDS
+  // accesses of one wave complete in order.
+  static_assert(kGJB == 32, "pivot-block entries: one column of 16 rows per lane of one wave");
+  if (tid < 64) {
+    const int pj = tid % kGJB, h = tid / kGJB;
+    double e[16];
 #pragma unroll
-  for (int q = 0; q < kPer; ++q) e[q] = P[tid + q * 256];
-  auto publish = [&](int k, double* R) {
-    double* C = R + kGJB;
-    const int qk = k >> 3;
-    if (pr0 == (k & 7)) R[pj] = qk == 0 ? e[0] : qk == 1 ? e[1] : qk == 2 ? e[2] : e[3];
-    if (pj == k) {
+    for (int q = 0; q < 16; ++q) e[q] = P[(16 * h + q) * kGJB + pj];
+    auto publish = [&](int k, double* R) {
+      double* C = R + kGJB;
+      if (h == (k >> 4)) R[pj] = e[k & 15];
+      if (pj == k) {
 #pragma unroll
-      for (int q = 0; q < kPer; ++q) C[pr0 + 8 * q] = e[q];
+        for (int q = 0; q < 16; ++q) C[16 * h + q] = e[q];
+      }
+    };
+    publish(0, Nt);
+#pragma unroll
+    for (int k = 0; k < kGJB; ++k) {
+      if (k < bk) {
+        wave_sync();
+        const double* R = Nt + (k & 1) * 2 * kGJB;  // row k, then column k
+        const double* C = R + kGJB;
+        const double rp = 1.0 / R[k];
+        const double akj = R[pj] * rp;
+        const bool colk = pj == k;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const double aik = C[16 * h + q];
+          double v = colk ? -aik * rp : e[q] - aik * akj;
+          if (q == (k & 15) && h == (k >> 4)) v = colk ? rp : akj;  // row k
+          e[q] = v;
+        }
+        if (k + 1 < bk) publish(k + 1, Nt + ((k + 1) & 1) * 2 * kGJB);
+      }
     }
-  };
-  if (bk > 0) publish(0, Nt);
-  __syncthreads();
-  for (int k = 0; k < bk; ++k) {
-    const double* R = Nt + (k & 1) * 2 * kGJB;  // row k, then column k
-    const double* C = R + kGJB;
-    const double rp = 1.0 / R[k];
-    const double akj = R[pj] * rp;
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int pi = pr0 + 8 * q;
-      const double aik = C[pi];
-      double v;
-      if (pi == k) v = pj == k ? rp : akj;
-      else if (pj == k) v = -aik * rp;
-      else v = e[q] - aik * akj;
-      e[q] = v;
-    }
-    if (k + 1 < bk) publish(k + 1, Nt + ((k + 1) & 1) * 2 * kGJB);
-    __syncthreads();
+    for (int q = 0; q < 16; ++q) P[(16 * h + q) * kGJB + pj] = e[q];
   }
-#pragma unroll
-  for (int q = 0; q < kPer; ++q) P[tid + q * 256] = e[q];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
