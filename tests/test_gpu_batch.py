@@ -50,6 +50,50 @@ def test_batch_solve_matches_individual_solves():
         f.close()
 
 
+def test_batch_solve_with_capped_inner_solves_and_iteration_budget():
+    """Members whose inner PCG is capped at 1 and 2 iterations take many refinement passes in the batch
+    (each problem's next iteration is launched before the host reads the current one, so a cap must stop
+    the launches exactly); the batch still meets the tolerance, agrees with the members' own solves, and
+    a budget below what the solves need ends them with the not-converged error."""
+    from anneal_amd.batch import BatchSolve
+    from anneal_amd.fem import FEM, SolverError
+    rng = np.random.default_rng(5)
+    specs = [((16, 8, 8), 2), ((12, 12, 6), 1), ((20, 10, 10), 3)]
+    fems, Ks, Fs, Fred = [], [], [], []
+    for shape, nr in specs:
+        g, fixed = _clamped(shape)
+        fem = FEM(g, np.ones(g.nelem, bool), fixed, 0.3, nrhs_max=4)
+        K = fem.assemble(rng.uniform(0.01, 1.0, g.nelem))
+        fem._activate(K)
+        fem.rebuild()
+        F = [fem.reduce_load(rng.normal(size=3 * g.nnode)) for _ in range(nr)]
+        fems.append(fem)
+        Ks.append(K)
+        Fred.append(F)
+        Fs.append(fem._to_device(F))
+    fems[0].set_option("inner_max", 2)
+    fems[2].set_option("inner_max", 1)
+    nrs = [nr for _, nr in specs]
+    print("batch solve with inner caps 2, -, 1 of", [s for s, _ in specs], flush=True)
+    b = BatchSolve(fems)
+    Us = [torch.zeros_like(F) for F in Fs]
+    its = b.solve(Fs, Us, nrs, 1e-9, 2000)
+    print("batch iterations", its, flush=True)
+    for fem, K, F, U, nr in zip(fems, Ks, Fred, Us, nrs):
+        ub = fem._from_device(U, nr)
+        ui = fem.solve(K, F, rtol=1e-9)
+        for r in range(nr):
+            assert np.linalg.norm(F[r] - K @ ub[r]) <= 1e-9 * np.linalg.norm(F[r]) * 1.0001
+            assert np.linalg.norm(ub[r] - ui[r]) <= 1e-7 * np.linalg.norm(ui[r])
+    print("batch solve with a budget of 3 PCG iterations", flush=True)
+    Us = [torch.zeros_like(F) for F in Fs]
+    with pytest.raises(SolverError):
+        b.solve(Fs, Us, nrs, 1e-9, 3)
+    b.close()
+    for f in fems:
+        f.close()
+
+
 def test_batch_simp_matches_individual_runs():
     import bench
     from anneal_amd.batch import BatchSimp
