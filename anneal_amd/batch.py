@@ -4,9 +4,10 @@ No counterpart in the reference, which runs one part at a time
 (anneal/optimize.py:50-180).  The solves of all problems go through one
 ``BatchSolve``: problems of the same grid shape form a ``FemGroup`` (one block
 of right-hand sides, every kernel of the solve covering all of them in one
-launch), and the groups and remaining single problems advance in lock-step
-(anneal_batch_solve: one iteration graph per context, launched concurrently,
-one host wait per iteration for the whole batch).  The rest of each SIMP
+launch), and the groups and remaining single problems advance together
+(anneal_batch_solve: each context's PCG iteration as a recorded graph on its
+own stream, the next iteration launched before the host reads the current
+one's norms, one host round per iteration for the whole batch).  The rest of each SIMP
 iteration (filter, projection, sensitivities, OC) is the per-problem device
 code of ``SimpRun``.
 """

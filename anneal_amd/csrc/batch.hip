@@ -1,4 +1,4 @@
-// Lock-step mixed-precision PCG over several independent FEM contexts (grids of
+// Mixed-precision PCG over several independent FEM contexts together (grids of
 // different sizes, each with its own block of right-hand sides).
 //
 // One inner PCG iteration of a problem is a hipGraph recorded once (per rhs

@@ -300,10 +300,12 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
 /* ---- several independent problems solved together (no counterpart in the
  * reference, whose FEM.solve handles one grid at a time, fem.py:206-232) ----
  * A batch groups FEM contexts of any grid sizes that share one stream.  One
- * PCG iteration of a problem is a hipGraph recorded once and replayed on the
- * problem's own stream; each batch iteration launches the graphs of the
- * unconverged problems between a fork and a join on the shared stream, and the
- * host waits once per iteration for the whole batch.  F[p], U[p]: FP64 [nrhs[p]][3][ldn_p];
+ * PCG iteration of a problem is a hipGraph recorded once (per host slot of its
+ * norms) and replayed on the problem's own stream, forked from the shared
+ * stream once per refinement pass and joined back at its end; each problem's
+ * next iteration is launched before the host waits for its current one, and
+ * the host reads the norms of every problem's iteration in one round.
+ * F[p], U[p]: FP64 [nrhs[p]][3][ldn_p];
  * status[p] as anneal_fem_solve (0, 3 not converged, 4 non-finite);
  * iters[p * 64 + r] the PCG iterations of rhs r.  Plain multigrid only. */
 int anneal_batch_create(void** batch, void* const* fems, int nprob);
