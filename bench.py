@@ -103,7 +103,7 @@ def large_part(shape=LARGE_SHAPE, seed=LARGE_SEED, keep_frac=0.05, solid_frac=0.
 
 
 BATCH_SEED = 3
-BENCH_WARM = 2
+BENCH_WARM = 3
 BATCH_SHAPES = ((64, 64, 64), (96, 48, 48), (128, 64, 64))
 
 
@@ -169,8 +169,9 @@ def batch_bench(steps, seq_steps, log):
     from anneal_amd.optimize import SimpRun
     probs = batch_problems()
     b = BatchSimp(probs)
-    # two untimed iterations per leg: the first records the graphs and solves from zero, the second
-    # allocates the extrapolation buffers of the warm start (each leg's allocations are then cached)
+    # three untimed iterations per leg: the first records the graphs and solves from zero, the second
+    # allocates the buffers of the extrapolated start, the third is the first extrapolation (its
+    # PyTorch kernel is loaded on first use, ≈ 25 ms); each leg's allocations are then cached
     for _ in range(BENCH_WARM):
         b.step()
     torch.cuda.synchronize()
