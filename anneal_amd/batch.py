@@ -5,9 +5,9 @@ No counterpart in the reference, which runs one part at a time
 ``BatchSolve``: problems of the same grid shape form a ``FemGroup`` (one block
 of right-hand sides, every kernel of the solve covering all of them in one
 launch), and the groups and remaining single problems advance together
-(anneal_batch_solve: each context's PCG iteration as a recorded graph on its
-own stream, the next iteration launched before the host reads the current
-one's norms, one host round per iteration for the whole batch).  The rest of each SIMP
+(anneal_batch_solve: each context's PCG iteration on its own stream, the next
+iteration launched before the host reads the current one's norms, one host
+round per iteration for the whole batch).  The rest of each SIMP
 iteration (filter, projection, sensitivities, OC) is the per-problem device
 code of ``SimpRun``.
 """

@@ -1,9 +1,10 @@
 // Mixed-precision PCG over several independent FEM contexts together (grids of
 // different sizes, each with its own block of right-hand sides).
 //
-// One inner PCG iteration of a problem is a hipGraph recorded once (per rhs
-// count and host slot) and replayed on the problem's own stream, forked from
-// the launch stream once per pass and joined back at its end, so independent
+// One inner PCG iteration of a problem is launched on the problem's own
+// stream (or replayed from a hipGraph recorded once per rhs count and host
+// slot, when the context's graph option is on), forked from the launch
+// stream once per pass and joined back at its end, so independent
 // problems run concurrently and finished ones cost nothing.  Each problem's
 // next iteration is queued before the host waits for its current one, so the
 // device does not idle while the host reads the norms.  A right-hand side
@@ -228,7 +229,8 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       active.push_back(p);
     }
     if (active.empty()) break;
-    for (int p : active) record(p, nrhs[p]);
+    for (int p : active)
+      if (f_[p]->use_graphs) record(p, nrhs[p]);
     // Every problem iterates on its own stream from the set-up above (one fork
     // per pass), and its iteration k + 1 is launched before the host waits for
     // its iteration k, so the device does not idle on the host's round trip.
@@ -244,7 +246,20 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       St& st = S[p];
       const int k = launched[p];
       if (!st.inner || k >= f_[p]->inner_max || st.total + (k - counted[p]) >= maxiter) return;
-      ANNEAL_HIP(hipGraphLaunch(graph_[2 * p + (k & 1)], br_[p]));
+      FemContext& f = *f_[p];
+      if (f.use_graphs) {
+        ANNEAL_HIP(hipGraphLaunch(graph_[2 * p + (k & 1)], br_[p]));
+      } else {  // the same kernels launched one by one on the problem's stream
+        hipStream_t saved = f.s_;
+        f.s_ = br_[p];
+        try {
+          issue_iteration(f, nrhs[p], k & 1);
+        } catch (...) {
+          f.s_ = saved;
+          throw;
+        }
+        f.s_ = saved;
+      }
       ANNEAL_HIP(hipEventRecord(ev_[2 * p + (k & 1)], br_[p]));
       ++launched[p];
     };

@@ -102,7 +102,11 @@ class FemContext {
   void vcycle_pcg(int nrhs);
   // Same on other buffers (x = M b), one graph per (nrhs, b, x).
   void vcycle_graph(const float* b, float* x, int nrhs);
-  bool use_graphs = true;
+  // Replay the V-cycle and the PCG iteration units from recorded hipGraphs.
+  // Off by default: launching the same kernels one by one measured 1-2 %
+  // faster at every grid size (the host stays ahead through the speculation,
+  // and a graph launch left the device idle at each graph boundary).
+  bool use_graphs = false;
 
   // Learned stage (a U-Net N on the node grid of multigrid level `level`,
   // z1 = gamma P N(R r) with the energy-optimal gamma = <r, z1> / <z1, K z1>):

@@ -95,8 +95,9 @@ int anneal_fem_autotune(void* ctx, int variant, int* chosen_host, float* us_host
  * hides the differences the autotuner measures. */
 int anneal_fem_set_level_tile(void* ctx, int level, int variant);
 /* Solver options (no reference counterpart; the reference's solver has none):
- * "graphs" (1 = replay the V-cycle from a captured hipGraph, 0 = launch
- * eagerly), "smoother_degree" (Chebyshev degree per side on the fine level,
+ * "graphs" (1 = replay the V-cycle and each PCG iteration from captured
+ * hipGraphs, 0 = launch the same kernels one by one; default 0, which
+ * measured 1-2 % faster), "smoother_degree" (Chebyshev degree per side on the fine level,
  * >= 1; default 3), "smoother_degree_coarse" (levels >= 1; default 3),
  * "smoother_degree_deep" (levels >= 2; 0 = as the coarse levels),
  * "deep_escalate_iters" (a solve averaging more PCG iterations per rhs than
@@ -300,8 +301,9 @@ int anneal_fem_learned_pair(void* fem, const double* F_dev, const double* U_dev,
 /* ---- several independent problems solved together (no counterpart in the
  * reference, whose FEM.solve handles one grid at a time, fem.py:206-232) ----
  * A batch groups FEM contexts of any grid sizes that share one stream.  One
- * PCG iteration of a problem is a hipGraph recorded once (per host slot of its
- * norms) and replayed on the problem's own stream, forked from the shared
+ * PCG iteration of a problem runs on the problem's own stream (replayed from a
+ * hipGraph recorded once per host slot of its norms when the member's "graphs"
+ * option is on), forked from the shared
  * stream once per refinement pass and joined back at its end; each problem's
  * next iteration is launched before the host waits for its current one, and
  * the host reads the norms of every problem's iteration in one round.

@@ -50,6 +50,43 @@ def test_batch_solve_matches_individual_solves():
         f.close()
 
 
+def test_batch_graphs_match_direct_launches():
+    """The batch with each member's iteration replayed from recorded graphs (graphs=1) gives the same bits
+    as the same kernels launched one by one (the default), solve after solve."""
+    from anneal_amd.batch import BatchSolve
+    from anneal_amd.fem import FEM
+    rng = np.random.default_rng(7)
+    specs = [((16, 8, 8), 2), ((20, 10, 10), 3), ((8, 8, 8), 1)]
+    fems, Fs = [], []
+    for shape, nr in specs:
+        g, fixed = _clamped(shape)
+        fem = FEM(g, np.ones(g.nelem, bool), fixed, 0.3, nrhs_max=4)
+        fem._activate(fem.assemble(rng.uniform(0.01, 1.0, g.nelem)))
+        fem.rebuild()
+        fems.append(fem)
+        Fs.append(fem._to_device([fem.reduce_load(rng.normal(size=3 * g.nnode)) for _ in range(nr)]))
+    nrs = [nr for _, nr in specs]
+    b = BatchSolve(fems)
+    results = {}
+    for graphs in (0, 1, 0, 1):
+        for f in fems:
+            f.set_option("graphs", graphs)
+        print("batch solve with graphs =", graphs, flush=True)
+        Us = [torch.zeros_like(F) for F in Fs]
+        its = b.solve(Fs, Us, nrs, 1e-9, 2000)
+        torch.cuda.synchronize()
+        got = (its, [U.cpu() for U in Us])
+        if graphs in results:
+            assert got[0] == results[graphs][0]
+            assert all(torch.equal(x, y) for x, y in zip(got[1], results[graphs][1]))
+        results[graphs] = got
+    assert results[0][0] == results[1][0]
+    assert all(torch.equal(x, y) for x, y in zip(results[0][1], results[1][1]))
+    b.close()
+    for f in fems:
+        f.close()
+
+
 def test_batch_solve_with_capped_inner_solves_and_iteration_budget():
     """Members whose inner PCG is capped at 1 and 2 iterations take many refinement passes in the batch
     (each problem's next iteration is launched before the host reads the current one, so a cap must stop
