@@ -264,6 +264,7 @@ void FemContext::alloc_solver(int nrhs_max) {
   pr_ = (float*)alloc(v32);
   pz_ = (float*)alloc(v32);
   pp_ = (float*)alloc(v32);
+  pp2_ = (float*)alloc(v32);
   pq_ = (float*)alloc(v32);
   x64_ = (double*)alloc(v64);
   r64_ = (double*)alloc(v64);
@@ -271,6 +272,7 @@ void FemContext::alloc_solver(int nrhs_max) {
   ANNEAL_HIP(hipMemset(pr_, 0, v32));
   ANNEAL_HIP(hipMemset(pz_, 0, v32));
   ANNEAL_HIP(hipMemset(pp_, 0, v32));
+  ANNEAL_HIP(hipMemset(pp2_, 0, v32));
   ANNEAL_HIP(hipMemset(pq_, 0, v32));
   ANNEAL_HIP(hipMemset(r64_, 0, v64));
   int nblk = kDotBlocks;
@@ -558,37 +560,44 @@ ApplyArgs<double> FemContext::args<double>(int l) const {
   return a;
 }
 
-void FemContext::issue_step(int nrhs, int slot) {
+void FemContext::issue_step(int nrhs, int slot, bool fused) {
   const GridDims& g0 = lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
   ApplyArgs<float> af = args<float>(0);
-  af.u = pp_;
   af.y = pq_;
   af.dot_part = part_;
   af.skip = st_;
-  launch_apply(af, kApply, nrhs, s_);
+  if (fused) {
+    // p = z + beta p_old staged by the apply itself (the apply is bound by its
+    // arithmetic, so the extra array costs less than a separate vector pass)
+    af.u = pdir(slot ^ 1);
+    af.b = pz_;
+    af.pout = pdir(slot);
+    launch_apply(af, kApplyDir, nrhs, s_);
+  } else {
+    af.u = pdir(slot);
+    launch_apply(af, kApply, nrhs, s_);
+  }
   finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kPQ, 1, s_);  // pq, alpha
   // |r|^2 to the host; a rhs that meets its inner tolerance is marked done on the device
-  pcg_update_converge(px_, pr_, pp_, pq_, st_, n, ld, nrhs, part_, s_, host_red_dev_ + rr_slot(slot),
+  pcg_update_converge(px_, pr_, pdir(slot), pq_, st_, n, ld, nrhs, part_, s_, host_red_dev_ + rr_slot(slot),
                       host_red_dev_ + done_slot(slot));
 }
 
 void FemContext::issue_direction(int nrhs) {
   const GridDims& g0 = lv_[0].g;
-  const int64_t n = g0.nnode(), ld = g0.ldn();
   // the cycle's last fine-level smoothing kernel leaves the <r, z> partials on the apply grid
   vcycle(0, pr_, pz_, nrhs, part_, st_);
   finalize_st(part_, apply_blocks(g0.nx, g0.ny, g0.nz, lv_[0].tile), nrhs, st_, kRZnew, 2, s_);  // rz, beta
-  pcg_dir(pp_, pz_, st_, n, ld, nrhs, 0, s_);
 }
 
 void FemContext::pcg_step(int nrhs, int slot) {
   if (!use_graphs) {
-    issue_step(nrhs, slot);
+    issue_step(nrhs, slot, false);
     return;
   }
   hipGraphExec_t& g = sgraph_[2 * nrhs + slot];
-  if (!g) g = capture([&] { issue_step(nrhs, slot); });
+  if (!g) g = capture([&] { issue_step(nrhs, slot, false); });
   ANNEAL_HIP(hipGraphLaunch(g, s_));
 }
 
@@ -604,13 +613,13 @@ void FemContext::pcg_direction(int nrhs) {
 void FemContext::pcg_direction_step(int nrhs, int slot) {
   if (!use_graphs) {
     issue_direction(nrhs);
-    issue_step(nrhs, slot);
+    issue_step(nrhs, slot, true);
     return;
   }
   hipGraphExec_t& g = cgraph_[2 * nrhs + slot];
   if (!g) g = capture([&] {
       issue_direction(nrhs);
-      issue_step(nrhs, slot);
+      issue_step(nrhs, slot, true);
     });
   ANNEAL_HIP(hipGraphLaunch(g, s_));
 }

@@ -200,6 +200,12 @@ class FemContext {
   double* Awork_ = nullptr;
   // PCG buffers
   float *px_ = nullptr, *pr_ = nullptr, *pz_ = nullptr, *pp_ = nullptr, *pq_ = nullptr;
+  // The merged PCG iteration forms the direction inside the K p apply (apply
+  // mode kApplyDir), which reads the old direction and writes the new one: the
+  // two live in pp_ and pp2_, in turn by host slot (step with slot s writes
+  // pdir(s)); a pass starts with its direction in pp_ and a step of slot 0.
+  float* pp2_ = nullptr;
+  float* pdir(int slot) const { return slot ? pp2_ : pp_; }
   double *x64_ = nullptr, *r64_ = nullptr;
   double *part_ = nullptr, *st_ = nullptr, *red_ = nullptr, *scale_ = nullptr;
   // pinned, 6 x 64 doubles: [0, 64) |r|^2 of the FP64 residual, [128, 192) |F|^2,
@@ -226,16 +232,18 @@ class FemContext {
   std::map<std::tuple<int, const void*, void*>, hipGraphExec_t> vgraph_bx_;
   // the two halves of a plain PCG iteration, per rhs count: step (K p with
   // <p, K p>, alpha, x and r update with |r|^2 to the host) and direction
-  // (V-cycle with <r, z>, beta, p = z + beta p)
-  // and both in one graph (direction of step k, then step k + 1): one graph
-  // launch per iteration once the loop runs.  The graphs holding a step are
-  // recorded per host slot: [2 * nrhs + slot]
+  // (V-cycle with <r, z>, beta), and both as one unit (direction of step k,
+  // then step k + 1, whose apply forms p = z + beta p): one launch per
+  // iteration once the loop runs.  With graphs on, the graphs holding a step
+  // are recorded per host slot: [2 * nrhs + slot]
   std::vector<hipGraphExec_t> sgraph_, dgraph_, cgraph_;
   void pcg_step(int nrhs, int slot);
   void pcg_direction(int nrhs);
   void pcg_direction_step(int nrhs, int slot);
-  void issue_step(int nrhs, int slot);
-  void issue_direction(int nrhs);
+  // fused: p = z + beta p formed by the apply (after issue_direction), else p
+  // is already in pdir(slot)
+  void issue_step(int nrhs, int slot, bool fused);
+  void issue_direction(int nrhs);  // V-cycle, <r, z>, beta (the direction itself is the next step's)
   // records what issue() launches on s_ (redirected to the capture stream)
   template <typename F>
   hipGraphExec_t capture(F&& issue);

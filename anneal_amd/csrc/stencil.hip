@@ -94,6 +94,9 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   T* y = a.y + (int64_t)3 * r * ldn;  // not restrict: kCheb may write over prev
   const T* __restrict__ bb = (MODE != kApply) ? a.b + (int64_t)3 * r * ldn : nullptr;
   const T* pr = (MODE == kCheb && a.prev) ? a.prev + (int64_t)3 * r * ldn : nullptr;
+  T* po = MODE == kApplyDir ? a.pout + (int64_t)3 * r * ldn : nullptr;  // the new direction
+  // kApplyDir: the staged u is the direction p = b + beta u_old (as the PCG's separate direction update)
+  const T beta = MODE == kApplyDir ? T(a.skip[r * kSt + kBeta]) : T(0);
   // operator of this right-hand side (one problem of a group, or the only one)
   const T* opEe = a.Ee;
   const uint8_t* opmask = a.mask;
@@ -139,15 +142,19 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   const __amdgpu_buffer_rsrc_t ru = rsrc(u, vbytes), ry = rsrc(y, vbytes);
   const __amdgpu_buffer_rsrc_t rb = rsrc(bb, MODE != kApply ? vbytes : 0u);
   const __amdgpu_buffer_rsrc_t rp = rsrc(pr ? pr : u, pr ? vbytes : 0u);  // empty: the loads return 0
+  const __amdgpu_buffer_rsrc_t rpo = rsrc(po, po ? vbytes : 0u);
   const __amdgpu_buffer_rsrc_t rm = rsrc(opmask, nnode);
   const __amdgpu_buffer_rsrc_t rd = rsrc(opdinv, (MODE == kJacobi || MODE == kCheb || MODE == kCheb0) ? nnode * 4u : 0u);
 
   // Node planes are prefetched two steps ahead (two register slots), so the
   // load latency of a plane is covered by two steps of compute.
-  // (the fused first smoothing steps stage mask, D^-1 and b per node, so they
-  // prefetch one plane ahead to stay within 96 VGPRs)
-  constexpr int kDepth = (MODE == kCheb0) ? 1 : 2;
+  // (the fused first smoothing steps stage mask, D^-1 and b per node, and the
+  // direction mode two arrays, so they prefetch one plane ahead to stay within
+  // 96 VGPRs)
+  constexpr int kDepth = (MODE == kCheb0 || MODE == kApplyDir) ? 1 : 2;
+  constexpr int kDir = MODE == kApplyDir ? 1 : 0;  // a second staged array (b) for the direction update
   T pre[kDepth][NLD][3];
+  T preb[kDir ? kDepth : 1][NLD][3];
   auto fetch = [&](int p, int slot) {
     const uint32_t o = plane_off(p);
     if (MODE == kCheb0) {
@@ -170,6 +177,10 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
         pre[slot][k][0] = bload<T>(ru, lvo[k], o);
         pre[slot][k][1] = bload<T>(ru, lvo[k], o + cstep);
         pre[slot][k][2] = bload<T>(ru, lvo[k], o + 2 * cstep);
+        if (kDir) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) preb[kDir ? slot : 0][k][c] = bload<T>(rb, lvo[k], o + c * cstep);
+        }
       }
     }
   };
@@ -200,7 +211,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
       E.bv[c] = T(0);
       E.pv[c] = T(0);
     }
-    if (MODE != kApply) {
+    if (MODE != kApply && MODE != kApplyDir) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) E.bv[c] = bload<T>(rb, ovo, o + c * cstep);
     }
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
   // would push them past 96 VGPRs (4 instead of 5 waves per SIMD), so they
   // fetch at the top of the step that uses them (the step's compute covers
   // the latency).
-  constexpr bool kEpiAhead = (MODE == kApply || MODE == kResid);
+  constexpr bool kEpiAhead = (MODE == kApply || MODE == kResid || MODE == kApplyDir);
   Epi ep[kEpiAhead ? 2 : 1];
   if (kEpiAhead) fetch_epi(Z0, ep[0]);  // plane finished at step 2
 
@@ -244,7 +255,10 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
       const int idx = tid + k * NT;
       if (idx < NP) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) buf[c * NP + idx] = pre[s % kDepth][k][c];
+        for (int c = 0; c < 3; ++c) {
+          if (kDir) buf[c * NP + idx] = preb[kDir ? s % kDepth : 0][k][c] + beta * pre[s % kDepth][k][c];
+          else buf[c * NP + idx] = pre[s % kDepth][k][c];
+        }
       }
     }
     if (s + kDepth <= ZC + 1) fetch(p + kDepth, s % kDepth);  // in flight for the next kDepth steps
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
                         sp[4 * c + 0][tid + TX + 1];
             const T uc = prevbuf[c * NP + o];
             T out;
-            if (MODE == kApply) out = v;
+            if (MODE == kApply || MODE == kApplyDir) out = v;
             else if (MODE == kResid) out = E.bv[c] - v;
             else if (MODE == kJacobi) out = uc + a.omega * T(E.dv) * (E.bv[c] - v);
             else if (MODE == kCheb0) out = uc + a.c1 * uc + a.c2 * T(E.dv) * (E.bv[c] - v);
@@ -329,7 +343,8 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
             }
             if (!((E.mk >> c) & 1)) out = T(0);
             bstore<T>(out, ry, outp ? ovo : kOob, lb + c * cstep);  // dropped for lanes without a node
-            if (MODE == kApply) {
+            if (MODE == kApplyDir) bstore<T>(uc, rpo, outp ? ovo : kOob, lb + c * cstep);  // the direction
+            if (MODE == kApply || MODE == kApplyDir) {
               if (a.dot_part && ncol) dot += (double)uc * (double)out;
             } else if (MODE == kCheb || MODE == kJacobi) {
               // <b, y>: the PCG's <r, z> when this step ends the fine-level V-cycle
@@ -351,7 +366,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 5 : 1) void k_apply(ApplyA
 #pragma unroll
       for (int m = 0; m < 4; ++m) Fprev[c][m] = Fcur[c][m];
   }
-  if ((MODE == kApply || MODE == kCheb || MODE == kJacobi || MODE == kResid) && a.dot_part) {
+  if ((MODE == kApply || MODE == kApplyDir || MODE == kCheb || MODE == kJacobi || MODE == kResid) && a.dot_part) {
     // fixed-order block reduction
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((tid & 63) == 0) sred[tid >> 6] = dot;
@@ -587,6 +602,14 @@ void launch_apply(const ApplyArgs<T>& a, int mode, int nrhs, hipStream_t s) {
     case kJacobi: launch_apply_mode<T, kJacobi>(a, nrhs, v, s); break;
     case kCheb: launch_apply_mode<T, kCheb>(a, nrhs, v, s); break;
     case kCheb0: launch_apply_mode<T, kCheb0>(a, nrhs, v, s); break;
+    case kApplyDir:
+      if constexpr (sizeof(T) == 4) {
+        ANNEAL_CHECK(a.pout != nullptr && a.b != nullptr && a.skip != nullptr && (const void*)a.pout != (const void*)a.u,
+                     "the direction apply needs z (b), the PCG scalars (skip) and a separate output direction");
+        launch_apply_mode<T, kApplyDir>(a, nrhs, v, s);
+        break;
+      }
+      throw std::invalid_argument("the direction apply is FP32 only");
     default: throw std::invalid_argument("bad apply mode");
   }
 }

@@ -16,6 +16,8 @@ enum ApplyMode : int {
   kJacobi = 2,  // y = M (u + omega * dinv * (b - K u))
   kCheb = 3,    // y = M (u + c1 (u - prev) + c2 * dinv * (b - K u)); prev may be null (0) or alias y
   kCheb0 = 4,   // kCheb with u = M omega dinv b formed on the fly and prev = 0 (first two smoothing steps fused)
+  kApplyDir = 5,  // the PCG direction and K times it: p = b + beta u (beta from the PCG scalar block) staged
+                  // on the fly and stored to pout at the owned nodes, y = M K p (FP32 only)
 };
 
 template <typename T>
@@ -41,6 +43,7 @@ struct ApplyArgs {
   const OpEntry* ops;  // per-problem operators (device table) or null: Ee, mask, dinv above serve every rhs
   int rpo;             // right-hand sides per problem when ops is set
   const double* skip;  // PCG scalar block: rhs with the done flag set are skipped (null: none)
+  T* pout;             // kApplyDir: the new direction [nrhs][3][ldn] (not u: other tiles still read u)
 };
 
 // Face (2-D) Walsh transform of 4 corner values a=(i,j) b=(i+1,j) c=(i,j+1) d=(i+1,j+1)
