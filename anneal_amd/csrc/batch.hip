@@ -28,7 +28,7 @@ BatchSolver::BatchSolver(const std::vector<FemContext*>& fems) : f_(fems) {
     ANNEAL_CHECK(f->s_ == s_, "the contexts of a batch must share one stream");
   }
   br_.resize(f_.size());
-  graph_.assign(2 * f_.size(), nullptr);
+  graph_.assign(3 * f_.size(), nullptr);
   graph_nrhs_.assign(f_.size(), 0);
   graph_gen_.assign(f_.size(), 0);
   join_.resize(f_.size());
@@ -60,37 +60,53 @@ void BatchSolver::drop_graphs() {
     }
 }
 
-// One inner iteration of problem p (FemContext::solve's step, then its
-// direction), issued on p's current stream.
-void BatchSolver::issue_iteration(FemContext& f, int nrhs, int slot) {
+// Unit k of a problem's inner PCG, issued on its current stream: unit 0 is
+// the pass's first step (the set-up formed its direction); a later unit is the
+// direction of step k - 1 (V-cycle with <r, z>, beta) followed by step k, whose
+// apply forms p = z + beta p (FemContext::solve's merged iteration).  Step k
+// stores its norms in host slot k % 2 and its direction in pdir(k % 2).
+void BatchSolver::issue_unit(FemContext& f, int nrhs, int k) {
   const GridDims& g0 = f.lv_[0].g;
   const int64_t n = g0.nnode(), ld = g0.ldn();
   const int nblk_apply = apply_blocks(g0.nx, g0.ny, g0.nz, f.lv_[0].tile);
+  const int slot = k & 1;
+  if (k > 0) {
+    // the cycle's last fine smoothing kernel forms the <r, z> partials (a
+    // one-level hierarchy is a dense solve only: separate dot)
+    const bool fused = f.lv_.size() > 1;
+    f.vcycle(0, f.pr_, f.pz_, nrhs, fused ? f.part_ : nullptr, f.st_);
+    if (!fused) dot_partial(f.pr_, f.pz_, n, ld, nrhs, f.part_, f.s_, f.st_);
+    finalize_st(f.part_, fused ? nblk_apply : kDotBlocks, nrhs, f.st_, kRZnew, 2, f.s_);
+  }
   ApplyArgs<float> af = f.args<float>(0);
-  af.u = f.pp_;
   af.y = f.pq_;
   af.dot_part = f.part_;
   af.skip = f.st_;
-  launch_apply(af, kApply, nrhs, f.s_);
+  if (k > 0) {
+    af.u = f.pdir(slot ^ 1);
+    af.b = f.pz_;
+    af.pout = f.pdir(slot);
+    launch_apply(af, kApplyDir, nrhs, f.s_);
+  } else {
+    af.u = f.pdir(0);
+    launch_apply(af, kApply, nrhs, f.s_);
+  }
   finalize_st(f.part_, nblk_apply, nrhs, f.st_, kPQ, 1, f.s_);
   // |r|^2 to the host; a rhs that meets its inner tolerance is marked done on the device
-  pcg_update_converge(f.px_, f.pr_, f.pp_, f.pq_, f.st_, n, ld, nrhs, f.part_, f.s_,
+  pcg_update_converge(f.px_, f.pr_, f.pdir(slot), f.pq_, f.st_, n, ld, nrhs, f.part_, f.s_,
                       f.host_red_dev_ + FemContext::rr_slot(slot), f.host_red_dev_ + FemContext::done_slot(slot));
-  // the cycle's last fine smoothing kernel forms the <r, z> partials (a
-  // one-level hierarchy is a dense solve only: separate dot)
-  const bool fused = f.lv_.size() > 1;
-  f.vcycle(0, f.pr_, f.pz_, nrhs, fused ? f.part_ : nullptr, f.st_);
-  if (!fused) dot_partial(f.pr_, f.pz_, n, ld, nrhs, f.part_, f.s_, f.st_);
-  finalize_st(f.part_, fused ? nblk_apply : kDotBlocks, nrhs, f.st_, kRZnew, 2, f.s_);
-  pcg_dir(f.pp_, f.pz_, f.st_, n, ld, nrhs, 0, f.s_);
 }
+
+// The recorded variants of a problem's units: 0 the first step, 1 + s a later
+// unit whose step has host slot s.
+static int unit_variant(int k) { return k == 0 ? 0 : 1 + (k & 1); }
 
 void BatchSolver::record(int p, int nrhs) {
   FemContext& f = *f_[p];
-  if (graph_[2 * p] && graph_nrhs_[p] == nrhs && graph_gen_[p] == f.graph_generation()) return;
+  if (graph_[3 * p] && graph_nrhs_[p] == nrhs && graph_gen_[p] == f.graph_generation()) return;
   ANNEAL_HIP(hipStreamSynchronize(br_[p]));
-  for (int slot = 0; slot < 2; ++slot) {
-    hipGraphExec_t& ge = graph_[2 * p + slot];
+  for (int k : {0, 1, 2}) {  // one unit of each variant
+    hipGraphExec_t& ge = graph_[3 * p + unit_variant(k)];
     if (ge) {
       (void)hipGraphExecDestroy(ge);
       ge = nullptr;
@@ -100,7 +116,7 @@ void BatchSolver::record(int p, int nrhs) {
     try {
       ANNEAL_HIP(hipStreamBeginCapture(br_[p], hipStreamCaptureModeThreadLocal));
       f.s_ = br_[p];
-      issue_iteration(f, nrhs, slot);
+      issue_unit(f, nrhs, k);
       f.s_ = saved;
       ANNEAL_HIP(hipStreamEndCapture(br_[p], &g));
     } catch (...) {
@@ -248,12 +264,12 @@ void BatchSolver::solve(const std::vector<const double*>& F, const std::vector<d
       if (!st.inner || k >= f_[p]->inner_max || st.total + (k - counted[p]) >= maxiter) return;
       FemContext& f = *f_[p];
       if (f.use_graphs) {
-        ANNEAL_HIP(hipGraphLaunch(graph_[2 * p + (k & 1)], br_[p]));
+        ANNEAL_HIP(hipGraphLaunch(graph_[3 * p + unit_variant(k)], br_[p]));
       } else {  // the same kernels launched one by one on the problem's stream
         hipStream_t saved = f.s_;
         f.s_ = br_[p];
         try {
-          issue_iteration(f, nrhs[p], k & 1);
+          issue_unit(f, nrhs[p], k);
         } catch (...) {
           f.s_ = saved;
           throw;

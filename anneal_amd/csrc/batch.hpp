@@ -22,10 +22,9 @@ class BatchSolver {
   int size() const { return (int)f_.size(); }
 
  private:
-  // one inner iteration of a problem; its step stores |r|^2 and the done
-  // flags in the context's host slot `slot`
-  void issue_iteration(FemContext& f, int nrhs, int slot);
-  void record(int p, int nrhs);  // problem p's iteration graphs, one per host slot
+  // unit k of a problem's inner PCG (direction of step k - 1, then step k)
+  void issue_unit(FemContext& f, int nrhs, int k);
+  void record(int p, int nrhs);  // problem p's unit graphs: the first step, and a later unit per host slot
   void drop_graphs();
 
   std::vector<FemContext*> f_;
@@ -34,7 +33,7 @@ class BatchSolver {
   std::vector<hipEvent_t> join_;
   std::vector<hipEvent_t> ev_;  // [2 * p + slot]: end of problem p's last iteration in that slot
   hipEvent_t fork_ = nullptr;
-  std::vector<hipGraphExec_t> graph_;  // [2 * p + slot]
+  std::vector<hipGraphExec_t> graph_;  // [3 * p + variant]
   std::vector<int> graph_nrhs_;
   std::vector<uint64_t> graph_gen_;
 };
